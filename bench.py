@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of bit-exact IWPP reconstruction by dilation on one MI355X.
+
+A step = one full reconstruction of the workload's marker under its mask
+(marker reset from a resident copy, then iwpp_reconstruct_ex), inputs already
+in HBM.  value = image pixels / step time, in Mpixel/s.  The output of the last
+timed step is compared bit for bit with the CPU oracle.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--queue uq]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "iwpp-mp_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "Mpixel/s bit-exact reconstruction, learned-schedule vs plain IWPP, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--queue", default="uq", choices=["mutex", "dq", "uq", "auto"])
+    ap.add_argument("--schedule", default="plain", choices=["plain", "learned"])
+    ap.add_argument("--sweeps", type=int, default=-1, help="sweep rounds (-1: schedule)")
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-iters", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the reconstruction "
+              "path is single-GPU and no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import iwpp
+    from iwpp import workloads as W
+
+    if not torch.cuda.is_available():
+        log("no HIP device")
+        sys.exit(1)
+    torch.cuda.set_device(0)
+
+    t = time.time()
+    case = W.CONFIGS[args.config]()
+    log(f"generated {case.name} {case.mask.shape} {case.mask.dtype} in {time.time() - t:.1f}s")
+    px = case.pixels
+    tsize = case.mask.itemsize
+    mask = iwpp.DeviceImage.from_numpy(case.mask)
+    marker0 = iwpp.DeviceImage.from_numpy(case.marker)
+    work = iwpp.DeviceImage(marker0.h, marker0.w, marker0.dtype, marker0.pitch)
+    ws = iwpp.Workspace(px, marker0.dtype)
+    opt = iwpp.default_options(sweep_rounds=args.sweeps)
+
+    def step():
+        work.buf.copy_(marker0.buf)
+        return iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue,
+                                args.schedule, opt, ws)
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"warmup {i}: {st}")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = px / (ms_step * 1e-3) / 1e6
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.1f} Mpixel/s")
+    result = work.numpy()
+
+    # --- per-kernel device times (HIP events on the stream the kernels run on)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, reset):
+        ts = []
+        for _ in range(args.kernel_iters):
+            reset()
+            ev[0].record(stream)
+            fn()
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        return statistics.median(ts)
+
+    scratch = iwpp.DeviceImage(marker0.h, marker0.w, marker0.dtype, marker0.pitch)
+
+    def reset_marker():
+        work.buf.copy_(marker0.buf)
+        iwpp.clip(work, mask)
+
+    ms_row = timed(lambda: iwpp.sweep_rows(work, mask), reset_marker)
+    ms_col = timed(lambda: iwpp.sweep_cols(work, scratch, mask, case.connectivity, 1,
+                                           opt.col_block_rows), reset_marker)
+    bytes_sweep = 3 * tsize * px
+    phases = {k: statistics.mean(s[k] for s in stats) for k in ("ms_sweep", "ms_seed", "ms_queue")}
+    counters = {k: stats[-1][k] for k in ("tile_visits", "seeded", "sweep_rounds", "queue_rounds",
+                                          "restarts", "queue", "tile")}
+    dominant = max(phases, key=phases.get)
+    kernel_names = {"ms_sweep": "k_row_pass+k_col_pass", "ms_seed": "k_unstable",
+                    "ms_queue": {"mutex": "k_queue_mutex", "dq": "k_queue_bsp<DQ>",
+                                 "uq": "k_queue_bsp<UQ>", "auto": "k_queue_bsp<UQ>"}[args.queue]}
+    dom_ms = phases[dominant]
+    achieved = bytes_sweep / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
+    roofline = {
+        "bound": "hbm",
+        "kernel": kernel_names[dominant],
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        "traffic": None,
+        "algorithmic_bytes": bytes_sweep,
+        "basis": "3*sizeof(T) bytes per pixel (read marker, read mask, write marker) "
+                 "over the dominant phase's average device time",
+    }
+    sweep_kernels = {
+        "row_pass_ms": ms_row,
+        "row_pass_GBps": bytes_sweep / (ms_row * 1e-3) / 1e9,
+        "row_pass_frac": bytes_sweep / (ms_row * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "col_pass_ms": ms_col,
+        "col_pass_GBps": bytes_sweep / (ms_col * 1e-3) / 1e9,
+        "col_pass_frac": bytes_sweep / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+    # --- CPU baseline: the oracle (a port of the sequential hybrid), one core
+    cpu = None
+    exact = None
+    if not args.no_cpu:
+        import oracle
+        times = []
+        want = None
+        for _ in range(max(1, args.cpu_runs)):
+            t = time.perf_counter()
+            want = oracle.reconstruct(case.marker, case.mask, case.connectivity)
+            times.append(time.perf_counter() - t)
+        med = statistics.median(times)
+        cpu = {"value": px / med / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
+               "sample": f"full {case.name} image, median of {len(times)} runs "
+                         f"({med * 1e3:.1f} ms each), oracle/iwpp_oracle.c -O3",
+               "host_cpus": os.cpu_count()}
+        exact = bool(np.array_equal(result, want))
+        log(f"cpu oracle {med * 1e3:.1f} ms; GPU output bit-exact: {exact}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8" if tsize == 1 else "u16",
+        "data": "synthetic",
+        "config": {"workload": case.name, "pixels": px, "connectivity": case.connectivity,
+                   "tile": case.tile, "queue": args.queue, "schedule": args.schedule,
+                   **case.meta},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "exact": exact,
+        "phases_ms": phases,
+        "counters": counters,
+        "sweep_kernels": sweep_kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

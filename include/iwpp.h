@@ -1,0 +1,155 @@
+/*
+ * iwpp.h - C ABI of the MI355X-native IWPP grayscale morphological
+ * reconstruction by dilation (library: iwpp-mp_amd/libiwpp.so, gfx950).
+ *
+ * The reference (oliveira-mathias/IWPP-MP) exposes one synchronous entry
+ * point, built three times (one per queue variant):
+ *   void MorphologicalReconstruction(unsigned char* marker, unsigned char* mask,
+ *                                    size_t pitchMarker, size_t pitchMask, int size);
+ *   IWPP MP/include/MorphologicalReconstruction.cuh:45-46 (mutex queue),
+ *   IWPP MP DQ/include/MorphologicalReconstruction.cuh:53-54 (double queue),
+ *   IWPP MP UQ/include/MorphologicalReconstruction.cuh:59-60 (unique queue).
+ * This header keeps that exact function (at the bottom) and adds the general
+ * entry point the reference hard-codes at build time: dtype, connectivity,
+ * width x height, tile size, queue variant and schedule become arguments.
+ *
+ * Conventions
+ *  - Images are device pointers (hipMalloc / hipMallocPitch / torch) with row
+ *    pitches in bytes.  Base pointers must be 4-byte aligned and pitches a
+ *    multiple of 4 (cudaMallocPitch/hipMallocPitch give both); 16-byte
+ *    alignment of base and pitch selects the wide-vector kernels.
+ *  - marker is updated in place to the reconstruction; mask is only read.
+ *    marker is clipped to mask first (IWPP MP/src/main.cu:55 does this in the
+ *    caller; here it is part of the call and is a no-op on clipped input).
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are ordered on
+ *    the stream; entry points that report counters or restart after a queue
+ *    overflow synchronise the stream before returning.
+ *  - Status codes instead of printf: IWPP_OK or a negative IWPP_ERR_*;
+ *    iwpp_last_error() gives a message for the calling thread.
+ *  - Workspaces own all scratch memory; one workspace per concurrent stream.
+ *    ws = NULL uses a per-device default workspace (not thread-safe).
+ */
+#ifndef IWPP_H
+#define IWPP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { IWPP_U8 = 1, IWPP_U16 = 2 };
+enum {
+    IWPP_QUEUE_MUTEX = 0, /* one global queue, persistent kernel (IWPP MP/src/globalQueue.cu) */
+    IWPP_QUEUE_DQ = 1,    /* read/write queues swapped between rounds (IWPP MP DQ/src/globalQueue.cu) */
+    IWPP_QUEUE_UQ = 2,    /* DQ with one entry per tile per round (IWPP MP UQ/src/MorphologicalReconstruction.cu:1152-1293) */
+    IWPP_QUEUE_AUTO = 3   /* autotuner choice */
+};
+enum { IWPP_SCHED_PLAIN = 0, IWPP_SCHED_LEARNED = 1 };
+enum {
+    IWPP_OK = 0,
+    IWPP_ERR_ARG = -1,        /* bad argument (null, size, alignment, enum) */
+    IWPP_ERR_HIP = -2,        /* a HIP runtime call failed */
+    IWPP_ERR_TIMEOUT = -3,    /* a device-side wait exceeded its time limit */
+    IWPP_ERR_UNSUPPORTED = -4,/* valid but unsupported combination */
+    IWPP_ERR_NOMEM = -5       /* workspace allocation failed */
+};
+
+/* Megapixel (queue work unit) edge in pixels: 64 x 64. */
+#define IWPP_MP_SIZE 64
+
+typedef struct iwpp_workspace iwpp_workspace;
+
+/* Counters of one reconstruction (filled when a non-NULL pointer is passed). */
+typedef struct {
+    long long tile_visits;   /* megapixel visits in the queue phase */
+    long long seeded;        /* megapixels put in the queue by the seeding pass(es) */
+    int sweep_rounds;        /* raster/anti-raster rounds before the queue phase */
+    int queue_rounds;        /* BSP rounds (DQ/UQ) */
+    int restarts;            /* queue overflows that re-seeded from the partial marker */
+    int queue;               /* queue variant actually used */
+    int tile;                /* tile size actually used */
+    int sweeps_skipped_tiles;/* tile-rounds skipped by the schedule */
+    float ms_sweep;          /* device time of the sweep phase (HIP events) */
+    float ms_seed;           /* device time of the seeding passes */
+    float ms_queue;          /* device time of the queue-phase kernels */
+} iwpp_stats;
+
+/* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
+typedef struct {
+    int sweep_rounds;     /* rounds of (row pass, column pass down, column pass up)
+                             before seeding the queue; -1 = the schedule decides */
+    int col_block_rows;   /* rows per block of the column passes (default 256) */
+    int max_local_iters;  /* local sweep rounds per megapixel visit before it
+                             re-queues itself (default 8) */
+    int queue_capacity;   /* entries per queue; 0 = sized from the image.
+                             Small values force the overflow/restart path. */
+    int workers_per_cu;   /* persistent workgroups per CU; 0 = from occupancy */
+    double timeout_s;     /* device-side wait limit (default 60 s) */
+} iwpp_options;
+
+void iwpp_default_options(iwpp_options* opt);
+
+/* Workspace sized for images up to max_pixels pixels of up to max_dtype bytes,
+ * on the current device.  It grows on demand when a call needs more. */
+iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtype);
+void iwpp_workspace_destroy(iwpp_workspace* ws);
+
+/* The general entry point (replaces MorphologicalReconstruction, see top).
+ * width, height >= 1; dtype IWPP_U8/U16; connectivity 4 or 8;
+ * tile = scheduling tile edge in pixels (a multiple of 64, or 0 = auto);
+ * queue IWPP_QUEUE_*; schedule IWPP_SCHED_*. */
+int iwpp_reconstruct(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask,
+                     int width, int height, int dtype, int connectivity, int tile, int queue,
+                     int schedule, iwpp_workspace* ws, void* stream);
+
+/* Same, with options (NULL = defaults) and counters (NULL = not wanted). */
+int iwpp_reconstruct_ex(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask,
+                        int width, int height, int dtype, int connectivity, int tile, int queue,
+                        int schedule, const iwpp_options* opt, iwpp_stats* stats,
+                        iwpp_workspace* ws, void* stream);
+
+/* marker = min(marker, mask) (IWPP MP/src/MorphologicalReconstruction.cu:827-836). */
+int iwpp_clip(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask, int width,
+              int height, int dtype, void* stream);
+
+/* One row pass, forward then backward, in place, over whole rows
+ * (the reference's X sweeps, IWPP MP DQ/src/MorphologicalReconstruction.cu:817-912, 949-1040).
+ * changed (device int, may be NULL) is OR-ed with 1 when a pixel rose. */
+int iwpp_sweep_rows(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask,
+                    int width, int height, int dtype, int* changed, void* stream);
+
+/* One column pass, out of place, in blocks of (64 lanes x 16 bytes) columns x
+ * block_rows rows (the reference's Y sweeps,
+ * IWPP MP DQ/src/MorphologicalReconstruction.cu:914-947, 1042-1074).
+ * dir +1 = downward, -1 = upward.  Neighbours outside a block are read from src. */
+int iwpp_sweep_cols(const void* src, void* dst, const void* mask, size_t pitch_src,
+                    size_t pitch_dst, size_t pitch_mask, int width, int height, int dtype,
+                    int connectivity, int dir, int block_rows, int* changed, void* stream);
+
+/* Column block width in pixels used by iwpp_sweep_cols for these pointers. */
+int iwpp_sweep_cols_block_width(const void* src, const void* dst, const void* mask,
+                                size_t pitch_src, size_t pitch_dst, size_t pitch_mask,
+                                int dtype);
+
+/* Seeding test: flags[b] (device uint8, ceil(h/64)*ceil(w/64) entries) = 1 when
+ * megapixel b holds a pixel that a neighbour can still raise.  Writes the
+ * number of such megapixels to *count (device int, may be NULL). */
+int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pitch_marker,
+                             size_t pitch_mask, int width, int height, int dtype,
+                             int connectivity, uint8_t* flags, int* count, void* stream);
+
+/* Message for the last error on the calling thread ("" if none). */
+const char* iwpp_last_error(void);
+
+/* Legacy drop-in with the reference's exact signature and behaviour:
+ * uint8, square size x size, 4-connectivity, synchronous, default stream.
+ * Queue variant from env IWPP_QUEUE = mutex | dq | uq (default: uq). */
+void MorphologicalReconstruction(unsigned char* marker, unsigned char* mask, size_t pitchMarker,
+                                 size_t pitchMask, int size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,62 @@
+/*
+ * Synthetic workload generators for IWPP reconstruction (host code, C ABI).
+ *
+ * These build the benchmark and test inputs: blurred-noise masks with filled
+ * disks, h-dome markers and sparse-seed markers.  They are deterministic and
+ * integer-only (the same bytes on every x86-64 host), so one seed names one
+ * image.  Library: iwpp-mp_amd/libiwpp_synth.so.
+ *
+ * Random numbers: a splitmix64 counter hash for per-pixel noise (so rows can
+ * be filled in parallel) and xoshiro256** seeded through splitmix64 for
+ * everything drawn in sequence (disks, seed pixels, batch shapes).
+ * Gaussian blur: three successive box filters per axis whose widths follow
+ * Kovesi's "fast almost-Gaussian filtering" rule for the requested sigma,
+ * reflect borders, 8 fractional bits kept between passes.
+ *
+ * dtype: 1 = uint8, 2 = uint16.  Pitches are row strides in bytes.
+ * Functions return 0 on success and a negative value on bad arguments.
+ */
+#ifndef IWPP_SYNTH_H
+#define IWPP_SYNTH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mask: uniform noise, blurred with sigma (sigma <= 0: no blur), min-max
+ * rescaled to [0, maxval] (round half to even), then ndisks filled disks of
+ * integer radius uniform in [rmin, rmax] and value uniform in [0, maxval]
+ * composited with max.  nthreads <= 0 uses the hardware thread count. */
+int iwpp_synth_mask(void* mask, size_t pitch, int width, int height, int dtype, int maxval,
+                    double sigma, int ndisks, int rmin, int rmax, uint64_t seed, int nthreads);
+
+/* h-dome marker: marker = max(mask - h, 0). */
+int iwpp_synth_marker_hdome(void* marker, const void* mask, size_t pitch_marker,
+                            size_t pitch_mask, int width, int height, int dtype, int h);
+
+/* Sparse-seed marker: 0 everywhere except `count` distinct pixels, drawn
+ * uniformly, where marker = mask. */
+int iwpp_synth_marker_seeds(void* marker, const void* mask, size_t pitch_marker,
+                            size_t pitch_mask, int width, int height, int dtype, long count,
+                            uint64_t seed);
+
+/* Shapes and marker kinds of the batched workload: side s = floor(256 * 16^u)
+ * (log-uniform in [256, 4096]), aspect a ~ U[1, 4], width = s,
+ * height = max(1, floor(s / a)); every second pair is transposed.  kind[i] is
+ * 0 for an h-dome marker (h[i] uniform in [10, 60]) and 1 for 0.1% seeds. */
+int iwpp_synth_batch_shapes(int n, uint64_t seed, int* widths, int* heights, int* kinds,
+                            int* hs);
+
+/* Random stress case in the spirit of the reference's InitRandomTestCase
+ * (IWPP MP/src/testCaseGenerator.cu:28-35): marker uniform in [0, 255],
+ * mask = marker + r % (256 - marker).  Not bit-compatible with cuRAND. */
+int iwpp_synth_random_case(void* marker, void* mask, size_t pitch_marker, size_t pitch_mask,
+                           int width, int height, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,320 @@
+"""Python binding of the MI355X IWPP reconstruction library (ctypes over the C ABI).
+
+The product is the C ABI in include/iwpp.h, implemented by libiwpp.so (HIP
+kernels for gfx950 plus the C++ host driver).  This module only binds it:
+device memory and streams come from PyTorch, every computation runs in the
+library.  There is no fallback: if libiwpp.so is missing, importing the
+binding raises.
+
+Reference interface mirrored here (IWPP MP/include/MorphologicalReconstruction.cuh:45-46):
+    MorphologicalReconstruction(marker, mask, pitchMarker, pitchMask, size)
+with device pointers, row pitches in bytes, a square uint8 image, in place.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_PKG, "libiwpp.so")
+SYNTH_PATH = os.path.join(_PKG, "libiwpp_synth.so")
+
+U8, U16 = 1, 2
+QUEUES = {"mutex": 0, "dq": 1, "uq": 2, "auto": 3}
+SCHEDULES = {"plain": 0, "learned": 1}
+MP_SIZE = 64
+
+IWPP_OK = 0
+ERRORS = {-1: "IWPP_ERR_ARG", -2: "IWPP_ERR_HIP", -3: "IWPP_ERR_TIMEOUT",
+          -4: "IWPP_ERR_UNSUPPORTED", -5: "IWPP_ERR_NOMEM"}
+
+
+class IWPPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("sweep_rounds", ctypes.c_int), ("col_block_rows", ctypes.c_int),
+                ("max_local_iters", ctypes.c_int), ("queue_capacity", ctypes.c_int),
+                ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("tile_visits", ctypes.c_longlong), ("seeded", ctypes.c_longlong),
+                ("sweep_rounds", ctypes.c_int), ("queue_rounds", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("queue", ctypes.c_int), ("tile", ctypes.c_int),
+                ("sweeps_skipped_tiles", ctypes.c_int), ("ms_sweep", ctypes.c_float),
+                ("ms_seed", ctypes.c_float), ("ms_queue", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Every entry point declared in include/iwpp.h, with its ctypes signature.
+_SZ, _VP, _I = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int
+SIGNATURES = {
+    "iwpp_default_options": (None, [ctypes.POINTER(Options)]),
+    "iwpp_workspace_create": (_VP, [_SZ, _I]),
+    "iwpp_workspace_destroy": (None, [_VP]),
+    "iwpp_reconstruct": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _I, _VP, _VP]),
+    "iwpp_reconstruct_ex": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _I,
+                                 ctypes.POINTER(Options), ctypes.POINTER(Stats), _VP, _VP]),
+    "iwpp_clip": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _VP]),
+    "iwpp_sweep_rows": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _VP, _VP]),
+    "iwpp_sweep_cols": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _VP, _VP]),
+    "iwpp_sweep_cols_block_width": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I]),
+    "iwpp_unstable_megapixels": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _VP, _VP, _VP]),
+    "iwpp_last_error": (ctypes.c_char_p, []),
+    "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
+}
+
+SYNTH_SIGNATURES = {
+    "iwpp_synth_mask": (_I, [_VP, _SZ, _I, _I, _I, _I, ctypes.c_double, _I, _I, _I,
+                             ctypes.c_uint64, _I]),
+    "iwpp_synth_marker_hdome": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I]),
+    "iwpp_synth_marker_seeds": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, ctypes.c_long,
+                                     ctypes.c_uint64]),
+    "iwpp_synth_batch_shapes": (_I, [_I, ctypes.c_uint64, _VP, _VP, _VP, _VP]),
+    "iwpp_synth_random_case": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, ctypes.c_uint64]),
+}
+
+_LIB = None
+_SYNTH = None
+
+
+def _bind(path, sigs):
+    if not os.path.exists(path):
+        raise ImportError(f"{os.path.basename(path)} is not built; run "
+                          "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def lib():
+    """The product library (loads it on first use; raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        _LIB = _bind(LIB_PATH, SIGNATURES)
+    return _LIB
+
+
+def synth_lib():
+    global _SYNTH
+    if _SYNTH is None:
+        _SYNTH = _bind(SYNTH_PATH, SYNTH_SIGNATURES)
+    return _SYNTH
+
+
+def _raise(rc):
+    if rc != IWPP_OK:
+        msg = lib().iwpp_last_error()
+        raise IWPPError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def default_options(**kw):
+    o = Options()
+    lib().iwpp_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+# --------------------------------------------------------------- images -----
+def _torch():
+    import torch
+    return torch
+
+
+class DeviceImage:
+    """A pitched image in device memory: a torch uint8 buffer of H x pitch bytes."""
+
+    def __init__(self, height, width, dtype=U8, pitch=None, device="cuda", buf=None):
+        torch = _torch()
+        self.h, self.w, self.dtype = int(height), int(width), int(dtype)
+        row = self.w * self.dtype
+        self.pitch = int(pitch) if pitch else (row + 255) // 256 * 256
+        if buf is None:
+            buf = torch.zeros((self.h, self.pitch), dtype=torch.uint8, device=device)
+        self.buf = buf
+
+    @property
+    def ptr(self):
+        return self.buf.data_ptr()
+
+    @classmethod
+    def from_numpy(cls, a, pitch=None, device="cuda"):
+        torch = _torch()
+        a = np.ascontiguousarray(a)
+        dt = U8 if a.dtype == np.uint8 else U16 if a.dtype == np.uint16 else None
+        if dt is None:
+            raise TypeError("uint8 or uint16 images only")
+        im = cls(a.shape[0], a.shape[1], dt, pitch, device)
+        host = torch.zeros((im.h, im.pitch), dtype=torch.uint8)
+        host[:, : im.w * dt] = torch.from_numpy(a.view(np.uint8).reshape(im.h, im.w * dt))
+        im.buf.copy_(host.to(device))
+        return im
+
+    def numpy(self):
+        raw = self.buf.cpu().numpy()[:, : self.w * self.dtype]
+        return np.ascontiguousarray(raw).view(np.uint8 if self.dtype == U8 else np.uint16)
+
+    def raw_numpy(self):
+        return self.buf.cpu().numpy()
+
+
+def _stream(stream):
+    if stream is None:
+        return _torch().cuda.current_stream().cuda_stream
+    return int(stream)
+
+
+class Workspace:
+    def __init__(self, max_pixels=0, max_dtype=U8):
+        self.handle = lib().iwpp_workspace_create(max_pixels, max_dtype)
+        if not self.handle:
+            raise IWPPError(-5, "iwpp_workspace_create failed")
+
+    def close(self):
+        if self.handle:
+            lib().iwpp_workspace_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def reconstruct(marker, mask, connectivity=8, tile=0, queue="uq", schedule="plain",
+                options=None, workspace=None, stream=None):
+    """In-place reconstruction of DeviceImage `marker` under DeviceImage `mask`.
+    Returns the run's counters as a dict."""
+    assert marker.h == mask.h and marker.w == mask.w and marker.dtype == mask.dtype
+    st = Stats()
+    opt = options if options is not None else default_options()
+    rc = lib().iwpp_reconstruct_ex(marker.ptr, mask.ptr, marker.pitch, mask.pitch, marker.w,
+                                   marker.h, marker.dtype, connectivity, tile,
+                                   QUEUES[queue] if isinstance(queue, str) else queue,
+                                   SCHEDULES[schedule] if isinstance(schedule, str) else schedule,
+                                   ctypes.byref(opt), ctypes.byref(st),
+                                   workspace.handle if workspace else None, _stream(stream))
+    _raise(rc)
+    return st.as_dict()
+
+
+def clip(marker, mask, stream=None):
+    _raise(lib().iwpp_clip(marker.ptr, mask.ptr, marker.pitch, mask.pitch, marker.w, marker.h,
+                           marker.dtype, _stream(stream)))
+
+
+def sweep_rows(marker, mask, changed=None, stream=None):
+    _raise(lib().iwpp_sweep_rows(marker.ptr, mask.ptr, marker.pitch, mask.pitch, marker.w,
+                                 marker.h, marker.dtype,
+                                 changed.data_ptr() if changed is not None else None,
+                                 _stream(stream)))
+
+
+def sweep_cols(src, dst, mask, connectivity, direction, block_rows, changed=None, stream=None):
+    _raise(lib().iwpp_sweep_cols(src.ptr, dst.ptr, mask.ptr, src.pitch, dst.pitch, mask.pitch,
+                                 src.w, src.h, src.dtype, connectivity, direction, block_rows,
+                                 changed.data_ptr() if changed is not None else None,
+                                 _stream(stream)))
+
+
+def sweep_cols_block_width(src, dst, mask):
+    return _raise_nonneg(lib().iwpp_sweep_cols_block_width(src.ptr, dst.ptr, mask.ptr, src.pitch,
+                                                           dst.pitch, mask.pitch, src.dtype))
+
+
+def _raise_nonneg(v):
+    if v < 0:
+        _raise(v)
+    return v
+
+
+def unstable_megapixels(marker, mask, connectivity, stream=None):
+    """uint8 numpy flags per 64 x 64 megapixel and their count."""
+    torch = _torch()
+    nmx = (marker.w + MP_SIZE - 1) // MP_SIZE
+    nmy = (marker.h + MP_SIZE - 1) // MP_SIZE
+    flags = torch.zeros((nmy, nmx), dtype=torch.uint8, device=marker.buf.device)
+    count = torch.zeros((1,), dtype=torch.int32, device=marker.buf.device)
+    _raise(lib().iwpp_unstable_megapixels(marker.ptr, mask.ptr, marker.pitch, mask.pitch,
+                                          marker.w, marker.h, marker.dtype, connectivity,
+                                          flags.data_ptr(), count.data_ptr(), _stream(stream)))
+    torch.cuda.synchronize()
+    return flags.cpu().numpy(), int(count.item())
+
+
+def MorphologicalReconstruction(marker, mask, pitchMarker, pitchMask, size):
+    """The reference's entry point: device pointers, pitches in bytes, square
+    uint8 image of side `size`, 4-connectivity, synchronous, in place."""
+    lib().MorphologicalReconstruction(marker, mask, pitchMarker, pitchMask, size)
+
+
+# ------------------------------------------------------------- synthetic ----
+def synth_mask(height, width, dtype=U8, maxval=None, sigma=4.0, ndisks=0, rmin=50, rmax=400,
+               seed=1313, nthreads=0):
+    np_dt = np.uint8 if dtype == U8 else np.uint16
+    maxval = maxval if maxval is not None else (255 if dtype == U8 else 4095)
+    out = np.zeros((height, width), np_dt)
+    rc = synth_lib().iwpp_synth_mask(out.ctypes.data, out.shape[1] * out.itemsize, width, height, dtype,
+                                     maxval, float(sigma), ndisks, rmin, rmax, seed, nthreads)
+    if rc:
+        raise ValueError(f"iwpp_synth_mask failed ({rc})")
+    return out
+
+
+def synth_marker_hdome(mask, h):
+    out = np.zeros_like(mask)
+    dt = U8 if mask.dtype == np.uint8 else U16
+    rc = synth_lib().iwpp_synth_marker_hdome(out.ctypes.data, mask.ctypes.data, out.shape[1] * out.itemsize,
+                                             mask.shape[1] * mask.itemsize, mask.shape[1], mask.shape[0], dt,
+                                             int(h))
+    if rc:
+        raise ValueError(f"iwpp_synth_marker_hdome failed ({rc})")
+    return out
+
+
+def synth_marker_seeds(mask, count, seed):
+    out = np.zeros_like(mask)
+    dt = U8 if mask.dtype == np.uint8 else U16
+    rc = synth_lib().iwpp_synth_marker_seeds(out.ctypes.data, mask.ctypes.data, out.shape[1] * out.itemsize,
+                                             mask.shape[1] * mask.itemsize, mask.shape[1], mask.shape[0], dt,
+                                             int(count), seed)
+    if rc:
+        raise ValueError(f"iwpp_synth_marker_seeds failed ({rc})")
+    return out
+
+
+def synth_batch_shapes(n, seed):
+    w = np.zeros(n, np.int32)
+    h = np.zeros(n, np.int32)
+    k = np.zeros(n, np.int32)
+    hh = np.zeros(n, np.int32)
+    rc = synth_lib().iwpp_synth_batch_shapes(n, seed, w.ctypes.data, h.ctypes.data,
+                                             k.ctypes.data, hh.ctypes.data)
+    if rc:
+        raise ValueError("iwpp_synth_batch_shapes failed")
+    return w, h, k, hh
+
+
+def synth_random_case(height, width, seed):
+    j = np.zeros((height, width), np.uint8)
+    i = np.zeros((height, width), np.uint8)
+    rc = synth_lib().iwpp_synth_random_case(j.ctypes.data, i.ctypes.data, j.shape[1],
+                                            i.shape[1], width, height, seed)
+    if rc:
+        raise ValueError("iwpp_synth_random_case failed")
+    return j, i

@@ -1,0 +1,202 @@
+"""GPU parity: every kernel and the full reconstruction, through the C ABI,
+against the CPU oracle on the same inputs.  Integer work: bit-exact."""
+import sys
+
+import numpy as np
+import pytest
+
+import iwpp
+import oracle
+from iwpp import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def up(a, pitch=None):
+    return iwpp.DeviceImage.from_numpy(a, pitch=pitch)
+
+
+def rand_pair(rng, h, w, dtype, kind="random"):
+    top = 255 if dtype == np.uint8 else 4095
+    mask = rng.integers(0, top + 1, (h, w)).astype(dtype)
+    if kind == "seeds":
+        marker = np.zeros_like(mask)
+        idx = rng.choice(h * w, max(1, h * w // 400), replace=False)
+        marker.flat[idx] = mask.flat[idx]
+    elif kind == "hdome":
+        marker = np.maximum(mask.astype(np.int64) - top // 6, 0).astype(dtype)
+    else:
+        marker = np.minimum(rng.integers(0, top + 1, (h, w)).astype(dtype), mask)
+    return marker, mask
+
+
+def odd_pitch(w, dtype):
+    # a multiple of 4 that is not a multiple of 16: selects the 4-byte kernels
+    p = (w * np.dtype(dtype).itemsize + 3) // 4 * 4
+    return p + 4 if p % 16 == 0 else p
+
+
+ROW_SHAPES = [(3, 1), (2, 17), (5, 1023), (4, 4099), (3, 8200), (2, 16400), (2, 33000)]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("pitch_kind", ["aligned", "odd"])
+def test_row_pass(torch_cuda, dtype, pitch_kind):
+    rng = np.random.default_rng(1)
+    for h, w in ROW_SHAPES:
+        say("row_pass", dtype.__name__, pitch_kind, h, w)
+        J, I = rand_pair(rng, h, w, dtype, "seeds")
+        pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
+        dj, di = up(J, pitch), up(I, pitch)
+        raw_before = dj.raw_numpy()
+        ch = torch_cuda.zeros(1, dtype=torch_cuda.int32, device="cuda")
+        iwpp.sweep_rows(dj, di, ch)
+        torch_cuda.cuda.synchronize()
+        want = oracle.pass_x(J, I)
+        assert np.array_equal(dj.numpy(), want), (h, w)
+        # padding bytes beyond the row are never written
+        rb = w * np.dtype(dtype).itemsize
+        assert np.array_equal(dj.raw_numpy()[:, rb:], raw_before[:, rb:])
+        assert int(ch.item()) == int(not np.array_equal(want, J))
+
+
+COL_SHAPES = [(1, 5), (17, 3), (130, 1100), (300, 513), (64, 2048)]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("pitch_kind", ["aligned", "odd"])
+def test_col_pass(torch_cuda, dtype, conn, pitch_kind):
+    rng = np.random.default_rng(2)
+    for h, w in COL_SHAPES:
+        for rows in (1, 7, 64, 1000):
+            for d in (1, -1):
+                say("col_pass", dtype.__name__, conn, pitch_kind, h, w, rows, d)
+                J, I = rand_pair(rng, h, w, dtype, "seeds")
+                pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
+                src, msk = up(J, pitch), up(I, pitch)
+                dst = iwpp.DeviceImage(h, w, src.dtype, pitch)
+                bw = iwpp.sweep_cols_block_width(src, dst, msk)
+                iwpp.sweep_cols(src, dst, msk, conn, d, rows)
+                torch_cuda.cuda.synchronize()
+                want = oracle.pass_y(J, I, conn, d, bw, rows)
+                assert np.array_equal(dst.numpy(), want), (h, w, rows, d, bw)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("conn", [4, 8])
+def test_unstable_megapixels(torch_cuda, dtype, conn):
+    rng = np.random.default_rng(3)
+    for h, w in [(1, 1), (64, 64), (65, 129), (200, 333), (512, 300)]:
+        say("unstable", dtype.__name__, conn, h, w)
+        J, I = rand_pair(rng, h, w, dtype, "seeds")
+        for stage in range(3):
+            if stage == 1:
+                J = oracle.reconstruct(J, I, conn)
+            if stage == 2:  # a few planted dips
+                for _ in range(3):
+                    J[rng.integers(0, h), rng.integers(0, w)] = 0
+            flags, count = iwpp.unstable_megapixels(up(J), up(I), conn)
+            want = oracle.unstable_blocks(J, I, conn, 64)
+            assert np.array_equal(flags, want), (h, w, stage)
+            assert count == int(want.sum())
+
+
+VARIANTS = ["mutex", "dq", "uq"]
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_reconstruct_random(torch_cuda, queue, conn, dtype):
+    rng = np.random.default_rng(4)
+    shapes = [(1, 1), (1, 300), (300, 1), (64, 64), (65, 129), (200, 333), (640, 257)]
+    for h, w in shapes:
+        for kind in ("random", "seeds", "hdome"):
+            for sweeps in (0, 1, 2):
+                say("reconstruct", queue, conn, dtype.__name__, h, w, kind, sweeps)
+                J, I = rand_pair(rng, h, w, dtype, kind)
+                dj, di = up(J), up(I)
+                st = iwpp.reconstruct(dj, di, conn, 0, queue, "plain",
+                                      iwpp.default_options(sweep_rounds=sweeps))
+                want = oracle.reconstruct(J, I, conn)
+                got = dj.numpy()
+                assert np.array_equal(got, want), (h, w, kind, sweeps, st)
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+def test_reconstruct_unclipped_marker_and_odd_pitch(torch_cuda, queue):
+    rng = np.random.default_rng(5)
+    for dtype in (np.uint8, np.uint16):
+        for conn in (4, 8):
+            say("unclipped", queue, dtype.__name__, conn)
+            I = rng.integers(0, 200, (150, 211)).astype(dtype)
+            J = rng.integers(0, 255, (150, 211)).astype(dtype)  # not clipped
+            pitch = odd_pitch(211, dtype)
+            dj, di = up(J, pitch), up(I, pitch)
+            iwpp.reconstruct(dj, di, conn, 0, queue, "plain",
+                             iwpp.default_options(sweep_rounds=0))
+            assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn))
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+def test_golden_jpeg_crops(torch_cuda, queue):
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "jpeg_crops.npz"))
+    for k in range(5):
+        for conn in (4, 8):
+            say("golden", queue, k, conn)
+            dj, di = up(d[f"crop{k}_marker"]), up(d[f"crop{k}_mask"])
+            iwpp.reconstruct(dj, di, conn, 0, queue)
+            assert np.array_equal(dj.numpy(), d[f"crop{k}_out{conn}"]), (k, conn)
+
+
+def test_legacy_entry_point(torch_cuda):
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "jpeg_crops.npz"))
+    for k in range(3):  # the square crops
+        say("legacy", k)
+        dj, di = up(d[f"crop{k}_marker"]), up(d[f"crop{k}_mask"])
+        iwpp.MorphologicalReconstruction(dj.ptr, di.ptr, dj.pitch, di.pitch, dj.w)
+        assert np.array_equal(dj.numpy(), d[f"crop{k}_out4"]), k
+
+
+def test_overflow_restart(torch_cuda):
+    rng = np.random.default_rng(6)
+    J, I = rand_pair(rng, 700, 500, np.uint8, "seeds")
+    want = oracle.reconstruct(J, I, 8)
+    for queue in ("dq", "uq"):
+        say("overflow", queue)
+        dj, di = up(J), up(I)
+        st = iwpp.reconstruct(dj, di, 8, 0, queue, "plain",
+                              iwpp.default_options(sweep_rounds=0, queue_capacity=16))
+        assert st["restarts"] > 0, st
+        assert np.array_equal(dj.numpy(), want)
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+def test_config1_full(torch_cuda, queue):
+    c = W.c1()
+    want = oracle.reconstruct(c.marker, c.mask, c.connectivity)
+    for sweeps in (0, 1):
+        say("c1", queue, sweeps)
+        dj, di = up(c.marker), up(c.mask)
+        st = iwpp.reconstruct(dj, di, c.connectivity, c.tile, queue, "plain",
+                              iwpp.default_options(sweep_rounds=sweeps))
+        assert np.array_equal(dj.numpy(), want), st
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+def test_config2_full(torch_cuda, queue):
+    c = W.c2()
+    want = oracle.reconstruct(c.marker, c.mask, c.connectivity)
+    say("c2", queue)
+    dj, di = up(c.marker), up(c.mask)
+    st = iwpp.reconstruct(dj, di, c.connectivity, c.tile, queue)
+    say("c2 stats", st)
+    assert np.array_equal(dj.numpy(), want), st
