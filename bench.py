@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
-    ap.add_argument("--queue", default="uq", choices=["mutex", "dq", "uq", "auto"])
+    ap.add_argument("--queue", default="auto", choices=["mutex", "dq", "uq", "auto"])
     ap.add_argument("--schedule", default="plain", choices=["plain", "learned"])
     ap.add_argument("--sweeps", type=int, default=-1, help="sweep rounds (-1: schedule)")
     ap.add_argument("--cpu-runs", type=int, default=5)
@@ -118,9 +118,10 @@ def main():
     counters = {k: stats[-1][k] for k in ("tile_visits", "seeded", "sweep_rounds", "queue_rounds",
                                           "restarts", "queue", "tile")}
     dominant = max(phases, key=phases.get)
+    used_queue = {0: "mutex", 1: "dq", 2: "uq"}[stats[-1]["queue"]]
     kernel_names = {"ms_sweep": "k_row_pass+k_col_pass", "ms_seed": "k_unstable",
                     "ms_queue": {"mutex": "k_queue_mutex", "dq": "k_queue_bsp<DQ>",
-                                 "uq": "k_queue_bsp<UQ>", "auto": "k_queue_bsp<UQ>"}[args.queue]}
+                                 "uq": "k_queue_bsp<UQ>"}[used_queue]}
     dom_ms = phases[dominant]
     achieved = bytes_sweep / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     roofline = {
@@ -177,7 +178,8 @@ def main():
         "dtype": "u8" if tsize == 1 else "u16",
         "data": "synthetic",
         "config": {"workload": case.name, "pixels": px, "connectivity": case.connectivity,
-                   "tile": case.tile, "queue": args.queue, "schedule": args.schedule,
+                   "tile": case.tile, "queue": args.queue, "queue_used": used_queue,
+                   "schedule": args.schedule,
                    **case.meta},
         "roofline": roofline,
         "cpu_baseline": cpu,

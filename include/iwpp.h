@@ -74,19 +74,29 @@ typedef struct {
     float ms_sweep;          /* device time of the sweep phase (HIP events) */
     float ms_seed;           /* device time of the seeding passes */
     float ms_queue;          /* device time of the queue-phase kernels */
+    /* With iwpp_options.trace_phases: wave-microseconds spent in the queue
+     * phase, summed over all waves (the reference's clock64 split,
+     * IWPP MP/src/MorphologicalReconstruction.cu:199-231,534-540). */
+    double us_wait;          /* taking a megapixel (incl. idle waiting, barriers) */
+    double us_fetch;         /* loading megapixel + halo into LDS */
+    double us_converge;      /* local sweeps */
+    double us_store;         /* write-back */
+    double us_notify;        /* neighbour tests and queue pushes */
 } iwpp_stats;
 
 /* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
 typedef struct {
     int sweep_rounds;     /* rounds of (row pass, column pass down, column pass up)
                              before seeding the queue; -1 = the schedule decides */
-    int col_block_rows;   /* rows per block of the column passes (default 256) */
+    int col_block_rows;   /* rows per block of the column passes (0 = from the
+                             image size, >= 8192 waves, default) */
     int max_local_iters;  /* local sweep rounds per megapixel visit before it
                              re-queues itself (default 8) */
     int queue_capacity;   /* entries per queue; 0 = sized from the image.
                              Small values force the overflow/restart path. */
     int workers_per_cu;   /* persistent workgroups per CU; 0 = from occupancy */
     double timeout_s;     /* device-side wait limit (default 60 s) */
+    int trace_phases;     /* 1 = fill the us_* phase timers of iwpp_stats */
 } iwpp_options;
 
 void iwpp_default_options(iwpp_options* opt);
@@ -120,18 +130,20 @@ int iwpp_clip(void* marker, const void* mask, size_t pitch_marker, size_t pitch_
 int iwpp_sweep_rows(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask,
                     int width, int height, int dtype, int* changed, void* stream);
 
-/* One column pass, out of place, in blocks of (64 lanes x 16 bytes) columns x
- * block_rows rows (the reference's Y sweeps,
+/* One column pass, out of place, in blocks of (64 lanes x 16 or 4 bytes)
+ * columns x block_rows rows (block_rows <= 0: chosen from the image size) (the reference's Y sweeps,
  * IWPP MP DQ/src/MorphologicalReconstruction.cu:914-947, 1042-1074).
  * dir +1 = downward, -1 = upward.  Neighbours outside a block are read from src. */
 int iwpp_sweep_cols(const void* src, void* dst, const void* mask, size_t pitch_src,
                     size_t pitch_dst, size_t pitch_mask, int width, int height, int dtype,
                     int connectivity, int dir, int block_rows, int* changed, void* stream);
 
-/* Column block width in pixels used by iwpp_sweep_cols for these pointers. */
+/* Column block width in pixels used by iwpp_sweep_cols for these pointers
+ * and this width (16-byte lanes for aligned rows of at least 16 KiB, else
+ * 4-byte lanes). */
 int iwpp_sweep_cols_block_width(const void* src, const void* dst, const void* mask,
                                 size_t pitch_src, size_t pitch_dst, size_t pitch_mask,
-                                int dtype);
+                                int width, int dtype);
 
 /* Seeding test: flags[b] (device uint8, ceil(h/64)*ceil(w/64) entries) = 1 when
  * megapixel b holds a pixel that a neighbour can still raise.  Writes the

@@ -79,8 +79,8 @@ struct iwpp_workspace {
     size_t qa_n = 0;
     uint32_t* qb = nullptr;
     size_t qb_n = 0;
-    uint32_t* seq = nullptr;
-    size_t seq_n = 0;
+    unsigned long long* ring = nullptr;  // mutex ring
+    size_t ring_n = 0;
     uint32_t* ctl = nullptr;    // control block
     size_t ctl_n = 0;
     uint32_t* host_ctl = nullptr;  // pinned copy of the control block
@@ -92,11 +92,12 @@ struct iwpp_workspace {
 extern "C" void iwpp_default_options(iwpp_options* o) {
     if (!o) return;
     o->sweep_rounds = -1;
-    o->col_block_rows = 256;
+    o->col_block_rows = 0;
     o->max_local_iters = 8;
     o->queue_capacity = 0;
     o->workers_per_cu = 0;
     o->timeout_s = 60.0;
+    o->trace_phases = 0;
 }
 
 extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtype) {
@@ -122,7 +123,7 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->flags);
     (void)hipFree(ws->qa);
     (void)hipFree(ws->qb);
-    (void)hipFree(ws->seq);
+    (void)hipFree(ws->ring);
     (void)hipFree(ws->ctl);
     (void)hipFree(ws->tiles);
     if (ws->host_ctl) (void)hipHostFree(ws->host_ctl);
@@ -223,8 +224,7 @@ int queue_phase(Run& r) {
     size_t cap;
     if (kind == Q_MUTEX) {
         cap = next_pow2(nmp + 16384);
-        HIP_TRY(grow(ws->qa, ws->qa_n, cap));
-        HIP_TRY(grow(ws->seq, ws->seq_n, cap));
+        HIP_TRY(grow(ws->ring, ws->ring_n, cap));
     } else {
         if (r.opt.queue_capacity > 0) cap = (size_t)r.opt.queue_capacity;
         else cap = kind == Q_DQ ? 8 * nmp + 1024 : nmp + 64;
@@ -239,21 +239,23 @@ int queue_phase(Run& r) {
     q.ctl = ws->ctl;
     q.st = ws->st;
     q.qa = ws->qa;
-    q.qb = kind == Q_MUTEX ? ws->qa : ws->qb;
-    q.seq = ws->seq;
+    q.qb = ws->qb;
+    q.ring = ws->ring;
     q.cap = (uint32_t)cap;
     q.nmx = r.nmx;
     q.nmy = r.nmy;
     q.max_iters = std::max(1, r.opt.max_local_iters);
     q.max_rounds = 1 << 30;
     q.timeout_ticks = timeout_ticks(r.opt.timeout_s > 0 ? r.opt.timeout_s : 60.0);
+    q.trace = r.opt.trace_phases ? 1 : 0;
+    const double us_per_tick = 1e6 / ((double)timeout_ticks(1.0));
 
     const int max_restarts = 100000;
     for (int attempt = 0;; ++attempt) {
         if (r.timed) HIP_TRY(hipEventRecord(ws->ev[1], r.s));
         HIP_TRY(launch_fill_u32(ws->ctl, 0u, (size_t)C_NSLOTS * kCtlStride, r.s));
         HIP_TRY(launch_fill_u32(ws->st, 0u, nmp, r.s));
-        if (kind == Q_MUTEX) HIP_TRY(launch_iota_u32(ws->seq, cap, r.s));
+        if (kind == Q_MUTEX) HIP_TRY(launch_ring_init(ws->ring, cap, r.s));
         HIP_TRY(launch_unstable(r.im, r.dtype, r.conn, ws->flags, nullptr, r.s));
         HIP_TRY(launch_seed_queue(ws->flags, r.nmp, kind, q, r.s));
         HIP_TRY(launch_prepare_queue(q, r.s));
@@ -274,6 +276,16 @@ int queue_phase(Run& r) {
         r.st.seeded += h[C_TAIL * kCtlStride] + 0ll;
         r.st.tile_visits += (long long)(*(const unsigned long long*)(h + C_VISITS_LO * kCtlStride));
         r.st.queue_rounds += (int)h[C_ROUND * kCtlStride];
+        if (q.trace) {
+            auto ph = [&](int slot) {
+                return (double)(*(const unsigned long long*)(h + slot * kCtlStride)) * us_per_tick;
+            };
+            r.st.us_wait += ph(C_PH_POP);
+            r.st.us_fetch += ph(C_PH_FETCH);
+            r.st.us_converge += ph(C_PH_CONVERGE);
+            r.st.us_store += ph(C_PH_STORE);
+            r.st.us_notify += ph(C_PH_NOTIFY);
+        }
         if (h[C_TIMEOUT * kCtlStride])
             return fail(IWPP_ERR_TIMEOUT, "device-side queue wait timed out");
         if (!h[C_OVERFLOW * kCtlStride]) break;
@@ -309,7 +321,6 @@ extern "C" int iwpp_reconstruct_ex(void* marker, const void* mask, size_t pm, si
     r.s = (hipStream_t)stream;
     iwpp_default_options(&r.opt);
     if (opt) r.opt = *opt;
-    if (r.opt.col_block_rows <= 0) r.opt.col_block_rows = 256;
     r.nmx = (w + kMP - 1) / kMP;
     r.nmy = (h + kMP - 1) / kMP;
     r.nmp = r.nmx * r.nmy;
@@ -385,7 +396,6 @@ extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, siz
     if (src == dst) return fail(IWPP_ERR_ARG, "column pass is out of place: src == dst");
     if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
     if (dir != 1 && dir != -1) return fail(IWPP_ERR_ARG, "dir must be +1 or -1");
-    if (block_rows <= 0) return fail(IWPP_ERR_ARG, "block_rows must be positive");
     const TileMask none{nullptr, kMP, kMP, 1};
     HIP_TRY(launch_col_pass((const uint8_t*)src, (uint8_t*)dst, (const uint8_t*)mask, ps, pd, pk,
                             w, h, dtype, conn, dir, block_rows, none, (uint32_t*)changed,
@@ -394,9 +404,9 @@ extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, siz
 }
 
 extern "C" int iwpp_sweep_cols_block_width(const void* src, const void* dst, const void* mask,
-                                           size_t ps, size_t pd, size_t pk, int dtype) {
+                                           size_t ps, size_t pd, size_t pk, int w, int dtype) {
     if (dtype != IWPP_U8 && dtype != IWPP_U16) return fail(IWPP_ERR_ARG, "bad dtype");
-    return col_block_width(src, dst, mask, ps, pd, pk, dtype);
+    return col_block_width(src, dst, mask, ps, pd, pk, w, dtype);
 }
 
 extern "C" int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pm,

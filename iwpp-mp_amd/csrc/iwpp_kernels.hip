@@ -16,34 +16,60 @@ namespace iwpp {
 #define DEV static __device__ __forceinline__
 
 // ---------------------------------------------------------------- atomics ---
+// Inside the queue kernels every access to shared mutable data -- marker
+// words, queue entries, counters, flags, state words -- is a relaxed
+// agent-scope atomic.  On gfx950 these are the device-coherent forms
+// (global_load/store ... sc1, and RMWs performed at the device coherence
+// point), so they are coherent across the 8 XCDs without L2 maintenance.
+// Ordering between a wave's pixel stores and the atomics that hand a
+// megapixel to another wave is enforced by waiting for the wave's outstanding
+// memory operations (mem_wait: s_waitcnt vmcnt(0), with compiler barriers):
+//   visit: take item (RMW) -> mem_wait -> load -> ... -> store -> mem_wait -> notify (RMW).
+// An agent-scope release/acquire fence would do the same job but, on
+// gfx950, writes back / invalidates the whole XCD L2 each time; measured, that
+// was the dominant cost of a visit.  The mask is read-only and uses plain loads.
 DEV uint32_t a_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-DEV uint32_t a_load_rlx(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV void a_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-DEV void a_store_rlx(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_add(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_or(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_or(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_and(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_and(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_xchg(uint32_t* p, uint32_t v) {
-    return __hip_atomic_exchange(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_cas(uint32_t* p, uint32_t expected, uint32_t desired) {
-    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_ACQ_REL,
-                                         __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return expected;  // the value seen before the exchange
+}
+DEV unsigned long long a_load64(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DEV void a_store64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+DEV void mem_wait() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0) expcnt(0) lgkmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// Device-coherent 32-bit accesses to the marker and the queues.
+DEV uint32_t ld_coh(const void* p) {
+    return __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+DEV void st_coh(void* p, uint32_t v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 DEV uint32_t* ctl_at(uint32_t* ctl, int slot) { return ctl + slot * kCtlStride; }
@@ -501,153 +527,185 @@ __global__ __launch_bounds__(kWG) void k_col_pass(const uint8_t* __restrict__ S,
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
-// Lane c holds column x0+c: rows 0..63 in registers.  Halo columns sit in two
-// registers each (lane k holds row y0-1+k, k < 64; lanes 0,1 of the second
-// register hold k = 64, 65), read back with readlane at compile-time lanes.
+// A wave owns one megapixel at a time.  Its LDS tile holds rows -1..64 and
+// columns -1..64 (halo included, absent pixels 0) as packed words
+// (marker | mask << 16).  Row stride 67 words: lane-per-row and
+// lane-per-column accesses are both bank-conflict free (67 is odd, 67 mod 64
+// = 3 is a unit).  Every sweep is a per-lane sequential scan: lane = row for
+// the row sweeps, lane = column for the column sweeps.
+constexpr int TS = 67;             // LDS row stride, words
+constexpr int TILE_WORDS = 66 * TS;
+constexpr int ROSE_BYTES = 4 * 64; // per-border "rose in this visit" flags
+constexpr int WAVE_LDS = TILE_WORDS * 4 + ROSE_BYTES;
+
+DEV int tix(int r, int c) { return (r + 1) * TS + (c + 1); }  // r, c in [-1, 64]
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+DEV u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+DEV uint32_t as1(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+// w = (j | m << 16); p carries the neighbour's value in its low half.
+// Returns (max(j, min(m, p)), m): packed 16-bit min/max with op_sel.
+DEV uint32_t upd(uint32_t w, uint32_t p) {
+    const u16x2 W = as2(w), P = as2(p);
+    const u16x2 t = __builtin_elementwise_min(W.yx, P.xx);  // (min(m,p), min(j,p) <= m)
+    return as1(__builtin_elementwise_max(W, t));
+}
+DEV uint32_t pmax(uint32_t a, uint32_t b) {
+    return as1(__builtin_elementwise_max(as2(a), as2(b)));
+}
+// min(J(p), I(q)) > J(q) for a pixel value p and a packed neighbour word q.
+DEV bool raises(uint32_t p, uint32_t q) { return min(p, q >> 16) > (q & 0xFFFFu); }
+
+DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T>
 DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
     return *reinterpret_cast<const T*>(base + (size_t)y * pitch + (size_t)x * sizeof(T));
 }
-
-struct Tile {
-    uint32_t j[kMP], m[kMP];                // marker, mask
-    uint32_t roseLo, roseHi;                // bit r: row r of this lane's column rose
-    uint32_t hT, iT, hB, iB;                // halo rows above/below (this lane's column)
-    uint32_t hLa, hLb, iLa, iLb, hRa, hRb, iRa, iRb;  // halo columns
-};
-
-template <int K>
-DEV uint32_t halo(uint32_t a, uint32_t b) {
-    if constexpr (K < 64) return rdlane(a, K);
-    else return rdlane(b, K - 64);
-}
-
+// One marker pixel through a device-coherent load of its (4-byte aligned) word.
 template <class T>
-DEV void tile_fetch(const Image& im, int mx, int my, Tile& t) {
+DEV uint32_t ldp_coh(const uint8_t* base, size_t pitch, int x, int y) {
+    const size_t off = (size_t)x * sizeof(T);
+    const uint32_t w = ld_coh(base + (size_t)y * pitch + (off & ~(size_t)3));
+    return (w >> (8 * (off & 3))) & (uint32_t)((1u << (8 * sizeof(T))) - 1u);
+}
+
+// Loads the megapixel and its halo into the wave's LDS tile.  Rows are read as
+// 32-bit words (4 u8 / 2 u16 pixels per lane), 4 or 2 rows per instruction.
+template <class T>
+DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
+    constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
+    constexpr int BITS = 8 * (int)sizeof(T);
+    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
     const int lane = lane_id();
-    const int x0 = mx * kMP, y0 = my * kMP, x = x0 + lane;
-    const bool inx = x < im.W;
+    const int x0 = mx * kMP, y0 = my * kMP;
+    const int wl = lane % WPR, rl = lane / WPR;
+    const int xw = x0 + wl * PPW;
+    const bool xin = xw < im.W;
+    // All loads first, unconditionally from clamped (always valid) addresses so
+    // that they issue back to back; out-of-image words are zeroed afterwards.
+    constexpr int NIT = (66 + RPI - 1) / RPI;
+    uint32_t gj[NIT], gi[NIT];
+    const size_t xoff = (size_t)(xin ? xw : x0) * sizeof(T);
 #pragma unroll
-    for (int r = 0; r < kMP; ++r) {
-        const int y = y0 + r;
-        const bool in = inx && y < im.H;
-        t.j[r] = in ? ldp<T>(im.J, im.pj, x, y) : 0u;
-        t.m[r] = in ? ldp<T>(im.I, im.pi, x, y) : 0u;
+    for (int it = 0; it < NIT; ++it) {
+        const int y = y0 - 1 + it * RPI + rl;
+        const int yc = min(max(y, 0), im.H - 1);
+        gj[it] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
+        gi[it] = *reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff);
     }
-    t.roseLo = 0u;
-    t.roseHi = 0u;
-    const bool top = inx && y0 > 0, bot = inx && y0 + kMP < im.H;
-    t.hT = top ? ldp<T>(im.J, im.pj, x, y0 - 1) : 0u;
-    t.iT = top ? ldp<T>(im.I, im.pi, x, y0 - 1) : 0u;
-    t.hB = bot ? ldp<T>(im.J, im.pj, x, y0 + kMP) : 0u;
-    t.iB = bot ? ldp<T>(im.I, im.pi, x, y0 + kMP) : 0u;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int row = -1 + it * RPI + rl;
+        const int y = y0 + row;
+        const bool valid = row <= 64 && xin && y >= 0 && y < im.H;
+        gj[it] = valid ? gj[it] : 0u;
+        gi[it] = valid ? gi[it] : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int row = -1 + it * RPI + rl;
+        if (row > 64) continue;
+#pragma unroll
+        for (int k = 0; k < PPW; ++k) {
+            const bool in = xw + k < im.W;
+            const uint32_t j = in ? (gj[it] >> (BITS * k)) & PM : 0u;
+            const uint32_t m = in ? (gi[it] >> (BITS * k)) & PM : 0u;
+            s[tix(row, wl * PPW + k)] = j | (m << 16);
+        }
+    }
+    // halo columns x0-1 and x0+64, rows -1..64 (loads clamped the same way)
     const bool hasL = x0 > 0, hasR = x0 + kMP < im.W;
-    {
-        const int y = y0 - 1 + lane;
+    const int xl = hasL ? x0 - 1 : x0, xr = hasR ? x0 + kMP : x0;
+    uint32_t hj[4], hi[4];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int y = y0 - 1 + b * 64 + lane;
+        const int yc = min(max(y, 0), im.H - 1);
+        hj[2 * b] = ldp_coh<T>(im.J, im.pj, xl, yc);
+        hi[2 * b] = ldp<T>(im.I, im.pi, xl, yc);
+        hj[2 * b + 1] = ldp_coh<T>(im.J, im.pj, xr, yc);
+        hi[2 * b + 1] = ldp<T>(im.I, im.pi, xr, yc);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int row = -1 + b * 64 + lane;
+        const int y = y0 + row;
+        if (row > 64) continue;
         const bool iny = y >= 0 && y < im.H;
-        t.hLa = (iny && hasL) ? ldp<T>(im.J, im.pj, x0 - 1, y) : 0u;
-        t.iLa = (iny && hasL) ? ldp<T>(im.I, im.pi, x0 - 1, y) : 0u;
-        t.hRa = (iny && hasR) ? ldp<T>(im.J, im.pj, x0 + kMP, y) : 0u;
-        t.iRa = (iny && hasR) ? ldp<T>(im.I, im.pi, x0 + kMP, y) : 0u;
+        s[tix(row, -1)] = (iny && hasL) ? hj[2 * b] | (hi[2 * b] << 16) : 0u;
+        s[tix(row, 64)] = (iny && hasR) ? hj[2 * b + 1] | (hi[2 * b + 1] << 16) : 0u;
     }
-    {
-        const int y = y0 + 63 + lane;
-        const bool iny = lane < 2 && y < im.H;
-        t.hLb = (iny && hasL) ? ldp<T>(im.J, im.pj, x0 - 1, y) : 0u;
-        t.iLb = (iny && hasL) ? ldp<T>(im.I, im.pi, x0 - 1, y) : 0u;
-        t.hRb = (iny && hasR) ? ldp<T>(im.J, im.pj, x0 + kMP, y) : 0u;
-        t.iRb = (iny && hasR) ? ldp<T>(im.I, im.pi, x0 + kMP, y) : 0u;
+    wave_lds_sync();
+}
+
+// Row sweep over the interior, lane = row.  DIR +1: left to right.
+template <int DIR>
+DEV bool sweep_rows(uint32_t* s, int lane) {
+    int idx = DIR > 0 ? tix(lane, -1) : tix(lane, 64);
+    uint32_t p = s[idx];
+    bool ch = false;
+#pragma unroll 16
+    for (int k = 0; k < kMP; ++k) {
+        idx += DIR;
+        const uint32_t w = s[idx];
+        const uint32_t nw = upd(w, p);
+        ch |= nw != w;
+        s[idx] = nw;
+        p = nw;
     }
+    return ch;
 }
 
-template <int R>
-DEV void tile_set(Tile& t, uint32_t out, bool& ch) {
-    const bool c = out != t.j[R];
-    ch |= c;
-    if constexpr (R < 32) t.roseLo |= (uint32_t)c << R;
-    else t.roseHi |= (uint32_t)c << (R - 32);
-    t.j[R] = out;
-}
-template <int R>
-DEV bool tile_rose(const Tile& t) {
-    if constexpr (R < 32) return (t.roseLo >> R) & 1u;
-    else return (t.roseHi >> (R - 32)) & 1u;
-}
-DEV bool rose_bit(uint32_t lo, uint32_t hi, int r) {
-    return ((r < 32 ? lo >> r : hi >> (r - 32)) & 1u) != 0u;
-}
-
-template <int R>
-DEV void tile_xf_row(Tile& t, bool& ch) {
-    uint32_t P = t.j[R], Q = t.m[R];
-    scan_lanes_fwd(P, Q);
-    const uint32_t out = max(P, min(halo<R + 1>(t.hLa, t.hLb), Q));
-    tile_set<R>(t, out, ch);
-}
-template <int R>
-DEV void tile_xb_row(Tile& t, bool& ch, int lane) {
-    uint32_t P = t.j[R], Q = t.m[R];
-    scan_lanes_bwd(P, Q, lane);
-    const uint32_t out = max(P, min(halo<R + 1>(t.hRa, t.hRb), Q));
-    tile_set<R>(t, out, ch);
-}
-template <int... Rs>
-DEV void tile_xf_all(Tile& t, bool& ch, std::integer_sequence<int, Rs...>) {
-    (tile_xf_row<Rs>(t, ch), ...);
-}
-template <int... Rs>
-DEV void tile_xb_all(Tile& t, bool& ch, int lane, std::integer_sequence<int, Rs...>) {
-    (tile_xb_row<Rs>(t, ch, lane), ...);
-}
-
-// Downward column sweep over the tile; K is the halo index of row R-1.
-template <int CONN, int R>
-DEV void tile_yd_row(Tile& t, uint32_t& prev, bool& ch) {
-    uint32_t mx = prev;
-    if constexpr (CONN == 8) {
-        mx = max(mx, from_left(prev, halo<R>(t.hLa, t.hLb)));
-        mx = max(mx, from_right(prev, halo<R>(t.hRa, t.hRb)));
+// Column sweep over the interior, lane = column.  DIR +1: top to bottom.
+// 8-connectivity adds the previous row's left/right neighbours (DPP), lanes 0
+// and 63 taking them from the halo columns.
+template <int CONN, int DIR>
+DEV bool sweep_cols(uint32_t* s, int lane) {
+    int idx = DIR > 0 ? tix(-1, lane) : tix(64, lane);
+    int hl = DIR > 0 ? tix(-1, -1) : tix(64, -1);
+    int hr = DIR > 0 ? tix(-1, 64) : tix(64, 64);
+    uint32_t p = s[idx];
+    bool ch = false;
+#pragma unroll 16
+    for (int k = 0; k < kMP; ++k) {
+        idx += DIR * TS;
+        uint32_t q = p;
+        if constexpr (CONN == 8) {
+            q = pmax(q, pmax(from_left(p, s[hl]), from_right(p, s[hr])));
+            hl += DIR * TS;
+            hr += DIR * TS;
+        }
+        const uint32_t w = s[idx];
+        const uint32_t nw = upd(w, q);
+        ch |= nw != w;
+        s[idx] = nw;
+        p = nw;
     }
-    const uint32_t out = max(t.j[R], min(mx, t.m[R]));
-    tile_set<R>(t, out, ch);
-    prev = out;
-}
-template <int CONN, int R>
-DEV void tile_yu_row(Tile& t, uint32_t& prev, bool& ch) {
-    uint32_t mx = prev;
-    if constexpr (CONN == 8) {
-        mx = max(mx, from_left(prev, halo<R + 2>(t.hLa, t.hLb)));
-        mx = max(mx, from_right(prev, halo<R + 2>(t.hRa, t.hRb)));
-    }
-    const uint32_t out = max(t.j[R], min(mx, t.m[R]));
-    tile_set<R>(t, out, ch);
-    prev = out;
-}
-template <int CONN, int... Rs>
-DEV void tile_yd_all(Tile& t, bool& ch, std::integer_sequence<int, Rs...>) {
-    uint32_t prev = t.hT;
-    (tile_yd_row<CONN, Rs>(t, prev, ch), ...);
-}
-template <int CONN, int... Rs>
-DEV void tile_yu_all(Tile& t, bool& ch, std::integer_sequence<int, Rs...>) {
-    uint32_t prev = t.hB;
-    (tile_yu_row<CONN, kMP - 1 - Rs>(t, prev, ch), ...);
+    return ch;
 }
 
-using Rows = std::make_integer_sequence<int, kMP>;
-
-// Local sweeps until nothing changes (or max_iters rounds).  Returns true when
-// the tile converged.
+// Sweeps until the four directions in a row change nothing (a sweep is
+// idempotent, so three quiet sweeps after a change suffice), or max_sweeps.
 template <int CONN>
-DEV bool tile_converge(Tile& t, int max_iters) {
+DEV bool tile_converge(uint32_t* s, int max_sweeps) {
     const int lane = lane_id();
-    for (int it = 0; it < max_iters; ++it) {
-        bool ch = false;
-        tile_xf_all(t, ch, Rows{});
-        tile_yd_all<CONN>(t, ch, Rows{});
-        tile_xb_all(t, ch, lane, Rows{});
-        tile_yu_all<CONN>(t, ch, Rows{});
-        if (!wave_any(ch)) return true;
+    int quiet = -1;
+    for (int k = 0; k < max_sweeps; ++k) {
+        bool ch;
+        switch (k & 3) {
+            case 0: ch = sweep_rows<+1>(s, lane); break;
+            case 1: ch = sweep_cols<CONN, +1>(s, lane); break;
+            case 2: ch = sweep_rows<-1>(s, lane); break;
+            default: ch = sweep_cols<CONN, -1>(s, lane); break;
+        }
+        wave_lds_sync();
+        if (wave_any(ch)) quiet = 0;
+        else if (++quiet >= 3) return true;
     }
     return false;
 }
@@ -657,101 +715,6 @@ enum : uint32_t {
     NB_SELF = 1u << 0, NB_T = 1u << 1, NB_B = 1u << 2, NB_L = 1u << 3, NB_R = 1u << 4,
     NB_TL = 1u << 5, NB_TR = 1u << 6, NB_BL = 1u << 7, NB_BR = 1u << 8
 };
-
-// Which neighbours hold a pixel q next to a pixel p of this tile that rose in
-// this visit, with min(J(p), I(q)) > J(q) on the fetched halo values.
-template <int CONN>
-DEV uint32_t tile_notify_mask(const Tile& t, const Image& im, int mx, int my) {
-    const int lane = lane_id();
-    const int x0 = mx * kMP, y0 = my * kMP;
-    const bool hasT = y0 > 0, hasB = y0 + kMP < im.H, hasL = x0 > 0, hasR = x0 + kMP < im.W;
-    uint32_t nb = 0;
-    if (hasT) {
-        const uint32_t p = t.j[0];
-        const bool up = tile_rose<0>(t);
-        bool c = up && min(p, t.iT) > t.hT;
-        if constexpr (CONN == 8) {
-            const uint32_t hl = from_left(t.hT, 0u), il = from_left(t.iT, 0u);
-            const uint32_t hr = from_right(t.hT, 0u), ir = from_right(t.iT, 0u);
-            c |= up && lane > 0 && min(p, il) > hl;
-            c |= up && lane < 63 && min(p, ir) > hr;
-        }
-        if (wave_any(c)) nb |= NB_T;
-    }
-    if (hasB) {
-        const uint32_t p = t.j[kMP - 1];
-        const bool up = tile_rose<kMP - 1>(t);
-        bool c = up && min(p, t.iB) > t.hB;
-        if constexpr (CONN == 8) {
-            const uint32_t hl = from_left(t.hB, 0u), il = from_left(t.iB, 0u);
-            const uint32_t hr = from_right(t.hB, 0u), ir = from_right(t.iB, 0u);
-            c |= up && lane > 0 && min(p, il) > hl;
-            c |= up && lane < 63 && min(p, ir) > hr;
-        }
-        if (wave_any(c)) nb |= NB_B;
-    }
-    // Left/right: gather column 0 / column 63 into lane r (row r) so every lane
-    // tests one row against the halo column held in lane r.
-    if (hasL || hasR) {
-        uint32_t cl = 0, cr = 0;
-#pragma unroll
-        for (int r = 0; r < kMP; ++r) {
-            const uint32_t a = rdlane(t.j[r], 0);
-            const uint32_t b = rdlane(t.j[r], 63);
-            if (lane == r) { cl = a; cr = b; }
-        }
-        const bool upL = rose_bit(rdlane(t.roseLo, 0), rdlane(t.roseHi, 0), lane);
-        const bool upR = rose_bit(rdlane(t.roseLo, 63), rdlane(t.roseHi, 63), lane);
-        if (hasL) {
-            const bool up = upL;
-            // lane r: halo row r is at index r+1 -> register a, lane r+1
-            const uint32_t h = from_right(t.hLa, rdlane(t.hLb, 0));
-            const uint32_t i = from_right(t.iLa, rdlane(t.iLb, 0));
-            bool c = up && min(cl, i) > h;
-            if constexpr (CONN == 8) {
-                // rows r-1 (index r) and r+1 (index r+2), kept inside rows 0..63
-                const uint32_t h0 = t.hLa, i0 = t.iLa;
-                const uint32_t h2 = from_right(h, rdlane(t.hLb, 1));
-                const uint32_t i2 = from_right(i, rdlane(t.iLb, 1));
-                c |= up && lane > 0 && min(cl, i0) > h0;
-                c |= up && lane < 63 && min(cl, i2) > h2;
-            }
-            if (wave_any(c)) nb |= NB_L;
-        }
-        if (hasR) {
-            const bool up = upR;
-            const uint32_t h = from_right(t.hRa, rdlane(t.hRb, 0));
-            const uint32_t i = from_right(t.iRa, rdlane(t.iRb, 0));
-            bool c = up && min(cr, i) > h;
-            if constexpr (CONN == 8) {
-                const uint32_t h0 = t.hRa, i0 = t.iRa;
-                const uint32_t h2 = from_right(h, rdlane(t.hRb, 1));
-                const uint32_t i2 = from_right(i, rdlane(t.iRb, 1));
-                c |= up && lane > 0 && min(cr, i0) > h0;
-                c |= up && lane < 63 && min(cr, i2) > h2;
-            }
-            if (wave_any(c)) nb |= NB_R;
-        }
-    }
-    if constexpr (CONN == 8) {
-        const uint32_t j00 = rdlane(t.j[0], 0), j063 = rdlane(t.j[0], 63);
-        const uint32_t j630 = rdlane(t.j[kMP - 1], 0), j6363 = rdlane(t.j[kMP - 1], 63);
-        const bool u00 = rdlane(t.roseLo, 0) & 1u, u063 = rdlane(t.roseLo, 63) & 1u;
-        const bool u630 = rdlane(t.roseHi, 0) >> 31, u6363 = rdlane(t.roseHi, 63) >> 31;
-        if (hasT && hasL && u00 && min(j00, halo<0>(t.iLa, t.iLb)) > halo<0>(t.hLa, t.hLb))
-            nb |= NB_TL;
-        if (hasT && hasR && u063 &&
-            min(j063, halo<0>(t.iRa, t.iRb)) > halo<0>(t.hRa, t.hRb))
-            nb |= NB_TR;
-        if (hasB && hasL && u630 &&
-            min(j630, halo<65>(t.iLa, t.iLb)) > halo<65>(t.hLa, t.hLb))
-            nb |= NB_BL;
-        if (hasB && hasR && u6363 &&
-            min(j6363, halo<65>(t.iRa, t.iRb)) > halo<65>(t.hRa, t.hRb))
-            nb |= NB_BR;
-    }
-    return nb;
-}
 
 DEV int neighbour_index(int m, int nmx, uint32_t bit) {
     switch (bit) {
@@ -767,124 +730,182 @@ DEV int neighbour_index(int m, int nmx, uint32_t bit) {
     }
 }
 
-// Write back the rows that rose.  EXCL: the visitor is the only writer of its
-// pixels (plain stores).  Otherwise other visitors of the same megapixel may
-// write concurrently: 32-bit compare-and-swap of the per-pixel max.
-// Returns true when a pixel in memory rose.
+// Writes back the pixels that rose, reading the marker again (in the
+// exclusive variants it still holds the fetched values).  EXCL: this wave is
+// the only writer of the megapixel (plain 32-bit stores).  Otherwise a
+// duplicate visitor may write concurrently: 32-bit compare-and-swap of the
+// per-pixel max.  Records which border pixels rose (rose[0..63] top row,
+// [64..127] bottom row, [128..191] left column, [192..255] right column).
+// Returns true when some pixel in memory rose.
 template <class T, bool EXCL>
-DEV bool tile_store(const Tile& t, const Image& im, int mx, int my) {
+DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, int my) {
+    constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
+    constexpr int BITS = 8 * (int)sizeof(T);
+    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
     const int lane = lane_id();
-    const int x = mx * kMP + lane, y0 = my * kMP;
-    bool rose = false;
-    if constexpr (EXCL) {
+    const int x0 = mx * kMP, y0 = my * kMP;
+    const int wl = lane % WPR, rl = lane / WPR;
+    const int xw = x0 + wl * PPW;
+    reinterpret_cast<uint32_t*>(rose)[lane] = 0u;
+    wave_lds_sync();
+    bool any = false;
+    if (xw < im.W) {
+        constexpr int NIT = kMP / RPI;
+        uint32_t gv[NIT];
 #pragma unroll
-        for (int r = 0; r < kMP; ++r) {
-            if (rose_bit(t.roseLo, t.roseHi, r)) {
-                *reinterpret_cast<T*>(im.J + (size_t)(y0 + r) * im.pj + (size_t)x * sizeof(T)) =
-                    (T)t.j[r];
-                rose = true;
-            }
-        }
-    } else {
-        constexpr int PPW = 4 / (int)sizeof(T), BITS = 8 * (int)sizeof(T);
-        constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
-        // Lanes are pixels; the word holding pixel x starts at pixel x & ~(PPW-1).
-        // The lane owning the word's first pixel (or the first in-tile pixel of a
-        // word that straddles the tile's left edge) gathers its PPW pixels.
-        const int xw = x & ~(PPW - 1);
-        const int sub = x - xw;
-        const bool leader = (sub == 0) || lane == 0;
-        uint32_t rlo[PPW], rhi[PPW];
-#pragma unroll
-        for (int k = 0; k < PPW; ++k) {
-            const int src = lane - sub + k;
-            const bool valid = src >= 0 && src < 64;
-            rlo[k] = valid ? __shfl(t.roseLo, src & 63) : 0u;
-            rhi[k] = valid ? __shfl(t.roseHi, src & 63) : 0u;
+        for (int it = 0; it < NIT; ++it) {  // all re-reads in flight together
+            const int yc = min(y0 + it * RPI + rl, im.H - 1);
+            gv[it] = ld_coh(im.J + (size_t)yc * im.pj + (size_t)xw * sizeof(T));
         }
 #pragma unroll
-        for (int r = 0; r < kMP; ++r) {
-            // new value and rise flag of pixel xw+k, taken from lane (lane - sub + k)
-            uint32_t nv[PPW];
-            bool any_up = false;
+        for (int it = 0; it < NIT; ++it) {
+            const int row = it * RPI + rl;
+            const int y = y0 + row;
+            if (y >= im.H) continue;
+            uint32_t* wp = reinterpret_cast<uint32_t*>(im.J + (size_t)y * im.pj + (size_t)xw * sizeof(T));
+            const uint32_t g = gv[it];
+            uint32_t f = g, up = 0;
 #pragma unroll
             for (int k = 0; k < PPW; ++k) {
-                const int src = lane - sub + k;
-                const uint32_t v = __shfl(t.j[r], src & 63);
-                const bool valid = src >= 0 && src < 64;
-                nv[k] = valid ? v : 0u;
-                any_up |= rose_bit(rlo[k], rhi[k], r);
+                const uint32_t v = s[tix(row, wl * PPW + k)] & 0xFFFFu;
+                const uint32_t old = (g >> (BITS * k)) & PM;
+                if (xw + k < im.W && v > old) {
+                    f = (f & ~(PM << (BITS * k))) | (v << (BITS * k));
+                    up |= 1u << k;
+                }
             }
-            if (leader && any_up && x < im.W && y0 + r < im.H) {
-                uint32_t* wp = reinterpret_cast<uint32_t*>(im.J + (size_t)(y0 + r) * im.pj +
-                                                           (size_t)xw * sizeof(T));
-                uint32_t cur = *wp;
-                while (true) {
-                    uint32_t want = cur;
+            if (up) {
+                if constexpr (EXCL) {
+                    st_coh(wp, f);
+                } else {
+                    uint32_t cur = g;
+                    while (true) {
+                        uint32_t want = cur;
 #pragma unroll
-                    for (int k = 0; k < PPW; ++k) {
-                        const uint32_t c = (cur >> (BITS * k)) & PM;
-                        if (nv[k] > c) want = (want & ~(PM << (BITS * k))) | (nv[k] << (BITS * k));
+                        for (int k = 0; k < PPW; ++k) {
+                            const uint32_t a = (cur >> (BITS * k)) & PM, b = (f >> (BITS * k)) & PM;
+                            if (b > a) want = (want & ~(PM << (BITS * k))) | (b << (BITS * k));
+                        }
+                        if (want == cur) break;
+                        const uint32_t seen = a_cas(wp, cur, want);
+                        if (seen == cur) break;
+                        cur = seen;
                     }
-                    if (want == cur) break;
-                    const uint32_t seen = a_cas(wp, cur, want);
-                    if (seen == cur) { rose = true; break; }
-                    cur = seen;
+                }
+                any = true;
+#pragma unroll
+                for (int k = 0; k < PPW; ++k) {
+                    if (!((up >> k) & 1u)) continue;
+                    const int c = wl * PPW + k;
+                    if (row == 0) rose[c] = 1;
+                    if (row == kMP - 1) rose[64 + c] = 1;
+                    if (c == 0) rose[128 + row] = 1;
+                    if (c == kMP - 1) rose[192 + row] = 1;
                 }
             }
         }
     }
-    return wave_any(rose);
+    wave_lds_sync();
+    return wave_any(any);
+}
+
+// Which neighbours hold a pixel q next to a pixel p of this megapixel that
+// rose in this visit, with min(J(p), I(q)) > J(q) on the fetched halo.
+template <int CONN>
+DEV uint32_t tile_notify_mask(const uint32_t* s, const uint8_t* rose, const Image& im, int mx,
+                              int my) {
+    const int lane = lane_id();
+    const int x0 = mx * kMP, y0 = my * kMP;
+    const bool hasT = y0 > 0, hasB = y0 + kMP < im.H, hasL = x0 > 0, hasR = x0 + kMP < im.W;
+    uint32_t nb = 0;
+    // top / bottom rows: lane = column
+    bool t = false, b = false, l = false, r = false;
+    if (hasT && rose[lane]) {
+        const uint32_t p = s[tix(0, lane)] & 0xFFFFu;
+        t = raises(p, s[tix(-1, lane)]);
+        if (CONN == 8) {
+            if (lane > 0) t |= raises(p, s[tix(-1, lane - 1)]);
+            if (lane < 63) t |= raises(p, s[tix(-1, lane + 1)]);
+        }
+    }
+    if (hasB && rose[64 + lane]) {
+        const uint32_t p = s[tix(63, lane)] & 0xFFFFu;
+        b = raises(p, s[tix(64, lane)]);
+        if (CONN == 8) {
+            if (lane > 0) b |= raises(p, s[tix(64, lane - 1)]);
+            if (lane < 63) b |= raises(p, s[tix(64, lane + 1)]);
+        }
+    }
+    // left / right columns: lane = row
+    if (hasL && rose[128 + lane]) {
+        const uint32_t p = s[tix(lane, 0)] & 0xFFFFu;
+        l = raises(p, s[tix(lane, -1)]);
+        if (CONN == 8) {
+            if (lane > 0) l |= raises(p, s[tix(lane - 1, -1)]);
+            if (lane < 63) l |= raises(p, s[tix(lane + 1, -1)]);
+        }
+    }
+    if (hasR && rose[192 + lane]) {
+        const uint32_t p = s[tix(lane, 63)] & 0xFFFFu;
+        r = raises(p, s[tix(lane, 64)]);
+        if (CONN == 8) {
+            if (lane > 0) r |= raises(p, s[tix(lane - 1, 64)]);
+            if (lane < 63) r |= raises(p, s[tix(lane + 1, 64)]);
+        }
+    }
+    if (wave_any(t)) nb |= NB_T;
+    if (wave_any(b)) nb |= NB_B;
+    if (wave_any(l)) nb |= NB_L;
+    if (wave_any(r)) nb |= NB_R;
+    if constexpr (CONN == 8) {
+        if (hasT && hasL && rose[0] && raises(s[tix(0, 0)] & 0xFFFFu, s[tix(-1, -1)])) nb |= NB_TL;
+        if (hasT && hasR && rose[63] && raises(s[tix(0, 63)] & 0xFFFFu, s[tix(-1, 64)])) nb |= NB_TR;
+        if (hasB && hasL && rose[64] && raises(s[tix(63, 0)] & 0xFFFFu, s[tix(64, -1)])) nb |= NB_BL;
+        if (hasB && hasR && rose[127] && raises(s[tix(63, 63)] & 0xFFFFu, s[tix(64, 64)]))
+            nb |= NB_BR;
+    }
+    return nb;
 }
 
 // ------------------------------------------------------------ seeding -------
 // flags[m] = 1 when megapixel m holds a pixel p with a neighbour q such that
-// min(J(q), I(p)) > J(p).  One wave per megapixel.
+// min(J(q), I(p)) > J(p).  One wave per megapixel (LDS tile, lane = column,
+// a three-row window in registers).
 template <class T, int CONN>
 __global__ __launch_bounds__(kWG) void k_unstable(Image im, int nmx, int nmy, uint8_t* flags,
                                                    uint32_t* count) {
-    const int lane = lane_id();
+    extern __shared__ uint32_t lds[];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    uint32_t* s = lds + wv * (WAVE_LDS / 4);
     const int wid = (blockIdx.x * kWG + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * kWG) >> 6;
     for (int m = wid; m < nmx * nmy; m += nwaves) {
         const int mx = m % nmx, my = m / nmx;
-        Tile t;
-        tile_fetch<T>(im, mx, my, t);
+        tile_fetch<T>(im, mx, my, s);
         bool un = false;
-#pragma unroll
+        uint32_t up = s[tix(-1, lane)], cur = s[tix(0, lane)];
         for (int r = 0; r < kMP; ++r) {
-            const uint32_t up = r > 0 ? t.j[r - 1] : t.hT;
-            const uint32_t dn = r < kMP - 1 ? t.j[r + 1] : t.hB;
-            uint32_t mx4 = max(up, dn);
-            uint32_t lft, rgt;
-            if (r + 1 < 64) {
-                lft = from_left(t.j[r], rdlane(t.hLa, r + 1));
-                rgt = from_right(t.j[r], rdlane(t.hRa, r + 1));
-            } else {
-                lft = from_left(t.j[r], rdlane(t.hLb, r + 1 - 64));
-                rgt = from_right(t.j[r], rdlane(t.hRb, r + 1 - 64));
-            }
-            mx4 = max(mx4, max(lft, rgt));
+            const uint32_t dn = s[tix(r + 1, lane)];
+            const uint32_t j = cur & 0xFFFFu;
+            uint32_t nbm = max(up & 0xFFFFu, dn & 0xFFFFu);
+            nbm = max(nbm, from_left(cur, s[tix(r, -1)]) & 0xFFFFu);
+            nbm = max(nbm, from_right(cur, s[tix(r, 64)]) & 0xFFFFu);
             if constexpr (CONN == 8) {
-                uint32_t ul, ur, dl, drr;
-                ul = from_left(up, r < 64 ? rdlane(t.hLa, r) : 0u);
-                ur = from_right(up, r < 64 ? rdlane(t.hRa, r) : 0u);
-                if (r + 2 < 64) {
-                    dl = from_left(dn, rdlane(t.hLa, r + 2));
-                    drr = from_right(dn, rdlane(t.hRa, r + 2));
-                } else {
-                    dl = from_left(dn, rdlane(t.hLb, r + 2 - 64));
-                    drr = from_right(dn, rdlane(t.hRb, r + 2 - 64));
-                }
-                mx4 = max(mx4, max(max(ul, ur), max(dl, drr)));
+                nbm = max(nbm, from_left(up, s[tix(r - 1, -1)]) & 0xFFFFu);
+                nbm = max(nbm, from_right(up, s[tix(r - 1, 64)]) & 0xFFFFu);
+                nbm = max(nbm, from_left(dn, s[tix(r + 1, -1)]) & 0xFFFFu);
+                nbm = max(nbm, from_right(dn, s[tix(r + 1, 64)]) & 0xFFFFu);
             }
-            un |= min(mx4, t.m[r]) > t.j[r];
+            un |= min(nbm, cur >> 16) > j;
+            up = cur;
+            cur = dn;
         }
         const bool u = wave_any(un);
         if (lane == 0) {
             flags[m] = u ? 1 : 0;
             if (u && count) atomicAdd(count, 1u);
         }
+        wave_lds_sync();  // the tile is reused by the next megapixel
     }
 }
 
@@ -894,10 +915,10 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
-__global__ void k_iota_u32(uint32_t* p, size_t n) {
+__global__ void k_ring_init(unsigned long long* ring, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
-        p[i] = (uint32_t)i;
+        ring[i] = (unsigned long long)i << 32;  // slot i expects push ticket i
 }
 
 // Compacts the unstable flags into queue A (ballot + popcount inside a wave,
@@ -921,10 +942,10 @@ __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nm
         if (f) {
             const uint32_t pos = start + off;
             if (pos < q.cap) {
-                q.qa[pos] = (uint32_t)m;
+                if (kind != Q_MUTEX) q.qa[pos] = (uint32_t)m;
                 if (kind == Q_MUTEX) {
                     q.st[m] = ST_PENDING;
-                    q.seq[pos] = pos + 1;
+                    q.ring[pos] = ((unsigned long long)(pos + 1) << 32) | (uint32_t)m;
                 } else if (kind == Q_UQ) {
                     q.st[m] = 1u;
                 }
@@ -932,6 +953,29 @@ __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nm
         }
     }
 }
+
+// ------------------------------------------------------------ phase timers ---
+// Per-wave accumulators of wall-clock ticks, added to the control block when
+// the wave leaves (only when QueueArgs::trace is set).
+struct PhaseTimer {
+    unsigned long long acc[5] = {0, 0, 0, 0, 0};
+    unsigned long long t = 0;
+    bool on;
+    __device__ explicit PhaseTimer(bool enabled) : on(enabled) {
+        if (on) t = wall_clock64();
+    }
+    __device__ void lap(int k) {
+        if (!on) return;
+        const unsigned long long n = wall_clock64();
+        acc[k] += n - t;
+        t = n;
+    }
+    __device__ void flush(uint32_t* ctl) {
+        if (!on || lane_id() != 0) return;
+        for (int k = 0; k < 5; ++k)
+            atomicAdd(reinterpret_cast<unsigned long long*>(ctl_at(ctl, C_PH_POP + k)), acc[k]);
+    }
+};
 
 // ----------------------------------------------------------- timeouts -------
 DEV unsigned long long now_ticks() { return wall_clock64(); }
@@ -976,21 +1020,22 @@ DEV bool ring_push_many(const QueueArgs& q, bool mine, uint32_t value) {
         const uint32_t t = start + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
         const uint32_t slot = t & (q.cap - 1u);
         Waiter w(q.timeout_ticks);
-        while (a_load(&q.seq[slot]) != t) {
+        while ((uint32_t)(a_load64(&q.ring[slot]) >> 32) != t) {
             if (!w.step(q.ctl)) { ok = false; break; }
         }
-        if (ok) {
-            q.qa[slot] = value;
-            a_store(&q.seq[slot], t + 1u);
-        }
+        if (ok) a_store64(&q.ring[slot], ((unsigned long long)(t + 1u) << 32) | value);
     }
     return !wave_any(!ok);
 }
 
 template <class T, int CONN>
 __global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
+    extern __shared__ uint32_t lds[];
     const int lane = lane_id();
+    uint32_t* s = lds + (threadIdx.x >> 6) * (WAVE_LDS / 4);
+    uint8_t* rose = reinterpret_cast<uint8_t*>(s + TILE_WORDS);
     unsigned long long visits = 0;
+    PhaseTimer pt(q.trace != 0);
     while (true) {
         // pop
         uint32_t h = 0;
@@ -998,31 +1043,37 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
         h = first_lane(h);
         const uint32_t slot = h & (q.cap - 1u);
         int state = 0;  // 1 = got an entry, 2 = done, 3 = timeout
+        uint32_t got = 0;
         if (lane == 0) {
             Waiter w(q.timeout_ticks);
             while (true) {
-                if (a_load(&q.seq[slot]) == h + 1u) { state = 1; break; }
+                const unsigned long long e = a_load64(&q.ring[slot]);
+                if ((uint32_t)(e >> 32) == h + 1u) { state = 1; got = (uint32_t)e; break; }
                 if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
-                if (a_load_rlx(ctl_at(q.ctl, C_TIMEOUT)) != 0u || !w.step(q.ctl)) { state = 3; break; }
+                if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u || !w.step(q.ctl)) { state = 3; break; }
             }
         }
         state = (int)first_lane((uint32_t)state);
         if (state != 1) break;
         uint32_t m = 0;
         if (lane == 0) {
-            m = q.qa[slot];
-            a_store(&q.seq[slot], h + q.cap);
+            m = got;
+            a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
             a_xchg(&q.st[m], ST_RUNNING);
         }
         m = first_lane(m);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        mem_wait();
+        pt.lap(0);
         const int mx = (int)m % q.nmx, my = (int)m / q.nmx;
-        Tile t;
-        tile_fetch<T>(im, mx, my, t);
-        const bool conv = tile_converge<CONN>(t, q.max_iters);
-        tile_store<T, true>(t, im, mx, my);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        uint32_t nb = tile_notify_mask<CONN>(t, im, mx, my);
+        tile_fetch<T>(im, mx, my, s);
+        pt.lap(1);
+        const bool conv = tile_converge<CONN>(s, 4 * q.max_iters);
+        pt.lap(2);
+        tile_store<T, true>(s, rose, im, mx, my);
+        mem_wait();
+        pt.lap(3);
+        uint32_t nb = tile_notify_mask<CONN>(s, rose, im, mx, my);
+        wave_lds_sync();
         if (!conv) nb |= NB_SELF;
         ++visits;
         // notify neighbours: lane b handles bit b
@@ -1045,8 +1096,11 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
         } else if (lane == 0) {
             a_add(ctl_at(q.ctl, C_PENDING), 0xFFFFFFFFu);
         }
+        pt.lap(4);
         if (!ok) break;
     }
+    pt.lap(0);
+    pt.flush(q.ctl);
     if (lane == 0 && visits) {
         atomicAdd(reinterpret_cast<unsigned long long*>(ctl_at(q.ctl, C_VISITS_LO)), visits);
     }
@@ -1057,7 +1111,7 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
 // claiming by atomicAdd), appends to the write queue, then a grid barrier,
 // a queue flip by one thread, and a second barrier.
 DEV bool grid_barrier(uint32_t* ctl, uint32_t nblocks, unsigned long long limit) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    mem_wait();
     __syncthreads();
     __shared__ int ok_s;
     if (threadIdx.x == 0) {
@@ -1065,7 +1119,8 @@ DEV bool grid_barrier(uint32_t* ctl, uint32_t nblocks, unsigned long long limit)
         const uint32_t gen = a_load(ctl_at(ctl, C_BAR_GEN));
         const uint32_t arrived = a_add(ctl_at(ctl, C_BAR_COUNT), 1u);
         if (arrived == nblocks - 1u) {
-            a_store_rlx(ctl_at(ctl, C_BAR_COUNT), 0u);
+            a_store(ctl_at(ctl, C_BAR_COUNT), 0u);
+            mem_wait();
             a_store(ctl_at(ctl, C_BAR_GEN), gen + 1u);
         } else {
             Waiter w(limit);
@@ -1075,15 +1130,19 @@ DEV bool grid_barrier(uint32_t* ctl, uint32_t nblocks, unsigned long long limit)
         }
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    mem_wait();
     return ok_s != 0;
 }
 
 template <class T, int CONN, int KIND>
 __global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
     constexpr bool EXCL = KIND == Q_UQ;
+    extern __shared__ uint32_t lds[];
     const int lane = lane_id();
+    uint32_t* s = lds + (threadIdx.x >> 6) * (WAVE_LDS / 4);
+    uint8_t* rose = reinterpret_cast<uint8_t*>(s + TILE_WORDS);
     unsigned long long visits = 0;
+    PhaseTimer pt(q.trace != 0);
     while (true) {
         const uint32_t round = a_load(ctl_at(q.ctl, C_ROUND));
         const uint32_t nr = a_load(ctl_at(q.ctl, C_RSIZE));
@@ -1096,21 +1155,25 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
             if (idx >= nr) break;
             uint32_t m = 0;
             if (lane == 0) {
-                m = rq[idx];
+                m = ld_coh(&rq[idx]);
                 if (EXCL) a_xchg(&q.st[m], 0u);  // may be queued again for the next round
             }
             m = first_lane(m);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            mem_wait();
+            pt.lap(0);
             const int mx = (int)m % q.nmx, my = (int)m / q.nmx;
-            Tile t;
-            tile_fetch<T>(im, mx, my, t);
-            const bool conv = tile_converge<CONN>(t, q.max_iters);
-            const bool rose = tile_store<T, EXCL>(t, im, mx, my);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            uint32_t nb = tile_notify_mask<CONN>(t, im, mx, my);
+            tile_fetch<T>(im, mx, my, s);
+            pt.lap(1);
+            const bool conv = tile_converge<CONN>(s, 4 * q.max_iters);
+            pt.lap(2);
+            const bool rose_any = tile_store<T, EXCL>(s, rose, im, mx, my);
+            mem_wait();
+            pt.lap(3);
+            uint32_t nb = tile_notify_mask<CONN>(s, rose, im, mx, my);
+            wave_lds_sync();
             // DQ: a duplicate visitor of the same megapixel may have used stale
             // pixels, so any rise re-queues the megapixel (as the reference does).
-            if (!conv || (!EXCL && rose)) nb |= NB_SELF;
+            if (!conv || (!EXCL && rose_any)) nb |= NB_SELF;
             ++visits;
             bool push = false;
             uint32_t target = 0;
@@ -1126,18 +1189,19 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
                 start = first_lane(start);
                 if (push) {
                     const uint32_t pos = start + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-                    if (pos < q.cap) wq[pos] = target;
-                    else a_store_rlx(ctl_at(q.ctl, C_OVERFLOW), 1u);
+                    if (pos < q.cap) st_coh(&wq[pos], target);
+                    else a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
                 }
             }
+            pt.lap(4);
         }
         if (!grid_barrier(q.ctl, gridDim.x, q.timeout_ticks)) break;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             uint32_t ws = a_load(ctl_at(q.ctl, C_WSIZE));
             if (ws > q.cap) ws = q.cap;
-            a_store_rlx(ctl_at(q.ctl, C_RSIZE), ws);
-            a_store_rlx(ctl_at(q.ctl, C_WSIZE), 0u);
-            a_store_rlx(ctl_at(q.ctl, C_CURSOR), 0u);
+            a_store(ctl_at(q.ctl, C_RSIZE), ws);
+            a_store(ctl_at(q.ctl, C_WSIZE), 0u);
+            a_store(ctl_at(q.ctl, C_CURSOR), 0u);
             a_store(ctl_at(q.ctl, C_ROUND), round + 1u);
         }
         if (!grid_barrier(q.ctl, gridDim.x, q.timeout_ticks)) break;
@@ -1146,6 +1210,8 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
         if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u) break;
         if ((int)round + 1 >= q.max_rounds) break;
     }
+    pt.lap(0);
+    pt.flush(q.ctl);
     if (lane == 0 && visits) {
         atomicAdd(reinterpret_cast<unsigned long long*>(ctl_at(q.ctl, C_VISITS_LO)), visits);
     }
@@ -1157,9 +1223,9 @@ __global__ void k_prepare_queue(QueueArgs q) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t n = a_load(ctl_at(q.ctl, C_TAIL));
         const uint32_t k = n < q.cap ? n : q.cap;
-        a_store_rlx(ctl_at(q.ctl, C_RSIZE), k);
-        a_store_rlx(ctl_at(q.ctl, C_PENDING), k);
-        if (n > q.cap) a_store_rlx(ctl_at(q.ctl, C_OVERFLOW), 1u);
+        a_store(ctl_at(q.ctl, C_RSIZE), k);
+        a_store(ctl_at(q.ctl, C_PENDING), k);
+        if (n > q.cap) a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
         a_store(ctl_at(q.ctl, C_TAIL), k);
     }
 }
@@ -1231,10 +1297,24 @@ hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask
     return hipGetLastError();
 }
 
+// 16-byte lanes need 16-byte aligned rows; narrow images use 4-byte lanes so
+// that there are enough column strips to fill the GPU.
+static bool col_wide(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
+                     size_t pi, int W, int dtype) {
+    return aligned16(src, ps) && aligned16(dst, pd) && aligned16(mask, pi) &&
+           (size_t)W * dtype >= 16384;
+}
+
 int col_block_width(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
-                    size_t pi, int dtype) {
-    const bool v16 = aligned16(src, ps) && aligned16(dst, pd) && aligned16(mask, pi);
-    return 64 * (v16 ? 16 : 4) / dtype;
+                    size_t pi, int W, int dtype) {
+    return 64 * (col_wide(src, dst, mask, ps, pd, pi, W, dtype) ? 16 : 4) / dtype;
+}
+
+int col_auto_rows(int W, int H, int dtype, bool wide) {
+    const long strips = ((long)W * dtype + 64 * (wide ? 16 : 4) - 1) / (64 * (wide ? 16 : 4));
+    long r = strips * (long)H / 8192;  // aim at >= 8192 waves
+    r = r / 32 * 32;
+    return (int)std::min<long>(512, std::max<long>(32, r));
 }
 
 template <class T, int VB>
@@ -1260,7 +1340,8 @@ static void col_pass_t(const uint8_t* src, uint8_t* dst, const uint8_t* mask, si
 hipError_t launch_col_pass(const uint8_t* src, uint8_t* dst, const uint8_t* mask, size_t ps,
                            size_t pd, size_t pi, int W, int H, int dtype, int conn, int dir,
                            int R, const TileMask& tm, uint32_t* changed, hipStream_t s) {
-    const bool v16 = aligned16(src, ps) && aligned16(dst, pd) && aligned16(mask, pi);
+    const bool v16 = col_wide(src, dst, mask, ps, pd, pi, W, dtype);
+    if (R <= 0) R = col_auto_rows(W, H, dtype, v16);
     if (dtype == 1) {
         if (v16) col_pass_t<uint8_t, 16>(src, dst, mask, ps, pd, pi, W, H, conn, dir, R, tm, changed, s);
         else col_pass_t<uint8_t, 4>(src, dst, mask, ps, pd, pi, W, H, conn, dir, R, tm, changed, s);
@@ -1277,7 +1358,7 @@ hipError_t launch_unstable(const Image& im, int dtype, int conn, uint8_t* flags,
     const int nmp = nmx * nmy;
     const int grid = std::min((nmp + 3) / 4, cu_count() * 8);
 #define IWPP_UN(TT, C) \
-    hipLaunchKernelGGL((k_unstable<TT, C>), grid, kWG, 0, s, im, nmx, nmy, flags, count)
+    hipLaunchKernelGGL((k_unstable<TT, C>), grid, kWG, 4 * WAVE_LDS, s, im, nmx, nmy, flags, count)
     if (dtype == 1) {
         if (conn == 8) IWPP_UN(uint8_t, 8);
         else IWPP_UN(uint8_t, 4);
@@ -1296,10 +1377,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_iota_u32(uint32_t* p, size_t n, hipStream_t s) {
+hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const int grid = (int)std::min<size_t>((n + kWG - 1) / kWG, (size_t)cu_count() * 8);
-    hipLaunchKernelGGL(k_iota_u32, grid, kWG, 0, s, p, n);
+    hipLaunchKernelGGL(k_ring_init, grid, kWG, 0, s, ring, n);
     return hipGetLastError();
 }
 
@@ -1323,7 +1404,8 @@ static hipError_t queue_phase_t(const Image& im, int kind, const QueueArgs& q, i
     else if (kind == Q_DQ) fn = (const void*)k_queue_bsp<T, CONN, Q_DQ>;
     else fn = (const void*)k_queue_bsp<T, CONN, Q_UQ>;
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWG, 0);
+    const size_t lds_bytes = 4 * WAVE_LDS;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWG, lds_bytes);
     if (e != hipSuccess) return e;
     if (per_cu <= 0) return hipErrorInvalidConfiguration;
     if (wpc > 0) per_cu = std::min(per_cu, wpc);
@@ -1333,11 +1415,11 @@ static hipError_t queue_phase_t(const Image& im, int kind, const QueueArgs& q, i
     Image imc = im;
     QueueArgs qc = q;
     if (kind == Q_MUTEX) {
-        hipLaunchKernelGGL((k_queue_mutex<T, CONN>), grid, kWG, 0, s, imc, qc);
+        hipLaunchKernelGGL((k_queue_mutex<T, CONN>), grid, kWG, lds_bytes, s, imc, qc);
         return hipGetLastError();
     }
     void* args[] = {&imc, &qc};
-    return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, 0, s);
+    return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, lds_bytes, s);
 }
 
 hipError_t launch_queue_phase(const Image& im, int dtype, int conn, int kind, const QueueArgs& q,

@@ -39,7 +39,8 @@ class IWPPError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("sweep_rounds", ctypes.c_int), ("col_block_rows", ctypes.c_int),
                 ("max_local_iters", ctypes.c_int), ("queue_capacity", ctypes.c_int),
-                ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double)]
+                ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double),
+                ("trace_phases", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -47,7 +48,10 @@ class Stats(ctypes.Structure):
                 ("sweep_rounds", ctypes.c_int), ("queue_rounds", ctypes.c_int),
                 ("restarts", ctypes.c_int), ("queue", ctypes.c_int), ("tile", ctypes.c_int),
                 ("sweeps_skipped_tiles", ctypes.c_int), ("ms_sweep", ctypes.c_float),
-                ("ms_seed", ctypes.c_float), ("ms_queue", ctypes.c_float)]
+                ("ms_seed", ctypes.c_float), ("ms_queue", ctypes.c_float),
+                ("us_wait", ctypes.c_double), ("us_fetch", ctypes.c_double),
+                ("us_converge", ctypes.c_double), ("us_store", ctypes.c_double),
+                ("us_notify", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -65,7 +69,7 @@ SIGNATURES = {
     "iwpp_clip": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _VP]),
     "iwpp_sweep_rows": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _VP, _VP]),
     "iwpp_sweep_cols": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _VP, _VP]),
-    "iwpp_sweep_cols_block_width": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I]),
+    "iwpp_sweep_cols_block_width": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I]),
     "iwpp_unstable_megapixels": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _VP, _VP, _VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
     "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
@@ -234,7 +238,8 @@ def sweep_cols(src, dst, mask, connectivity, direction, block_rows, changed=None
 
 def sweep_cols_block_width(src, dst, mask):
     return _raise_nonneg(lib().iwpp_sweep_cols_block_width(src.ptr, dst.ptr, mask.ptr, src.pitch,
-                                                           dst.pitch, mask.pitch, src.dtype))
+                                                           dst.pitch, mask.pitch, src.w,
+                                                           src.dtype))
 
 
 def _raise_nonneg(v):

@@ -40,5 +40,5 @@ def test_library_carries_gfx950_code():
 
 def test_default_options():
     o = iwpp.default_options()
-    assert o.sweep_rounds == -1 and o.col_block_rows == 256 and o.max_local_iters == 8
+    assert o.sweep_rounds == -1 and o.col_block_rows == 0 and o.max_local_iters == 8
     assert o.timeout_s == 60.0
