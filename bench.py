@@ -113,28 +113,38 @@ def main():
     ms_row = timed(lambda: iwpp.sweep_rows(work, mask), reset_marker)
     ms_col = timed(lambda: iwpp.sweep_cols(work, scratch, mask, case.connectivity, 1,
                                            opt.col_block_rows), reset_marker)
-    bytes_sweep = 3 * tsize * px
+    bytes_sweep = 3 * tsize * px  # algorithmic bytes of one directional pass
     phases = {k: statistics.mean(s[k] for s in stats) for k in ("ms_sweep", "ms_seed", "ms_queue")}
     counters = {k: stats[-1][k] for k in ("tile_visits", "seeded", "sweep_rounds", "queue_rounds",
                                           "restarts", "queue", "tile")}
-    dominant = max(phases, key=phases.get)
     used_queue = {0: "mutex", 1: "dq", 2: "uq"}[stats[-1]["queue"]]
-    kernel_names = {"ms_sweep": "k_row_pass+k_col_pass", "ms_seed": "k_unstable",
-                    "ms_queue": {"mutex": "k_queue_mutex", "dq": "k_queue_bsp<DQ>",
-                                 "uq": "k_queue_bsp<UQ>"}[used_queue]}
-    dom_ms = phases[dominant]
-    achieved = bytes_sweep / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
+    rounds = counters["sweep_rounds"]
+    visits = statistics.mean(s["tile_visits"] for s in stats)
+    # Per reconstruction: device time of each kernel family, its launches, and
+    # the algorithmic bytes one launch must move (3*sizeof(T) per pixel it owns:
+    # read marker, read mask, write marker; a megapixel visit owns 64*64 pixels).
+    kernels = {
+        "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>"):
+            (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize),
+        "k_col_pass": (2 * rounds * ms_col, 2 * rounds, bytes_sweep),
+        "k_row_pass": (rounds * ms_row, rounds, bytes_sweep),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    tot_ms, launches, alg_bytes = kernels[dom]
+    avg_ms = tot_ms / max(1, launches)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     roofline = {
         "bound": "hbm",
-        "kernel": kernel_names[dominant],
+        "kernel": dom,
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None,
         "traffic": None,
-        "algorithmic_bytes": bytes_sweep,
-        "basis": "3*sizeof(T) bytes per pixel (read marker, read mask, write marker) "
-                 "over the dominant phase's average device time",
+        "algorithmic_bytes_per_launch": alg_bytes,
+        "avg_launch_ms": avg_ms,
+        "basis": "3*sizeof(T) bytes per owned pixel per launch / average launch time "
+                 "(HIP events on the library's stream)",
     }
     sweep_kernels = {
         "row_pass_ms": ms_row,
@@ -143,6 +153,7 @@ def main():
         "col_pass_ms": ms_col,
         "col_pass_GBps": bytes_sweep / (ms_col * 1e-3) / 1e9,
         "col_pass_frac": bytes_sweep / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "per_reconstruction_ms": {k: v[0] for k, v in kernels.items()},
     }
 
     # --- CPU baseline: the oracle (a port of the sequential hybrid), one core
@@ -156,6 +167,8 @@ def main():
             t = time.perf_counter()
             want = oracle.reconstruct(case.marker, case.mask, case.connectivity)
             times.append(time.perf_counter() - t)
+            if sum(times) > 20.0:  # bounded sample: about 20 s of CPU work at most
+                break
         med = statistics.median(times)
         cpu = {"value": px / med / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
                "sample": f"full {case.name} image, median of {len(times)} runs "

@@ -200,3 +200,28 @@ def test_config2_full(torch_cuda, queue):
     st = iwpp.reconstruct(dj, di, c.connectivity, c.tile, queue)
     say("c2 stats", st)
     assert np.array_equal(dj.numpy(), want), st
+
+
+def _full_case_check(case, queues):
+    want = oracle.reconstruct(case.marker, case.mask, case.connectivity)
+    mask = up(case.mask)
+    for queue in queues:
+        say(case.name, queue)
+        dj = up(case.marker)
+        st = iwpp.reconstruct(dj, mask, case.connectivity, case.tile, queue)
+        got = dj.numpy()
+        assert np.array_equal(got, want), (queue, st, int((got != want).sum()))
+        # size-independent properties on the device result
+        assert (got <= case.mask).all() and (got >= np.minimum(case.marker, case.mask)).all()
+        again = up(got)
+        st2 = iwpp.reconstruct(again, mask, case.connectivity, case.tile, queue)
+        assert st2["tile_visits"] == 0 or np.array_equal(again.numpy(), got)
+        assert np.array_equal(again.numpy(), got)  # idempotent
+
+
+def test_config3_full(torch_cuda):
+    _full_case_check(W.c3(), ("mutex", "dq", "uq"))
+
+
+def test_config5_full(torch_cuda):
+    _full_case_check(W.c5(), ("mutex", "uq"))
