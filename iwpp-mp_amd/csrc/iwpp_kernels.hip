@@ -362,8 +362,12 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
     if (any_change && changed) *changed = 1u;
 }
 
-// Rows wider than the register-resident kernel: one direction per launch,
-// chunks streamed with a carry, every chunk stored.
+// Streaming row pass for wide rows: one chunk (256 lanes x VB bytes) in
+// registers at a time, carried along the row.  DIR = +1 / -1 sweeps one
+// direction; DIR = 0 sweeps forward then backward in the same launch, the
+// backward sweep re-reading the row this workgroup just wrote (each lane
+// reads back its own vectors, which the caches still hold), so the memory
+// traffic stays at one read of marker and mask and one write of the marker.
 template <class T, int VB, int DIR, bool CLIP>
 __global__ __launch_bounds__(kWG) void k_row_pass_long(Image im, TileMask tm, uint32_t* changed) {
     using V = VecIO<T, VB>;
@@ -376,70 +380,74 @@ __global__ __launch_bounds__(kWG) void k_row_pass_long(Image im, TileMask tm, ui
     for (int y = blockIdx.x; y < im.H; y += gridDim.x) {
         uint8_t* jr = im.J + (size_t)y * im.pj;
         const uint8_t* ir = im.I + (size_t)y * im.pi;
-        uint32_t carry = 0;
-        for (int n = 0; n < nch; ++n) {
-            const int ch = DIR > 0 ? n : nch - 1 - n;
-            const int o = (ch * kWG + tid) * VB;
-            uint32_t jw[NW], iw[NW], v[NPX], c[NPX], ov[NPX];
-            V::load(jr, o, rowbytes, jw);
-            V::load(ir, o, rowbytes, iw);
-            V::unpack(jw, v);
-            V::unpack(iw, c);
+#pragma unroll 1
+        for (int pass = 0; pass < (DIR == 0 ? 2 : 1); ++pass) {
+            const int dir = DIR != 0 ? DIR : (pass == 0 ? 1 : -1);
+            const bool clip = CLIP && pass == 0;
+            uint32_t carry = 0;
+            for (int n = 0; n < nch; ++n) {
+                const int ch = dir > 0 ? n : nch - 1 - n;
+                const int o = (ch * kWG + tid) * VB;
+                uint32_t jw[NW], iw[NW], v[NPX], c[NPX], ov[NPX];
+                V::load(jr, o, rowbytes, jw);
+                V::load(ir, o, rowbytes, iw);
+                V::unpack(jw, v);
+                V::unpack(iw, c);
 #pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                ov[k] = v[k];
-                if (CLIP) v[k] = min(v[k], c[k]);
-            }
-            if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) {
+                for (int k = 0; k < NPX; ++k) {
+                    ov[k] = v[k];
+                    if (clip) v[k] = min(v[k], c[k]);
+                }
+                if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) {
 #pragma unroll
-                for (int k = 0; k < NPX; ++k) c[k] = 0;
-            }
-            uint32_t P = 0, Q = 0xFFFFFFFFu;
+                    for (int k = 0; k < NPX; ++k) c[k] = 0;
+                }
+                uint32_t P = 0, Q = 0xFFFFFFFFu;
 #pragma unroll
-            for (int kk = 0; kk < NPX; ++kk) {
-                const int k = DIR > 0 ? kk : NPX - 1 - kk;
-                P = max(v[k], min(P, c[k]));
-                Q = min(Q, c[k]);
-                v[k] = P;
-            }
-            uint32_t Pi = P, Qi = Q, Pe, Qe;
-            if (DIR > 0) {
-                scan_lanes_fwd(Pi, Qi);
-                Pe = from_left(Pi, 0u);
-                Qe = from_left(Qi, 0xFFFFFFFFu);
-                if (lane == 63) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
-            } else {
-                scan_lanes_bwd(Pi, Qi, lane);
-                Pe = from_right(Pi, 0u);
-                Qe = from_right(Qi, 0xFFFFFFFFu);
-                if (lane == 0) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
-            }
-            __syncthreads();
-            uint32_t xin = carry, xall = carry;
+                for (int kk = 0; kk < NPX; ++kk) {
+                    const int k = dir > 0 ? kk : NPX - 1 - kk;
+                    P = max(v[k], min(P, c[k]));
+                    Q = min(Q, c[k]);
+                }
+                uint32_t Pi = P, Qi = Q, Pe, Qe;
+                if (dir > 0) {
+                    scan_lanes_fwd(Pi, Qi);
+                    Pe = from_left(Pi, 0u);
+                    Qe = from_left(Qi, 0xFFFFFFFFu);
+                    if (lane == 63) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
+                } else {
+                    scan_lanes_bwd(Pi, Qi, lane);
+                    Pe = from_right(Pi, 0u);
+                    Qe = from_right(Qi, 0xFFFFFFFFu);
+                    if (lane == 0) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
+                }
+                __syncthreads();
+                uint32_t xin = carry, xall = carry;
 #pragma unroll
-            for (int uu = 0; uu < 4; ++uu) {
-                const int u = DIR > 0 ? uu : 3 - uu;
-                const uint32_t tp = sP[n & 1][u], tq = sQ[n & 1][u];
-                if (DIR > 0 ? (u < wv) : (u > wv)) xin = max(tp, min(xin, tq));
-                xall = max(tp, min(xall, tq));
-            }
-            carry = xall;
-            const uint32_t xl = max(Pe, min(xin, Qe));
-            uint32_t q = 0xFFFFFFFFu;
-            bool d = false;
+                for (int uu = 0; uu < 4; ++uu) {
+                    const int u = dir > 0 ? uu : 3 - uu;
+                    const uint32_t tp = sP[n & 1][u], tq = sQ[n & 1][u];
+                    if (dir > 0 ? (u < wv) : (u > wv)) xin = max(tp, min(xin, tq));
+                    xall = max(tp, min(xall, tq));
+                }
+                carry = xall;
+                uint32_t x = max(Pe, min(xin, Qe));
+                bool d = false;
 #pragma unroll
-            for (int kk = 0; kk < NPX; ++kk) {
-                const int k = DIR > 0 ? kk : NPX - 1 - kk;
-                q = min(q, c[k]);
-                v[k] = max(v[k], min(xl, q));
-                d |= v[k] != ov[k];
+                for (int kk = 0; kk < NPX; ++kk) {
+                    const int k = dir > 0 ? kk : NPX - 1 - kk;
+                    const uint32_t out = max(v[k], min(x, c[k]));
+                    d |= out != ov[k];
+                    v[k] = out;
+                    x = out;
+                }
+                if (d) {
+                    V::pack(v, jw);
+                    V::store(jr, o, rowbytes, jw);
+                }
+                any_change |= d;
+                __syncthreads();  // the LDS slot pair is reused two chunks later
             }
-            if (d) {
-                V::pack(v, jw);
-                V::store(jr, o, rowbytes, jw);
-            }
-            any_change |= d;
-            __syncthreads();  // the LDS slot pair is reused two chunks later
         }
     }
     if (any_change && changed) *changed = 1u;
@@ -868,44 +876,81 @@ DEV uint32_t tile_notify_mask(const uint32_t* s, const uint8_t* rose, const Imag
 }
 
 // ------------------------------------------------------------ seeding -------
-// flags[m] = 1 when megapixel m holds a pixel p with a neighbour q such that
-// min(J(q), I(p)) > J(p).  One wave per megapixel (LDS tile, lane = column,
-// a three-row window in registers).
-template <class T, int CONN>
-__global__ __launch_bounds__(kWG) void k_unstable(Image im, int nmx, int nmy, uint8_t* flags,
-                                                   uint32_t* count) {
-    extern __shared__ uint32_t lds[];
-    const int lane = lane_id(), wv = threadIdx.x >> 6;
-    uint32_t* s = lds + wv * (WAVE_LDS / 4);
-    const int wid = (blockIdx.x * kWG + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * kWG) >> 6;
-    for (int m = wid; m < nmx * nmy; m += nwaves) {
-        const int mx = m % nmx, my = m / nmx;
-        tile_fetch<T>(im, mx, my, s);
-        bool un = false;
-        uint32_t up = s[tix(-1, lane)], cur = s[tix(0, lane)];
-        for (int r = 0; r < kMP; ++r) {
-            const uint32_t dn = s[tix(r + 1, lane)];
-            const uint32_t j = cur & 0xFFFFu;
-            uint32_t nbm = max(up & 0xFFFFu, dn & 0xFFFFu);
-            nbm = max(nbm, from_left(cur, s[tix(r, -1)]) & 0xFFFFu);
-            nbm = max(nbm, from_right(cur, s[tix(r, 64)]) & 0xFFFFu);
+// Streaming form of the seeding test: a wave sweeps one band of 64 image rows
+// (one megapixel row) over a strip of 64 lanes x VB bytes, keeping the rows
+// above / at / below in registers; left and right neighbours come from the
+// adjacent lanes (DPP) and, at the strip edges, from memory.  Each lane ORs
+// its pixels' instability; a ballot folds the lanes of one megapixel.
+template <class T, int VB, int CONN>
+__global__ __launch_bounds__(kWG) void k_unstable_scan(Image im, int nmx, uint8_t* flags,
+                                                        uint32_t* count) {
+    using V = VecIO<T, VB>;
+    constexpr int NW = V::NW, NPX = V::NPX;
+    constexpr int LPM = kMP / NPX;          // lanes per megapixel
+    constexpr int MPS = 64 / LPM;           // megapixels per strip
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rowbytes = im.W * (int)sizeof(T);
+    const int strip = blockIdx.x * 4 + wv;
+    const int xb0 = strip * 64 * VB;
+    if (xb0 >= rowbytes) return;
+    const int o = xb0 + lane * VB;
+    const int xs0 = xb0 / (int)sizeof(T), xs1 = (xb0 + 64 * VB) / (int)sizeof(T);
+    const int my = blockIdx.y, y0 = my * kMP, y1 = min(im.H, y0 + kMP);
+    auto load_row = [&](const uint8_t* base, size_t pitch, int y, uint32_t (&px)[NPX]) {
+        uint32_t w[NW];
+        if (y >= 0 && y < im.H) V::load(base + (size_t)y * pitch, o, rowbytes, w);
+        else {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) w[i] = 0;
+        }
+        V::unpack(w, px);
+    };
+    auto edge = [&](int y, int x) -> uint32_t {
+        return (y >= 0 && y < im.H && x >= 0 && x < im.W) ? ldp<T>(im.J, im.pj, x, y) : 0u;
+    };
+    uint32_t up[NPX], cur[NPX], dn[NPX], msk[NPX];
+    load_row(im.J, im.pj, y0 - 1, up);
+    load_row(im.J, im.pj, y0, cur);
+    uint32_t upL = lane == 0 ? edge(y0 - 1, xs0 - 1) : 0u, upR = lane == 63 ? edge(y0 - 1, xs1) : 0u;
+    uint32_t cuL = lane == 0 ? edge(y0, xs0 - 1) : 0u, cuR = lane == 63 ? edge(y0, xs1) : 0u;
+    bool un = false;
+    for (int y = y0; y < y1; ++y) {
+        load_row(im.J, im.pj, y + 1, dn);
+        load_row(im.I, im.pi, y, msk);
+        const uint32_t dnL = lane == 0 ? edge(y + 1, xs0 - 1) : 0u;
+        const uint32_t dnR = lane == 63 ? edge(y + 1, xs1) : 0u;
+        // neighbours left of pixel 0 and right of pixel NPX-1 in rows y-1, y, y+1
+        const uint32_t lc = from_left(cur[NPX - 1], cuL), rc = from_right(cur[0], cuR);
+        uint32_t lu = 0, ru = 0, ld = 0, rd = 0;
+        if constexpr (CONN == 8) {
+            lu = from_left(up[NPX - 1], upL);
+            ru = from_right(up[0], upR);
+            ld = from_left(dn[NPX - 1], dnL);
+            rd = from_right(dn[0], dnR);
+        }
+#pragma unroll
+        for (int k = 0; k < NPX; ++k) {
+            uint32_t m = max(up[k], dn[k]);
+            m = max(m, max(k > 0 ? cur[k - 1] : lc, k < NPX - 1 ? cur[k + 1] : rc));
             if constexpr (CONN == 8) {
-                nbm = max(nbm, from_left(up, s[tix(r - 1, -1)]) & 0xFFFFu);
-                nbm = max(nbm, from_right(up, s[tix(r - 1, 64)]) & 0xFFFFu);
-                nbm = max(nbm, from_left(dn, s[tix(r + 1, -1)]) & 0xFFFFu);
-                nbm = max(nbm, from_right(dn, s[tix(r + 1, 64)]) & 0xFFFFu);
+                m = max(m, max(k > 0 ? up[k - 1] : lu, k < NPX - 1 ? up[k + 1] : ru));
+                m = max(m, max(k > 0 ? dn[k - 1] : ld, k < NPX - 1 ? dn[k + 1] : rd));
             }
-            un |= min(nbm, cur >> 16) > j;
-            up = cur;
-            cur = dn;
+            un |= min(m, msk[k]) > cur[k];
         }
-        const bool u = wave_any(un);
-        if (lane == 0) {
-            flags[m] = u ? 1 : 0;
-            if (u && count) atomicAdd(count, 1u);
+#pragma unroll
+        for (int k = 0; k < NPX; ++k) {
+            up[k] = cur[k];
+            cur[k] = dn[k];
         }
-        wave_lds_sync();  // the tile is reused by the next megapixel
+        upL = cuL; upR = cuR; cuL = dnL; cuR = dnR;
+    }
+    const uint64_t b = ballot(un);
+    const int mx0 = xs0 / kMP;
+    if (lane < MPS && mx0 + lane < nmx) {
+        const bool f = ((b >> (lane * LPM)) & ((LPM == 64) ? ~0ull : ((1ull << LPM) - 1ull))) != 0;
+        flags[(size_t)my * nmx + mx0 + lane] = f ? 1 : 0;
+        if (f && count) atomicAdd(count, 1u);
     }
 }
 
@@ -1264,20 +1309,24 @@ hipError_t launch_clip(const Image& im, int dtype, hipStream_t s) {
 
 template <class T, int VB, bool CLIP>
 static void row_pass_t(const Image& im, const TileMask& tm, uint32_t* changed, hipStream_t s) {
+    // u16 rows up to 32 KiB are held in registers (256 or 1024 lanes, 1-2
+    // chunks per lane); u8 rows beyond 4 KiB (16 pixels per lane blow up the
+    // register count) and anything wider are streamed chunk by chunk, forward
+    // and backward in one launch.  Measured on MI355X: 16384-wide u16 rows
+    // 3.4 TB/s in registers vs 2.0 TB/s streamed; 32768-wide u8 rows 2.8 TB/s
+    // streamed vs 1.1 TB/s in registers.
     const int rowbytes = im.W * (int)sizeof(T);
     const int grid = std::min(im.H, 65535);
     if (rowbytes <= 256 * VB)
         hipLaunchKernelGGL((k_row_pass<T, VB, 1, 4, CLIP>), grid, 256, 0, s, im, tm, changed);
-    else if (rowbytes <= 2 * 256 * VB)
+    else if (sizeof(T) == 2 && rowbytes <= 2 * 256 * VB)
         hipLaunchKernelGGL((k_row_pass<T, VB, 2, 4, CLIP>), grid, 256, 0, s, im, tm, changed);
-    else if (rowbytes <= 1024 * VB)
+    else if (sizeof(T) == 2 && rowbytes <= 1024 * VB)
         hipLaunchKernelGGL((k_row_pass<T, VB, 1, 16, CLIP>), grid, 1024, 0, s, im, tm, changed);
-    else if (rowbytes <= 2 * 1024 * VB)
+    else if (sizeof(T) == 2 && rowbytes <= 2 * 1024 * VB)
         hipLaunchKernelGGL((k_row_pass<T, VB, 2, 16, CLIP>), grid, 1024, 0, s, im, tm, changed);
-    else {
-        hipLaunchKernelGGL((k_row_pass_long<T, VB, 1, CLIP>), grid, kWG, 0, s, im, tm, changed);
-        hipLaunchKernelGGL((k_row_pass_long<T, VB, -1, false>), grid, kWG, 0, s, im, tm, changed);
-    }
+    else
+        hipLaunchKernelGGL((k_row_pass_long<T, VB, 0, CLIP>), grid, kWG, 0, s, im, tm, changed);
 }
 
 hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask& tm,
@@ -1355,16 +1404,18 @@ hipError_t launch_col_pass(const uint8_t* src, uint8_t* dst, const uint8_t* mask
 hipError_t launch_unstable(const Image& im, int dtype, int conn, uint8_t* flags,
                            uint32_t* count, hipStream_t s) {
     const int nmx = (im.W + kMP - 1) / kMP, nmy = (im.H + kMP - 1) / kMP;
-    const int nmp = nmx * nmy;
-    const int grid = std::min((nmp + 3) / 4, cu_count() * 8);
-#define IWPP_UN(TT, C) \
-    hipLaunchKernelGGL((k_unstable<TT, C>), grid, kWG, 4 * WAVE_LDS, s, im, nmx, nmy, flags, count)
+    const bool v16 = col_wide(im.J, im.J, im.I, im.pj, im.pj, im.pi, im.W, dtype);
+    const int vb = v16 ? 16 : 4;
+    const int strips = (im.W * dtype + 64 * vb - 1) / (64 * vb);
+    const dim3 grid((strips + 3) / 4, nmy);
+#define IWPP_UN(TT, VBB, C) \
+    hipLaunchKernelGGL((k_unstable_scan<TT, VBB, C>), grid, kWG, 0, s, im, nmx, flags, count)
     if (dtype == 1) {
-        if (conn == 8) IWPP_UN(uint8_t, 8);
-        else IWPP_UN(uint8_t, 4);
+        if (v16) { if (conn == 8) IWPP_UN(uint8_t, 16, 8); else IWPP_UN(uint8_t, 16, 4); }
+        else { if (conn == 8) IWPP_UN(uint8_t, 4, 8); else IWPP_UN(uint8_t, 4, 4); }
     } else {
-        if (conn == 8) IWPP_UN(uint16_t, 8);
-        else IWPP_UN(uint16_t, 4);
+        if (v16) { if (conn == 8) IWPP_UN(uint16_t, 16, 8); else IWPP_UN(uint16_t, 16, 4); }
+        else { if (conn == 8) IWPP_UN(uint16_t, 4, 8); else IWPP_UN(uint16_t, 4, 4); }
     }
 #undef IWPP_UN
     return hipGetLastError();

@@ -65,7 +65,7 @@ def test_row_pass(torch_cuda, dtype, pitch_kind):
         assert int(ch.item()) == int(not np.array_equal(want, J))
 
 
-COL_SHAPES = [(1, 5), (17, 3), (130, 1100), (300, 513), (64, 2048)]
+COL_SHAPES = [(1, 5), (17, 3), (130, 1100), (300, 513), (64, 2048), (70, 16500), (33, 8200)]
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
@@ -92,7 +92,7 @@ def test_col_pass(torch_cuda, dtype, conn, pitch_kind):
 @pytest.mark.parametrize("conn", [4, 8])
 def test_unstable_megapixels(torch_cuda, dtype, conn):
     rng = np.random.default_rng(3)
-    for h, w in [(1, 1), (64, 64), (65, 129), (200, 333), (512, 300)]:
+    for h, w in [(1, 1), (64, 64), (65, 129), (200, 333), (512, 300), (130, 16500), (70, 8200)]:
         say("unstable", dtype.__name__, conn, h, w)
         J, I = rand_pair(rng, h, w, dtype, "seeds")
         for stage in range(3):
