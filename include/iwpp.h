@@ -120,6 +120,40 @@ int iwpp_reconstruct_ex(void* marker, const void* mask, size_t pitch_marker, siz
                         int schedule, const iwpp_options* opt, iwpp_stats* stats,
                         iwpp_workspace* ws, void* stream);
 
+/* One image of a batch: device pointers, pitches in bytes, size. */
+typedef struct {
+    void* marker;
+    const void* mask;
+    size_t pitch_marker, pitch_mask;
+    int width, height;
+} iwpp_image;
+
+/* Batched reconstruction: n images of one dtype and connectivity, each in
+ * place, in one set of launches (one persistent queue kernel pulls
+ * (image, megapixel) work items for all of them).  No reference
+ * counterpart: the reference reconstructs one square image per call
+ * (IWPP MP/include/MorphologicalReconstruction.cuh:45-46); this is that call
+ * for many images at once. */
+int iwpp_reconstruct_batch(const iwpp_image* imgs, int n, int dtype, int connectivity, int tile,
+                           int queue, int schedule, const iwpp_options* opt, iwpp_stats* stats,
+                           iwpp_workspace* ws, void* stream);
+
+/* A batch reconstruction captured once into a hipGraph and replayed.  The
+ * capture records every launch of one iwpp_reconstruct_batch (mutex queue
+ * variant only: the DQ/UQ kernels use cooperative launches) on `stream`,
+ * which must not be the NULL stream; ws must be explicit and must not be
+ * reused by other calls while the graph exists.  A
+ * replay reconstructs the markers in place: the caller restores the input
+ * markers before each replay.  iwpp_batch_graph_stats synchronises `stream`
+ * and returns the counters of the last replay. */
+typedef struct iwpp_graph iwpp_graph;
+iwpp_graph* iwpp_batch_graph_create(const iwpp_image* imgs, int n, int dtype, int connectivity,
+                                    int tile, int queue, int schedule, const iwpp_options* opt,
+                                    iwpp_workspace* ws, void* stream);
+int iwpp_batch_graph_launch(iwpp_graph* g, void* stream);
+int iwpp_batch_graph_stats(iwpp_graph* g, iwpp_stats* stats, void* stream);
+void iwpp_batch_graph_destroy(iwpp_graph* g);
+
 /* marker = min(marker, mask) (IWPP MP/src/MorphologicalReconstruction.cu:827-836). */
 int iwpp_clip(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask, int width,
               int height, int dtype, void* stream);

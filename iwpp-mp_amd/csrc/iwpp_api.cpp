@@ -87,6 +87,10 @@ struct iwpp_workspace {
     uint8_t* tiles = nullptr;   // per-tile active flags (schedule)
     size_t tiles_n = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // phase timing
+    ImgDesc* descs = nullptr;   // batch descriptors (device)
+    size_t descs_n = 0;
+    uint32_t* img_of_mp = nullptr;  // global megapixel -> image (device)
+    size_t img_of_mp_n = 0;
 };
 
 extern "C" void iwpp_default_options(iwpp_options* o) {
@@ -126,6 +130,8 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->ring);
     (void)hipFree(ws->ctl);
     (void)hipFree(ws->tiles);
+    (void)hipFree(ws->descs);
+    (void)hipFree(ws->img_of_mp);
     if (ws->host_ctl) (void)hipHostFree(ws->host_ctl);
     for (hipEvent_t e : ws->ev)
         if (e) (void)hipEventDestroy(e);
@@ -160,16 +166,24 @@ int check_image(const void* a, const void* b, size_t pa, size_t pb, int w, int h
     return IWPP_OK;
 }
 
+// One reconstruction launch: one image, or several of the same dtype and
+// connectivity.
 struct Run {
     bool timed;  // record phase times with HIP events
-    Image im;
-    int dtype, conn, tile, queue, schedule;
+    bool graph;  // graph mode: mutex only, launches without host synchronisation
+    bool capturing;  // inside stream capture: no uploads or synchronisation
+    int dtype, conn, queue;
     iwpp_options opt;
     iwpp_stats st;
     iwpp_workspace* ws;
     hipStream_t s;
-    int nmx, nmy, nmp;
-    size_t plane_pitch;
+    std::vector<ImgDesc> h;  // host copy of the descriptors
+    Descs ds{};              // what the kernels get (totals set per launch)
+    bool wide = false;       // 16-byte lanes in the column / seeding passes
+    int single_row_pass = 0; // one image: use the register row kernels
+    int total_rows = 0, total_cw = 0, total_sw = 0, total_mp = 0;
+    QueueArgs q{};
+    size_t cap = 0;
 };
 
 int ensure_events(iwpp_workspace* ws) {
@@ -193,32 +207,126 @@ unsigned long long timeout_ticks(double seconds) {
     return (unsigned long long)(seconds * khz * 1000.0);
 }
 
-// Sweep phase: `rounds` rounds of row pass + column pass down + column pass up.
-int sweep_phase(Run& r, int rounds, const TileMask& tm) {
-    const Image& im = r.im;
-    if (rounds <= 0) {
-        HIP_TRY(launch_clip(im, r.dtype, r.s));
+// Builds descriptors, scratch planes and the device-side image tables.
+int setup_images(Run& r, const iwpp_image* imgs, int n) {
+    iwpp_workspace* ws = r.ws;
+    r.h.resize(n);
+    // scratch planes for the column passes, one after the other
+    size_t plane_bytes = 0;
+    std::vector<size_t> poff(n);
+    for (int i = 0; i < n; ++i) {
+        poff[i] = plane_bytes;
+        plane_bytes += round_up((size_t)imgs[i].width * r.dtype, 256) * imgs[i].height + 256;
+    }
+    HIP_TRY(grow(ws->plane, ws->plane_n, plane_bytes + 256));
+    uint8_t* plane0 = (uint8_t*)round_up((uintptr_t)ws->plane, 256);
+    r.wide = true;
+    long strip_rows = 0;
+    for (int i = 0; i < n; ++i) {
+        ImgDesc& d = r.h[i];
+        d = ImgDesc{};
+        d.J = (uint8_t*)imgs[i].marker;
+        d.I = (const uint8_t*)imgs[i].mask;
+        d.P = plane0 + poff[i];
+        d.pj = imgs[i].pitch_marker;
+        d.pi = imgs[i].pitch_mask;
+        d.pp = round_up((size_t)imgs[i].width * r.dtype, 256);
+        d.W = imgs[i].width;
+        d.H = imgs[i].height;
+        d.nmx = (d.W + kMP - 1) / kMP;
+        d.nmy = (d.H + kMP - 1) / kMP;
+        d.tm = TileMask{nullptr, kMP, kMP, 1};
+        r.wide = r.wide && col_wide(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, r.dtype);
+    }
+    for (int i = 0; i < n; ++i) {
+        r.h[i].strips = col_strips(r.h[i].W, r.dtype, r.wide);
+        strip_rows += (long)r.h[i].strips * r.h[i].H;
+    }
+    // one block height for the launch, aiming at >= 8192 column waves
+    int R = r.opt.col_block_rows;
+    if (R <= 0) {
+        long rr = strip_rows / 8192 / 32 * 32;
+        R = (int)std::min<long>(512, std::max<long>(32, rr));
+    }
+    long rows = 0, cw = 0, sw = 0, mp = 0;
+    for (int i = 0; i < n; ++i) {
+        ImgDesc& d = r.h[i];
+        d.R = R;
+        d.row_base = (int)rows;
+        d.cw_base = (int)cw;
+        d.sw_base = (int)sw;
+        d.mp_base = (int)mp;
+        rows += d.H;
+        cw += (long)d.strips * ((d.H + R - 1) / R);
+        sw += (long)d.strips * d.nmy;
+        mp += (long)d.nmx * d.nmy;
+    }
+    if (rows > 0x7FFFFFFF || cw > 0x7FFFFFFF || sw > 0x7FFFFFFF || mp > 0x7FFFFFF0)
+        return fail(IWPP_ERR_ARG, "batch too large");
+    r.total_rows = (int)rows;
+    r.total_cw = (int)cw;
+    r.total_sw = (int)sw;
+    r.total_mp = (int)mp;
+    r.ds = Descs{};
+    r.ds.n = n;
+    if (n == 1) {
+        r.ds.one = r.h[0];
         return IWPP_OK;
     }
-    const size_t need = r.plane_pitch * (size_t)im.H + 256;
-    HIP_TRY(grow(r.ws->plane, r.ws->plane_n, need));
-    uint8_t* plane = (uint8_t*)round_up((uintptr_t)r.ws->plane, 256);
+    // device tables: descriptors and megapixel -> image
+    std::vector<uint32_t> img_of_mp((size_t)mp);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < r.h[i].nmx * r.h[i].nmy; ++k) img_of_mp[r.h[i].mp_base + k] = (uint32_t)i;
+    if (!r.capturing) {  // a capture reuses the tables its dry run uploaded
+        HIP_TRY(grow(ws->descs, ws->descs_n, (size_t)n));
+        HIP_TRY(grow(ws->img_of_mp, ws->img_of_mp_n, (size_t)mp));
+        HIP_TRY(hipMemcpyAsync(ws->descs, r.h.data(), sizeof(ImgDesc) * n,
+                               hipMemcpyHostToDevice, r.s));
+        HIP_TRY(hipMemcpyAsync(ws->img_of_mp, img_of_mp.data(), 4 * (size_t)mp,
+                               hipMemcpyHostToDevice, r.s));
+        // the host vectors die with this call: make the copies complete first
+        HIP_TRY(hipStreamSynchronize(r.s));
+    }
+    r.ds.d = ws->descs;
+    r.ds.img_of_mp = ws->img_of_mp;
+    return IWPP_OK;
+}
+
+Descs with_total(Descs d, int total) {
+    d.total = total;
+    return d;
+}
+
+// Sweep phase: `rounds` rounds of row pass + column pass down + column pass up.
+int sweep_phase(Run& r, int rounds) {
+    if (rounds <= 0) {
+        // clip only
+        for (const ImgDesc& d : r.h)
+            HIP_TRY(launch_clip(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.s));
+        return IWPP_OK;
+    }
     for (int k = 0; k < rounds; ++k) {
-        HIP_TRY(launch_row_pass(im, r.dtype, k == 0, tm, nullptr, r.s));
-        HIP_TRY(launch_col_pass(im.J, plane, im.I, im.pj, r.plane_pitch, im.pi, im.W, im.H,
-                                r.dtype, r.conn, +1, r.opt.col_block_rows, tm, nullptr, r.s));
-        HIP_TRY(launch_col_pass(plane, im.J, im.I, r.plane_pitch, im.pj, im.pi, im.W, im.H,
-                                r.dtype, r.conn, -1, r.opt.col_block_rows, tm, nullptr, r.s));
+        if (r.ds.n == 1) {
+            const ImgDesc& d = r.h[0];
+            HIP_TRY(launch_row_pass(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, k == 0, d.tm,
+                                    nullptr, r.s));
+        } else {
+            HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows), r.dtype, k == 0,
+                                          nullptr, r.s));
+        }
+        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw), r.dtype, r.conn, +1, true,
+                                      r.wide, nullptr, r.s));
+        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw), r.dtype, r.conn, -1, false,
+                                      r.wide, nullptr, r.s));
     }
     r.st.sweep_rounds += rounds;
     return IWPP_OK;
 }
 
-// Seeding + queue phase, restarted while a queue overflowed.
-int queue_phase(Run& r) {
+int setup_queue(Run& r) {
     iwpp_workspace* ws = r.ws;
     const int kind = r.queue;
-    const size_t nmp = (size_t)r.nmp;
+    const size_t nmp = (size_t)r.total_mp;
     HIP_TRY(grow(ws->st, ws->st_n, nmp));
     HIP_TRY(grow(ws->flags, ws->flags_n, nmp));
     size_t cap;
@@ -234,34 +342,72 @@ int queue_phase(Run& r) {
     if (cap > 0xFFFFFFF0ull) return fail(IWPP_ERR_ARG, "queue capacity too large");
     int rc = ensure_ctl(ws);
     if (rc) return rc;
-
-    QueueArgs q{};
+    QueueArgs& q = r.q;
+    q = QueueArgs{};
     q.ctl = ws->ctl;
     q.st = ws->st;
     q.qa = ws->qa;
     q.qb = ws->qb;
     q.ring = ws->ring;
     q.cap = (uint32_t)cap;
-    q.nmx = r.nmx;
-    q.nmy = r.nmy;
+    q.nmp = r.total_mp;
     q.max_iters = std::max(1, r.opt.max_local_iters);
     q.max_rounds = 1 << 30;
     q.timeout_ticks = timeout_ticks(r.opt.timeout_s > 0 ? r.opt.timeout_s : 60.0);
     q.trace = r.opt.trace_phases ? 1 : 0;
-    const double us_per_tick = 1e6 / ((double)timeout_ticks(1.0));
+    r.cap = cap;
+    return IWPP_OK;
+}
 
+// Seeding + queue kernel launches (no synchronisation).
+int queue_launches(Run& r) {
+    iwpp_workspace* ws = r.ws;
+    const int kind = r.queue;
+    if (r.timed) HIP_TRY(hipEventRecord(ws->ev[1], r.s));
+    HIP_TRY(launch_fill_u32(ws->ctl, 0u, (size_t)C_NSLOTS * kCtlStride, r.s));
+    HIP_TRY(launch_fill_u32(ws->st, 0u, (size_t)r.total_mp, r.s));
+    if (kind == Q_MUTEX) HIP_TRY(launch_ring_init(ws->ring, r.cap, r.s));
+    HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
+                                  ws->flags, nullptr, r.s));
+    HIP_TRY(launch_seed_queue(ws->flags, r.total_mp, kind, r.q, r.s));
+    HIP_TRY(launch_prepare_queue(r.q, r.s));
+    if (r.timed) HIP_TRY(hipEventRecord(ws->ev[2], r.s));
+    HIP_TRY(launch_queue_phase(r.ds, r.dtype, r.conn, kind, r.q, r.opt.workers_per_cu, r.s));
+    if (r.timed) HIP_TRY(hipEventRecord(ws->ev[3], r.s));
+    return IWPP_OK;
+}
+
+// Reads the control block (after the stream reached it) into the counters.
+// Returns 1 when the queue overflowed (restart needed).
+int read_counters(Run& r) {
+    const uint32_t* h = r.ws->host_ctl;
+    const double us_per_tick = 1e6 / ((double)timeout_ticks(1.0));
+    r.st.seeded += h[C_TAIL * kCtlStride] + 0ll;
+    r.st.tile_visits += (long long)(*(const unsigned long long*)(h + C_VISITS_LO * kCtlStride));
+    r.st.queue_rounds += (int)h[C_ROUND * kCtlStride];
+    if (r.q.trace) {
+        auto ph = [&](int slot) {
+            return (double)(*(const unsigned long long*)(h + slot * kCtlStride)) * us_per_tick;
+        };
+        r.st.us_wait += ph(C_PH_POP);
+        r.st.us_fetch += ph(C_PH_FETCH);
+        r.st.us_converge += ph(C_PH_CONVERGE);
+        r.st.us_store += ph(C_PH_STORE);
+        r.st.us_notify += ph(C_PH_NOTIFY);
+    }
+    if (h[C_TIMEOUT * kCtlStride]) return fail(IWPP_ERR_TIMEOUT, "device-side queue wait timed out");
+    return h[C_OVERFLOW * kCtlStride] ? 1 : 0;
+}
+
+// Seeding + queue phase, restarted while a queue overflowed.
+int queue_phase(Run& r) {
+    iwpp_workspace* ws = r.ws;
+    int rc = setup_queue(r);
+    if (rc) return rc;
     const int max_restarts = 100000;
     for (int attempt = 0;; ++attempt) {
-        if (r.timed) HIP_TRY(hipEventRecord(ws->ev[1], r.s));
-        HIP_TRY(launch_fill_u32(ws->ctl, 0u, (size_t)C_NSLOTS * kCtlStride, r.s));
-        HIP_TRY(launch_fill_u32(ws->st, 0u, nmp, r.s));
-        if (kind == Q_MUTEX) HIP_TRY(launch_ring_init(ws->ring, cap, r.s));
-        HIP_TRY(launch_unstable(r.im, r.dtype, r.conn, ws->flags, nullptr, r.s));
-        HIP_TRY(launch_seed_queue(ws->flags, r.nmp, kind, q, r.s));
-        HIP_TRY(launch_prepare_queue(q, r.s));
-        if (r.timed) HIP_TRY(hipEventRecord(ws->ev[2], r.s));
-        HIP_TRY(launch_queue_phase(r.im, r.dtype, r.conn, kind, q, r.opt.workers_per_cu, r.s));
-        if (r.timed) HIP_TRY(hipEventRecord(ws->ev[3], r.s));
+        rc = queue_launches(r);
+        if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(ws->host_ctl, ws->ctl, (size_t)C_NSLOTS * kCtlStride * 4,
                                hipMemcpyDeviceToHost, r.s));
         HIP_TRY(hipStreamSynchronize(r.s));
@@ -272,25 +418,67 @@ int queue_phase(Run& r) {
             r.st.ms_seed += a;
             r.st.ms_queue += b;
         }
-        const uint32_t* h = ws->host_ctl;
-        r.st.seeded += h[C_TAIL * kCtlStride] + 0ll;
-        r.st.tile_visits += (long long)(*(const unsigned long long*)(h + C_VISITS_LO * kCtlStride));
-        r.st.queue_rounds += (int)h[C_ROUND * kCtlStride];
-        if (q.trace) {
-            auto ph = [&](int slot) {
-                return (double)(*(const unsigned long long*)(h + slot * kCtlStride)) * us_per_tick;
-            };
-            r.st.us_wait += ph(C_PH_POP);
-            r.st.us_fetch += ph(C_PH_FETCH);
-            r.st.us_converge += ph(C_PH_CONVERGE);
-            r.st.us_store += ph(C_PH_STORE);
-            r.st.us_notify += ph(C_PH_NOTIFY);
-        }
-        if (h[C_TIMEOUT * kCtlStride])
-            return fail(IWPP_ERR_TIMEOUT, "device-side queue wait timed out");
-        if (!h[C_OVERFLOW * kCtlStride]) break;
+        rc = read_counters(r);
+        if (rc < 0) return rc;
+        if (rc == 0) break;
         r.st.restarts++;
         if (attempt >= max_restarts) return fail(IWPP_ERR_TIMEOUT, "too many queue restarts");
+    }
+    return IWPP_OK;
+}
+
+int check_common(int dtype, int conn, int tile, int queue, int schedule) {
+    if (dtype != IWPP_U8 && dtype != IWPP_U16) return fail(IWPP_ERR_ARG, "bad dtype %d", dtype);
+    if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
+    if (tile < 0 || tile % kMP) return fail(IWPP_ERR_ARG, "tile must be 0 or a multiple of 64");
+    if (queue < IWPP_QUEUE_MUTEX || queue > IWPP_QUEUE_AUTO)
+        return fail(IWPP_ERR_ARG, "bad queue variant %d", queue);
+    if (schedule != IWPP_SCHED_PLAIN && schedule != IWPP_SCHED_LEARNED)
+        return fail(IWPP_ERR_ARG, "bad schedule %d", schedule);
+    return IWPP_OK;
+}
+
+// Shared body of the single-image and batch entry points.  With graph = true
+// only the launches are issued (nothing synchronises, no restart).
+int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, int schedule,
+                    const iwpp_options* opt) {
+    iwpp_default_options(&r.opt);
+    if (opt) r.opt = *opt;
+    int maxw = 0, maxh = 0;
+    for (int i = 0; i < n; ++i) {
+        maxw = std::max(maxw, imgs[i].width);
+        maxh = std::max(maxh, imgs[i].height);
+    }
+    SchedulePlan plan;
+    int rc = plan_schedule(n == 1 ? imgs[0].width : maxw, n == 1 ? imgs[0].height : maxh,
+                           r.dtype, r.conn, tile, queue, schedule, r.opt.sweep_rounds, &plan);
+    if (rc) return fail(IWPP_ERR_ARG, "no schedule for this configuration");
+    r.queue = plan.queue;
+    r.st.queue = plan.queue;
+    r.st.tile = plan.tile;
+    if (r.graph && r.queue != Q_MUTEX)
+        return fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
+    rc = setup_images(r, imgs, n);
+    if (rc) return rc;
+    if (r.graph) {
+        rc = setup_queue(r);
+        if (rc) return rc;
+    }
+    r.timed = r.timed && !r.graph;
+    if (r.timed) {
+        rc = ensure_events(r.ws);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(r.ws->ev[0], r.s));
+    }
+    rc = sweep_phase(r, plan.sweep_rounds);
+    if (rc) return rc;
+    if (r.graph) return queue_launches(r);
+    rc = queue_phase(r);
+    if (rc) return rc;
+    if (r.timed) {
+        float a = 0;
+        if (r.st.restarts == 0) HIP_TRY(hipEventElapsedTime(&a, r.ws->ev[0], r.ws->ev[1]));
+        r.st.ms_sweep = a;
     }
     return IWPP_OK;
 }
@@ -304,55 +492,19 @@ extern "C" int iwpp_reconstruct_ex(void* marker, const void* mask, size_t pm, si
     g_err.clear();
     int rc = check_image(marker, mask, pm, pk, w, h, dtype);
     if (rc) return rc;
-    if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
-    if (tile < 0 || tile % kMP) return fail(IWPP_ERR_ARG, "tile must be 0 or a multiple of 64");
-    if (queue < IWPP_QUEUE_MUTEX || queue > IWPP_QUEUE_AUTO)
-        return fail(IWPP_ERR_ARG, "bad queue variant %d", queue);
-    if (schedule != IWPP_SCHED_PLAIN && schedule != IWPP_SCHED_LEARNED)
-        return fail(IWPP_ERR_ARG, "bad schedule %d", schedule);
+    rc = check_common(dtype, conn, tile, queue, schedule);
+    if (rc) return rc;
     if (!ws) ws = default_ws();
     if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
-
     Run r{};
-    r.im = Image{(uint8_t*)marker, (const uint8_t*)mask, pm, pk, w, h};
     r.dtype = dtype;
     r.conn = conn;
     r.ws = ws;
     r.s = (hipStream_t)stream;
-    iwpp_default_options(&r.opt);
-    if (opt) r.opt = *opt;
-    r.nmx = (w + kMP - 1) / kMP;
-    r.nmy = (h + kMP - 1) / kMP;
-    r.nmp = r.nmx * r.nmy;
-    r.plane_pitch = round_up((size_t)w * dtype, 256);
-
-    SchedulePlan plan;
-    rc = plan_schedule(w, h, dtype, conn, tile, queue, schedule, r.opt.sweep_rounds, &plan);
-    if (rc) return fail(IWPP_ERR_ARG, "no schedule for this configuration");
-    r.tile = plan.tile;
-    r.queue = plan.queue;
-    r.schedule = schedule;
-    r.st.queue = plan.queue;
-    r.st.tile = plan.tile;
-
     r.timed = stats != nullptr;
-    if (r.timed) {
-        rc = ensure_events(ws);
-        if (rc) return rc;
-        HIP_TRY(hipEventRecord(ws->ev[0], r.s));
-    }
-    const TileMask none{nullptr, kMP, kMP, 1};
-    rc = sweep_phase(r, plan.sweep_rounds, none);
+    const iwpp_image img{marker, mask, pm, pk, w, h};
+    rc = reconstruct_run(r, &img, 1, tile, queue, schedule, opt);
     if (rc) return rc;
-    rc = queue_phase(r);
-    if (rc) return rc;
-    if (r.timed) {
-        // ev[0] -> first ev[1] would be exact; the seeding record of the last
-        // attempt is the upper bound of the sweep phase when no restart happened.
-        float a = 0;
-        if (r.st.restarts == 0) HIP_TRY(hipEventElapsedTime(&a, ws->ev[0], ws->ev[1]));
-        r.st.ms_sweep = a;
-    }
     if (stats) *stats = r.st;
     return IWPP_OK;
 }
@@ -362,6 +514,157 @@ extern "C" int iwpp_reconstruct(void* marker, const void* mask, size_t pm, size_
                                 iwpp_workspace* ws, void* stream) {
     return iwpp_reconstruct_ex(marker, mask, pm, pk, w, h, dtype, conn, tile, queue, schedule,
                                nullptr, nullptr, ws, stream);
+}
+
+extern "C" int iwpp_reconstruct_batch(const iwpp_image* imgs, int n, int dtype, int conn,
+                                      int tile, int queue, int schedule,
+                                      const iwpp_options* opt, iwpp_stats* stats,
+                                      iwpp_workspace* ws, void* stream) {
+    g_err.clear();
+    if (!imgs || n <= 0) return fail(IWPP_ERR_ARG, "empty batch");
+    for (int i = 0; i < n; ++i) {
+        int rc = check_image(imgs[i].marker, imgs[i].mask, imgs[i].pitch_marker,
+                             imgs[i].pitch_mask, imgs[i].width, imgs[i].height, dtype);
+        if (rc) return rc;
+    }
+    int rc = check_common(dtype, conn, tile, queue, schedule);
+    if (rc) return rc;
+    if (!ws) ws = default_ws();
+    if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
+    Run r{};
+    r.dtype = dtype;
+    r.conn = conn;
+    r.ws = ws;
+    r.s = (hipStream_t)stream;
+    r.timed = stats != nullptr;
+    rc = reconstruct_run(r, imgs, n, tile, queue, schedule, opt);
+    if (rc) return rc;
+    if (stats) *stats = r.st;
+    return IWPP_OK;
+}
+
+struct iwpp_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    iwpp_workspace* ws = nullptr;
+    Run run;  // queue arguments and counters layout of the captured launch
+};
+
+extern "C" iwpp_graph* iwpp_batch_graph_create(const iwpp_image* imgs, int n, int dtype,
+                                               int conn, int tile, int queue, int schedule,
+                                               const iwpp_options* opt, iwpp_workspace* ws,
+                                               void* stream) {
+    g_err.clear();
+    if (!imgs || n <= 0 || !ws) {
+        fail(IWPP_ERR_ARG, "graph capture needs images and an explicit workspace");
+        return nullptr;
+    }
+    for (int i = 0; i < n; ++i)
+        if (check_image(imgs[i].marker, imgs[i].mask, imgs[i].pitch_marker, imgs[i].pitch_mask,
+                        imgs[i].width, imgs[i].height, dtype))
+            return nullptr;
+    if (check_common(dtype, conn, tile, queue, schedule)) return nullptr;
+    auto* g = new (std::nothrow) iwpp_graph();
+    if (!g) return nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    Run& r = g->run;
+    r.dtype = dtype;
+    r.conn = conn;
+    r.ws = ws;
+    r.s = s;
+    r.graph = true;
+    r.capturing = true;
+    g->ws = ws;
+    // Workspace growth, descriptor upload and every allocation happen during
+    // a dry run of the set-up before capture; the capture then only records
+    // kernel launches.
+    {
+        Run dry{};
+        dry.dtype = dtype;
+        dry.conn = conn;
+        dry.ws = ws;
+        dry.s = s;
+        dry.graph = true;
+        iwpp_default_options(&dry.opt);
+        if (opt) dry.opt = *opt;
+        SchedulePlan plan;
+        int maxw = 0, maxh = 0;
+        for (int i = 0; i < n; ++i) {
+            maxw = std::max(maxw, imgs[i].width);
+            maxh = std::max(maxh, imgs[i].height);
+        }
+        if (plan_schedule(maxw, maxh, dtype, conn, tile, queue, schedule, dry.opt.sweep_rounds,
+                          &plan) ||
+            plan.queue != Q_MUTEX) {
+            fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
+            delete g;
+            return nullptr;
+        }
+        dry.queue = plan.queue;
+        if (setup_images(dry, imgs, n) || setup_queue(dry)) {
+            delete g;
+            return nullptr;
+        }
+    }
+    if (s == nullptr) {
+        fail(IWPP_ERR_ARG, "graph capture needs a stream other than the NULL stream");
+        delete g;
+        return nullptr;
+    }
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        fail(IWPP_ERR_HIP, "hipStreamBeginCapture failed");
+        delete g;
+        return nullptr;
+    }
+    const int rc = reconstruct_run(r, imgs, n, tile, queue, schedule, opt);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc || e != hipSuccess || !graph) {
+        if (!rc) fail(IWPP_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        if (graph) (void)hipGraphDestroy(graph);
+        delete g;
+        return nullptr;
+    }
+    g->graph = graph;
+    if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        fail(IWPP_ERR_HIP, "hipGraphInstantiate failed");
+        (void)hipGraphDestroy(graph);
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
+
+extern "C" int iwpp_batch_graph_launch(iwpp_graph* g, void* stream) {
+    g_err.clear();
+    if (!g || !g->exec) return fail(IWPP_ERR_ARG, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return IWPP_OK;
+}
+
+extern "C" int iwpp_batch_graph_stats(iwpp_graph* g, iwpp_stats* stats, void* stream) {
+    g_err.clear();
+    if (!g) return fail(IWPP_ERR_ARG, "null graph");
+    hipStream_t s = (hipStream_t)stream;
+    iwpp_workspace* ws = g->ws;
+    HIP_TRY(hipMemcpyAsync(ws->host_ctl, ws->ctl, (size_t)C_NSLOTS * kCtlStride * 4,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    Run& r = g->run;
+    r.st = iwpp_stats{};
+    r.st.queue = r.queue;
+    const int rc = read_counters(r);
+    if (rc < 0) return rc;
+    if (rc > 0) return fail(IWPP_ERR_UNSUPPORTED, "queue overflow inside a graph replay");
+    if (stats) *stats = r.st;
+    return IWPP_OK;
+}
+
+extern "C" void iwpp_batch_graph_destroy(iwpp_graph* g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
 }
 
 extern "C" int iwpp_clip(void* marker, const void* mask, size_t pm, size_t pk, int w, int h,
@@ -396,10 +699,24 @@ extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, siz
     if (src == dst) return fail(IWPP_ERR_ARG, "column pass is out of place: src == dst");
     if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
     if (dir != 1 && dir != -1) return fail(IWPP_ERR_ARG, "dir must be +1 or -1");
-    const TileMask none{nullptr, kMP, kMP, 1};
-    HIP_TRY(launch_col_pass((const uint8_t*)src, (uint8_t*)dst, (const uint8_t*)mask, ps, pd, pk,
-                            w, h, dtype, conn, dir, block_rows, none, (uint32_t*)changed,
-                            (hipStream_t)stream));
+    const bool wide = col_wide(src, dst, mask, ps, pd, pk, w, dtype);
+    Descs ds{};
+    ds.n = 1;
+    ImgDesc& d = ds.one;
+    d.J = (uint8_t*)src;  // read J, write P
+    d.P = (uint8_t*)dst;
+    d.I = (const uint8_t*)mask;
+    d.pj = ps;
+    d.pp = pd;
+    d.pi = pk;
+    d.W = w;
+    d.H = h;
+    d.strips = col_strips(w, dtype, wide);
+    d.R = block_rows > 0 ? block_rows : col_auto_rows(w, h, dtype, wide);
+    d.tm = TileMask{nullptr, kMP, kMP, 1};
+    ds.total = d.strips * ((h + d.R - 1) / d.R);
+    HIP_TRY(launch_col_pass_batch(ds, dtype, conn, dir, true, wide, (uint32_t*)changed,
+                                  (hipStream_t)stream));
     return IWPP_OK;
 }
 
@@ -417,8 +734,22 @@ extern "C" int iwpp_unstable_megapixels(const void* marker, const void* mask, si
     if (rc) return rc;
     if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
     if (!flags) return fail(IWPP_ERR_ARG, "null flags");
-    HIP_TRY(launch_unstable(Image{(uint8_t*)marker, (const uint8_t*)mask, pm, pk, w, h}, dtype,
-                            conn, flags, (uint32_t*)count, (hipStream_t)stream));
+    const bool wide = col_wide(marker, marker, mask, pm, pm, pk, w, dtype);
+    Descs ds{};
+    ds.n = 1;
+    ImgDesc& d = ds.one;
+    d.J = (uint8_t*)marker;
+    d.I = (const uint8_t*)mask;
+    d.pj = pm;
+    d.pi = pk;
+    d.W = w;
+    d.H = h;
+    d.nmx = (w + kMP - 1) / kMP;
+    d.nmy = (h + kMP - 1) / kMP;
+    d.strips = col_strips(w, dtype, wide);
+    ds.total = d.strips * d.nmy;
+    HIP_TRY(launch_unstable_batch(ds, dtype, conn, wide, flags, (uint32_t*)count,
+                                  (hipStream_t)stream));
     return IWPP_OK;
 }
 

@@ -54,6 +54,36 @@ struct Image {
     int W, H;
 };
 
+// One image of a launch.  Every kernel that can run over several images
+// enumerates global work items (rows, column-pass waves, seeding waves,
+// megapixels) and finds the image through the *_base prefix offsets.
+struct ImgDesc {
+    uint8_t* J;          // marker (in/out)
+    const uint8_t* I;    // mask
+    uint8_t* P;          // scratch plane for the column passes (may be null)
+    size_t pj, pi, pp;   // pitches in bytes
+    int W, H;
+    int nmx, nmy;        // megapixel grid
+    int R;               // column-pass block rows
+    int strips;          // column strips at the launch's lane width
+    int mp_base;         // first global megapixel index
+    int row_base;        // first global row
+    int cw_base;         // first column-pass wave
+    int sw_base;         // first seeding wave
+    TileMask tm;         // walls for the sweep passes
+};
+
+// Images of a launch: a device array, or (d == nullptr) the single image `one`
+// passed by value.  img_of_mp maps a global megapixel to its image (device,
+// may be null when n == 1).
+struct Descs {
+    const ImgDesc* d;
+    const uint32_t* img_of_mp;
+    int n;
+    int total;           // work items of the launch (kernel specific)
+    ImgDesc one;
+};
+
 struct QueueArgs {
     uint32_t* ctl;      // control block
     uint32_t* st;       // per-megapixel state / dedup flag
@@ -61,7 +91,7 @@ struct QueueArgs {
     uint32_t* qb;       // queue B (bsp write queue on even rounds)
     unsigned long long* ring;  // mutex ring: (sequence << 32) | megapixel per slot
     uint32_t cap;       // entries per queue (mutex: power of two)
-    int nmx, nmy;       // megapixel grid
+    int nmp;            // megapixels of all images
     int max_iters;      // local sweep rounds per visit
     int max_rounds;     // bsp round limit
     unsigned long long timeout_ticks;  // wall-clock ticks a wait may last
@@ -74,21 +104,29 @@ enum QueueKind : int { Q_MUTEX = 0, Q_DQ = 1, Q_UQ = 2 };
 hipError_t launch_clip(const Image& im, int dtype, hipStream_t s);
 hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask& tm,
                            uint32_t* changed, hipStream_t s);
-// block_rows <= 0 picks the block height from the image size.
-hipError_t launch_col_pass(const uint8_t* src, uint8_t* dst, const uint8_t* mask, size_t ps,
-                           size_t pd, size_t pi, int W, int H, int dtype, int conn, int dir,
-                           int block_rows, const TileMask& tm, uint32_t* changed,
-                           hipStream_t s);
+// Streaming row pass over the rows of every image (ds.total = all rows).
+hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, uint32_t* changed,
+                                 hipStream_t s);
+// Column pass over every image's column blocks (ds.total = all waves).
+// j_to_p: read J write P (else read P write J); dir +1 down, -1 up; wide =
+// 16-byte lanes (every image 16-byte aligned), else 4-byte lanes.
+hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, bool j_to_p,
+                                 bool wide, uint32_t* changed, hipStream_t s);
+bool col_wide(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
+              size_t pi, int W, int dtype);
 int col_block_width(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
                     size_t pi, int W, int dtype);
-hipError_t launch_unstable(const Image& im, int dtype, int conn, uint8_t* flags,
-                           uint32_t* count, hipStream_t s);
+int col_auto_rows(int W, int H, int dtype, bool wide);
+int col_strips(int W, int dtype, bool wide);
+// Seeding test over every image (ds.total = all seeding waves: strips x nmy).
+hipError_t launch_unstable_batch(const Descs& ds, int dtype, int conn, bool wide,
+                                 uint8_t* flags, uint32_t* count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s);
 hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const QueueArgs& q,
                              hipStream_t s);
 hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s);
-hipError_t launch_queue_phase(const Image& im, int dtype, int conn, int kind, const QueueArgs& q,
-                              int workers_per_cu, hipStream_t s);
+hipError_t launch_queue_phase(const Descs& ds, int dtype, int conn, int kind,
+                              const QueueArgs& q, int workers_per_cu, hipStream_t s);
 
 }  // namespace iwpp

@@ -202,6 +202,27 @@ DEV bool tile_on(const TileMask& tm, int x, int y) {
     return tm.active == nullptr || tm.active[(size_t)(y / tm.th) * tm.ntx + x / tm.tw] != 0;
 }
 
+// ------------------------------------------------------------ descriptors ---
+DEV const ImgDesc& desc(const Descs& a, int i) { return a.d ? a.d[i] : a.one; }
+// Image owning global work item `idx` for a prefix field (binary search).
+#define IWPP_FIND(NAME, FIELD)                                           \
+    DEV int NAME(const Descs& a, int idx) {                              \
+        if (a.d == nullptr) return 0;                                    \
+        int lo = 0, hi = a.n - 1;                                        \
+        while (lo < hi) {                                                \
+            const int mid = (lo + hi + 1) >> 1;                          \
+            if (a.d[mid].FIELD <= idx) lo = mid;                         \
+            else hi = mid - 1;                                           \
+        }                                                                \
+        return lo;                                                       \
+    }
+IWPP_FIND(find_row, row_base)
+IWPP_FIND(find_cw, cw_base)
+IWPP_FIND(find_sw, sw_base)
+#undef IWPP_FIND
+DEV int find_mp(const Descs& a, int g) { return a.img_of_mp ? (int)a.img_of_mp[g] : 0; }
+DEV Image image_of(const ImgDesc& d) { return Image{d.J, d.I, d.pj, d.pi, d.W, d.H}; }
+
 // ------------------------------------------------------------------ clip ----
 template <class T, int VB>
 __global__ __launch_bounds__(kWG) void k_clip(Image im) {
@@ -369,86 +390,109 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
 // reads back its own vectors, which the caches still hold), so the memory
 // traffic stays at one read of marker and mask and one write of the marker.
 template <class T, int VB, int DIR, bool CLIP>
-__global__ __launch_bounds__(kWG) void k_row_pass_long(Image im, TileMask tm, uint32_t* changed) {
+DEV void row_stream(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2][4],
+                    uint32_t (&sQ)[2][4], bool& any_change) {
     using V = VecIO<T, VB>;
     constexpr int NW = V::NW, NPX = V::NPX;
-    __shared__ uint32_t sP[2][4], sQ[2][4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rowbytes = im.W * (int)sizeof(T);
     const int nch = (rowbytes + kWG * VB - 1) / (kWG * VB);
-    bool any_change = false;
-    for (int y = blockIdx.x; y < im.H; y += gridDim.x) {
-        uint8_t* jr = im.J + (size_t)y * im.pj;
-        const uint8_t* ir = im.I + (size_t)y * im.pi;
+    uint8_t* jr = im.J + (size_t)y * im.pj;
+    const uint8_t* ir = im.I + (size_t)y * im.pi;
 #pragma unroll 1
-        for (int pass = 0; pass < (DIR == 0 ? 2 : 1); ++pass) {
-            const int dir = DIR != 0 ? DIR : (pass == 0 ? 1 : -1);
-            const bool clip = CLIP && pass == 0;
-            uint32_t carry = 0;
-            for (int n = 0; n < nch; ++n) {
-                const int ch = dir > 0 ? n : nch - 1 - n;
-                const int o = (ch * kWG + tid) * VB;
-                uint32_t jw[NW], iw[NW], v[NPX], c[NPX], ov[NPX];
-                V::load(jr, o, rowbytes, jw);
-                V::load(ir, o, rowbytes, iw);
-                V::unpack(jw, v);
-                V::unpack(iw, c);
+    for (int pass = 0; pass < (DIR == 0 ? 2 : 1); ++pass) {
+        const int dir = DIR != 0 ? DIR : (pass == 0 ? 1 : -1);
+        const bool clip = CLIP && pass == 0;
+        uint32_t carry = 0;
+        for (int n = 0; n < nch; ++n) {
+            const int ch = dir > 0 ? n : nch - 1 - n;
+            const int o = (ch * kWG + tid) * VB;
+            uint32_t jw[NW], iw[NW], v[NPX], c[NPX], ov[NPX];
+            V::load(jr, o, rowbytes, jw);
+            V::load(ir, o, rowbytes, iw);
+            V::unpack(jw, v);
+            V::unpack(iw, c);
 #pragma unroll
-                for (int k = 0; k < NPX; ++k) {
-                    ov[k] = v[k];
-                    if (clip) v[k] = min(v[k], c[k]);
-                }
-                if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) {
-#pragma unroll
-                    for (int k = 0; k < NPX; ++k) c[k] = 0;
-                }
-                uint32_t P = 0, Q = 0xFFFFFFFFu;
-#pragma unroll
-                for (int kk = 0; kk < NPX; ++kk) {
-                    const int k = dir > 0 ? kk : NPX - 1 - kk;
-                    P = max(v[k], min(P, c[k]));
-                    Q = min(Q, c[k]);
-                }
-                uint32_t Pi = P, Qi = Q, Pe, Qe;
-                if (dir > 0) {
-                    scan_lanes_fwd(Pi, Qi);
-                    Pe = from_left(Pi, 0u);
-                    Qe = from_left(Qi, 0xFFFFFFFFu);
-                    if (lane == 63) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
-                } else {
-                    scan_lanes_bwd(Pi, Qi, lane);
-                    Pe = from_right(Pi, 0u);
-                    Qe = from_right(Qi, 0xFFFFFFFFu);
-                    if (lane == 0) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
-                }
-                __syncthreads();
-                uint32_t xin = carry, xall = carry;
-#pragma unroll
-                for (int uu = 0; uu < 4; ++uu) {
-                    const int u = dir > 0 ? uu : 3 - uu;
-                    const uint32_t tp = sP[n & 1][u], tq = sQ[n & 1][u];
-                    if (dir > 0 ? (u < wv) : (u > wv)) xin = max(tp, min(xin, tq));
-                    xall = max(tp, min(xall, tq));
-                }
-                carry = xall;
-                uint32_t x = max(Pe, min(xin, Qe));
-                bool d = false;
-#pragma unroll
-                for (int kk = 0; kk < NPX; ++kk) {
-                    const int k = dir > 0 ? kk : NPX - 1 - kk;
-                    const uint32_t out = max(v[k], min(x, c[k]));
-                    d |= out != ov[k];
-                    v[k] = out;
-                    x = out;
-                }
-                if (d) {
-                    V::pack(v, jw);
-                    V::store(jr, o, rowbytes, jw);
-                }
-                any_change |= d;
-                __syncthreads();  // the LDS slot pair is reused two chunks later
+            for (int k = 0; k < NPX; ++k) {
+                ov[k] = v[k];
+                if (clip) v[k] = min(v[k], c[k]);
             }
+            if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) {
+#pragma unroll
+                for (int k = 0; k < NPX; ++k) c[k] = 0;
+            }
+            uint32_t P = 0, Q = 0xFFFFFFFFu;
+#pragma unroll
+            for (int kk = 0; kk < NPX; ++kk) {
+                const int k = dir > 0 ? kk : NPX - 1 - kk;
+                P = max(v[k], min(P, c[k]));
+                Q = min(Q, c[k]);
+            }
+            uint32_t Pi = P, Qi = Q, Pe, Qe;
+            if (dir > 0) {
+                scan_lanes_fwd(Pi, Qi);
+                Pe = from_left(Pi, 0u);
+                Qe = from_left(Qi, 0xFFFFFFFFu);
+                if (lane == 63) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
+            } else {
+                scan_lanes_bwd(Pi, Qi, lane);
+                Pe = from_right(Pi, 0u);
+                Qe = from_right(Qi, 0xFFFFFFFFu);
+                if (lane == 0) { sP[n & 1][wv] = Pi; sQ[n & 1][wv] = Qi; }
+            }
+            __syncthreads();
+            uint32_t xin = carry, xall = carry;
+#pragma unroll
+            for (int uu = 0; uu < 4; ++uu) {
+                const int u = dir > 0 ? uu : 3 - uu;
+                const uint32_t tp = sP[n & 1][u], tq = sQ[n & 1][u];
+                if (dir > 0 ? (u < wv) : (u > wv)) xin = max(tp, min(xin, tq));
+                xall = max(tp, min(xall, tq));
+            }
+            carry = xall;
+            uint32_t x = max(Pe, min(xin, Qe));
+            bool d = false;
+#pragma unroll
+            for (int kk = 0; kk < NPX; ++kk) {
+                const int k = dir > 0 ? kk : NPX - 1 - kk;
+                const uint32_t out = max(v[k], min(x, c[k]));
+                d |= out != ov[k];
+                v[k] = out;
+                x = out;
+            }
+            if (d) {
+                V::pack(v, jw);
+                V::store(jr, o, rowbytes, jw);
+            }
+            any_change |= d;
+            __syncthreads();  // the LDS slot pair is reused two chunks later
         }
+    }
+}
+
+// Streaming row pass for wide rows: one chunk (256 lanes x VB bytes) in
+// registers at a time, carried along the row.  DIR = +1 / -1 sweeps one
+// direction; DIR = 0 sweeps forward then backward in the same launch, the
+// backward sweep re-reading the row this workgroup just wrote (each lane
+// reads back its own vectors, which the caches still hold), so the memory
+// traffic stays at one read of marker and mask and one write of the marker.
+template <class T, int VB, int DIR, bool CLIP>
+__global__ __launch_bounds__(kWG) void k_row_pass_long(Image im, TileMask tm, uint32_t* changed) {
+    __shared__ uint32_t sP[2][4], sQ[2][4];
+    bool any_change = false;
+    for (int y = blockIdx.x; y < im.H; y += gridDim.x)
+        row_stream<T, VB, DIR, CLIP>(im, y, tm, sP, sQ, any_change);
+    if (any_change && changed) *changed = 1u;
+}
+
+// The same over the rows of several images (global row index -> image).
+template <class T, int VB, bool CLIP>
+__global__ __launch_bounds__(kWG) void k_row_pass_batch(Descs ds, uint32_t* changed) {
+    __shared__ uint32_t sP[2][4], sQ[2][4];
+    bool any_change = false;
+    for (int r = blockIdx.x; r < ds.total; r += gridDim.x) {
+        const ImgDesc& d = desc(ds, find_row(ds, r));
+        row_stream<T, VB, 0, CLIP>(image_of(d), r - d.row_base, d.tm, sP, sQ, any_change);
     }
     if (any_change && changed) *changed = 1u;
 }
@@ -459,22 +503,19 @@ __global__ __launch_bounds__(kWG) void k_row_pass_long(Image im, TileMask tm, ui
 // uses the block's previous output row; neighbours outside the block come
 // from src, so the pass is deterministic.
 template <class T, int VB, int CONN, int DIR>
-__global__ __launch_bounds__(kWG) void k_col_pass(const uint8_t* __restrict__ S,
-                                                   uint8_t* __restrict__ D,
-                                                   const uint8_t* __restrict__ I, size_t ps,
-                                                   size_t pd, size_t pi, int W, int H, int R,
-                                                   TileMask tm, uint32_t* changed) {
+DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
+                   const uint8_t* __restrict__ I, size_t ps, size_t pd, size_t pi, int W, int H,
+                   int R, int strip, int seg, const TileMask& tm, uint32_t* changed) {
     using V = VecIO<T, VB>;
     constexpr int NW = V::NW, NPX = V::NPX;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lane = lane_id();
     const int rowbytes = W * (int)sizeof(T);
-    const int strip = blockIdx.x * 4 + wv;
     const int xb0 = strip * 64 * VB;  // first byte of the strip
     if (xb0 >= rowbytes) return;      // whole wave past the row end (no barriers below)
     const int o = xb0 + lane * VB;
     const int xs0 = xb0 / (int)sizeof(T);             // first pixel of the strip
     const int xs1 = (xb0 + 64 * VB) / (int)sizeof(T); // first pixel right of the strip
-    const int y0 = blockIdx.y * R, y1 = min(H, y0 + R);
+    const int y0 = seg * R, y1 = min(H, y0 + R);
     const bool on = o < rowbytes ? tile_on(tm, o / (int)sizeof(T), y0) : false;
     uint32_t prev[NPX];
     {
@@ -532,6 +573,19 @@ __global__ __launch_bounds__(kWG) void k_col_pass(const uint8_t* __restrict__ S,
         V::store(D + (size_t)y * pd, o, rowbytes, jw);
     }
     if (any_change && changed) *changed = 1u;
+}
+
+// Global wave w -> (image, strip, segment).  JP: read J write P, else read P
+// write J.
+template <class T, int VB, int CONN, int DIR, bool JP>
+__global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ds.total) return;
+    const ImgDesc& d = desc(ds, find_cw(ds, w));
+    const int local = w - d.cw_base;
+    const int strip = local % d.strips, seg = local / d.strips;
+    if (JP) col_block<T, VB, CONN, DIR>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+    else col_block<T, VB, CONN, DIR>(d.P, d.J, d.I, d.pp, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
@@ -882,20 +936,27 @@ DEV uint32_t tile_notify_mask(const uint32_t* s, const uint8_t* rose, const Imag
 // adjacent lanes (DPP) and, at the strip edges, from memory.  Each lane ORs
 // its pixels' instability; a ballot folds the lanes of one megapixel.
 template <class T, int VB, int CONN>
-__global__ __launch_bounds__(kWG) void k_unstable_scan(Image im, int nmx, uint8_t* flags,
+__global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_all,
                                                         uint32_t* count) {
     using V = VecIO<T, VB>;
     constexpr int NW = V::NW, NPX = V::NPX;
     constexpr int LPM = kMP / NPX;          // lanes per megapixel
     constexpr int MPS = 64 / LPM;           // megapixels per strip
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lane = lane_id();
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ds.total) return;
+    const ImgDesc& dsc = desc(ds, find_sw(ds, w));
+    const Image im = image_of(dsc);
+    const int nmx = dsc.nmx;
+    uint8_t* flags = flags_all + dsc.mp_base;
+    const int local = w - dsc.sw_base;
+    const int strip = local % dsc.strips, my = local / dsc.strips;
     const int rowbytes = im.W * (int)sizeof(T);
-    const int strip = blockIdx.x * 4 + wv;
     const int xb0 = strip * 64 * VB;
     if (xb0 >= rowbytes) return;
     const int o = xb0 + lane * VB;
     const int xs0 = xb0 / (int)sizeof(T), xs1 = (xb0 + 64 * VB) / (int)sizeof(T);
-    const int my = blockIdx.y, y0 = my * kMP, y1 = min(im.H, y0 + kMP);
+    const int y0 = my * kMP, y1 = min(im.H, y0 + kMP);
     auto load_row = [&](const uint8_t* base, size_t pitch, int y, uint32_t (&px)[NPX]) {
         uint32_t w[NW];
         if (y >= 0 && y < im.H) V::load(base + (size_t)y * pitch, o, rowbytes, w);
@@ -1074,7 +1135,7 @@ DEV bool ring_push_many(const QueueArgs& q, bool mine, uint32_t value) {
 }
 
 template <class T, int CONN>
-__global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
+__global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
     extern __shared__ uint32_t lds[];
     const int lane = lane_id();
     uint32_t* s = lds + (threadIdx.x >> 6) * (WAVE_LDS / 4);
@@ -1109,7 +1170,10 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
         m = first_lane(m);
         mem_wait();
         pt.lap(0);
-        const int mx = (int)m % q.nmx, my = (int)m / q.nmx;
+        const ImgDesc& dsc = desc(ds, find_mp(ds, (int)m));
+        const Image im = image_of(dsc);
+        const int ml = (int)m - dsc.mp_base;
+        const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
         tile_fetch<T>(im, mx, my, s);
         pt.lap(1);
         const bool conv = tile_converge<CONN>(s, 4 * q.max_iters);
@@ -1125,7 +1189,7 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Image im, QueueArgs q) {
         bool push = false;
         uint32_t target = 0;
         if (lane < 9 && ((nb >> lane) & 1u)) {
-            target = (uint32_t)neighbour_index((int)m, q.nmx, 1u << lane);
+            target = (uint32_t)(dsc.mp_base + neighbour_index(ml, dsc.nmx, 1u << lane));
             const uint32_t old = a_or(&q.st[target], ST_PENDING);
             push = old == 0u;
         }
@@ -1180,7 +1244,7 @@ DEV bool grid_barrier(uint32_t* ctl, uint32_t nblocks, unsigned long long limit)
 }
 
 template <class T, int CONN, int KIND>
-__global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
+__global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
     constexpr bool EXCL = KIND == Q_UQ;
     extern __shared__ uint32_t lds[];
     const int lane = lane_id();
@@ -1206,7 +1270,10 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
             m = first_lane(m);
             mem_wait();
             pt.lap(0);
-            const int mx = (int)m % q.nmx, my = (int)m / q.nmx;
+            const ImgDesc& dsc = desc(ds, find_mp(ds, (int)m));
+            const Image im = image_of(dsc);
+            const int ml = (int)m - dsc.mp_base;
+            const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
             tile_fetch<T>(im, mx, my, s);
             pt.lap(1);
             const bool conv = tile_converge<CONN>(s, 4 * q.max_iters);
@@ -1223,7 +1290,7 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Image im, QueueArgs q) {
             bool push = false;
             uint32_t target = 0;
             if (lane < 9 && ((nb >> lane) & 1u)) {
-                target = (uint32_t)neighbour_index((int)m, q.nmx, 1u << lane);
+                target = (uint32_t)(dsc.mp_base + neighbour_index(ml, dsc.nmx, 1u << lane));
                 push = EXCL ? (a_or(&q.st[target], 1u) == 0u) : true;
             }
             const uint64_t b = ballot(push);
@@ -1348,8 +1415,10 @@ hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask
 
 // 16-byte lanes need 16-byte aligned rows; narrow images use 4-byte lanes so
 // that there are enough column strips to fill the GPU.
-static bool col_wide(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
-                     size_t pi, int W, int dtype) {
+// 16-byte lanes need 16-byte aligned rows; narrow images use 4-byte lanes so
+// that there are enough column strips to fill the GPU.
+bool col_wide(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
+              size_t pi, int W, int dtype) {
     return aligned16(src, ps) && aligned16(dst, pd) && aligned16(mask, pi) &&
            (size_t)W * dtype >= 16384;
 }
@@ -1359,62 +1428,72 @@ int col_block_width(const void* src, const void* dst, const void* mask, size_t p
     return 64 * (col_wide(src, dst, mask, ps, pd, pi, W, dtype) ? 16 : 4) / dtype;
 }
 
+int col_strips(int W, int dtype, bool wide) {
+    const int vb = wide ? 16 : 4;
+    return (W * dtype + 64 * vb - 1) / (64 * vb);
+}
+
 int col_auto_rows(int W, int H, int dtype, bool wide) {
-    const long strips = ((long)W * dtype + 64 * (wide ? 16 : 4) - 1) / (64 * (wide ? 16 : 4));
+    const long strips = col_strips(W, dtype, wide);
     long r = strips * (long)H / 8192;  // aim at >= 8192 waves
     r = r / 32 * 32;
     return (int)std::min<long>(512, std::max<long>(32, r));
 }
 
-template <class T, int VB>
-static void col_pass_t(const uint8_t* src, uint8_t* dst, const uint8_t* mask, size_t ps,
-                       size_t pd, size_t pi, int W, int H, int conn, int dir, int R,
-                       const TileMask& tm, uint32_t* changed, hipStream_t s) {
-    const int rowbytes = W * (int)sizeof(T);
-    const int strips = (rowbytes + 64 * VB - 1) / (64 * VB);
-    const dim3 grid((strips + 3) / 4, (H + R - 1) / R);
-#define IWPP_COL(C, D)                                                                      \
-    hipLaunchKernelGGL((k_col_pass<T, VB, C, D>), grid, kWG, 0, s, src, dst, mask, ps, pd, pi, \
-                       W, H, R, tm, changed)
-    if (conn == 8) {
-        if (dir > 0) IWPP_COL(8, 1);
-        else IWPP_COL(8, -1);
-    } else {
-        if (dir > 0) IWPP_COL(4, 1);
-        else IWPP_COL(4, -1);
-    }
-#undef IWPP_COL
-}
-
-hipError_t launch_col_pass(const uint8_t* src, uint8_t* dst, const uint8_t* mask, size_t ps,
-                           size_t pd, size_t pi, int W, int H, int dtype, int conn, int dir,
-                           int R, const TileMask& tm, uint32_t* changed, hipStream_t s) {
-    const bool v16 = col_wide(src, dst, mask, ps, pd, pi, W, dtype);
-    if (R <= 0) R = col_auto_rows(W, H, dtype, v16);
+hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, uint32_t* changed,
+                                 hipStream_t s) {
+    if (ds.total <= 0) return hipSuccess;
+    const int grid = std::min(ds.total, 65535);
     if (dtype == 1) {
-        if (v16) col_pass_t<uint8_t, 16>(src, dst, mask, ps, pd, pi, W, H, conn, dir, R, tm, changed, s);
-        else col_pass_t<uint8_t, 4>(src, dst, mask, ps, pd, pi, W, H, conn, dir, R, tm, changed, s);
+        if (clip) hipLaunchKernelGGL((k_row_pass_batch<uint8_t, 4, true>), grid, kWG, 0, s, ds, changed);
+        else hipLaunchKernelGGL((k_row_pass_batch<uint8_t, 4, false>), grid, kWG, 0, s, ds, changed);
     } else {
-        if (v16) col_pass_t<uint16_t, 16>(src, dst, mask, ps, pd, pi, W, H, conn, dir, R, tm, changed, s);
-        else col_pass_t<uint16_t, 4>(src, dst, mask, ps, pd, pi, W, H, conn, dir, R, tm, changed, s);
+        if (clip) hipLaunchKernelGGL((k_row_pass_batch<uint16_t, 4, true>), grid, kWG, 0, s, ds, changed);
+        else hipLaunchKernelGGL((k_row_pass_batch<uint16_t, 4, false>), grid, kWG, 0, s, ds, changed);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_unstable(const Image& im, int dtype, int conn, uint8_t* flags,
-                           uint32_t* count, hipStream_t s) {
-    const int nmx = (im.W + kMP - 1) / kMP, nmy = (im.H + kMP - 1) / kMP;
-    const bool v16 = col_wide(im.J, im.J, im.I, im.pj, im.pj, im.pi, im.W, dtype);
-    const int vb = v16 ? 16 : 4;
-    const int strips = (im.W * dtype + 64 * vb - 1) / (64 * vb);
-    const dim3 grid((strips + 3) / 4, nmy);
-#define IWPP_UN(TT, VBB, C) \
-    hipLaunchKernelGGL((k_unstable_scan<TT, VBB, C>), grid, kWG, 0, s, im, nmx, flags, count)
+template <class T, int VB>
+static void col_pass_t(const Descs& ds, int conn, int dir, bool jp, uint32_t* changed,
+                       hipStream_t s) {
+    const int grid = (ds.total + 3) / 4;
+#define IWPP_COL(C, D, JP) \
+    hipLaunchKernelGGL((k_col_pass<T, VB, C, D, JP>), grid, kWG, 0, s, ds, changed)
+    if (conn == 8) {
+        if (dir > 0) { if (jp) IWPP_COL(8, 1, true); else IWPP_COL(8, 1, false); }
+        else { if (jp) IWPP_COL(8, -1, true); else IWPP_COL(8, -1, false); }
+    } else {
+        if (dir > 0) { if (jp) IWPP_COL(4, 1, true); else IWPP_COL(4, 1, false); }
+        else { if (jp) IWPP_COL(4, -1, true); else IWPP_COL(4, -1, false); }
+    }
+#undef IWPP_COL
+}
+
+hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, bool j_to_p,
+                                 bool wide, uint32_t* changed, hipStream_t s) {
+    if (ds.total <= 0) return hipSuccess;
     if (dtype == 1) {
-        if (v16) { if (conn == 8) IWPP_UN(uint8_t, 16, 8); else IWPP_UN(uint8_t, 16, 4); }
+        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, j_to_p, changed, s);
+        else col_pass_t<uint8_t, 4>(ds, conn, dir, j_to_p, changed, s);
+    } else {
+        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, j_to_p, changed, s);
+        else col_pass_t<uint16_t, 4>(ds, conn, dir, j_to_p, changed, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_unstable_batch(const Descs& ds, int dtype, int conn, bool wide,
+                                 uint8_t* flags, uint32_t* count, hipStream_t s) {
+    if (ds.total <= 0) return hipSuccess;
+    const int grid = (ds.total + 3) / 4;
+#define IWPP_UN(TT, VBB, C) \
+    hipLaunchKernelGGL((k_unstable_scan<TT, VBB, C>), grid, kWG, 0, s, ds, flags, count)
+    if (dtype == 1) {
+        if (wide) { if (conn == 8) IWPP_UN(uint8_t, 16, 8); else IWPP_UN(uint8_t, 16, 4); }
         else { if (conn == 8) IWPP_UN(uint8_t, 4, 8); else IWPP_UN(uint8_t, 4, 4); }
     } else {
-        if (v16) { if (conn == 8) IWPP_UN(uint16_t, 16, 8); else IWPP_UN(uint16_t, 16, 4); }
+        if (wide) { if (conn == 8) IWPP_UN(uint16_t, 16, 8); else IWPP_UN(uint16_t, 16, 4); }
         else { if (conn == 8) IWPP_UN(uint16_t, 4, 8); else IWPP_UN(uint16_t, 4, 4); }
     }
 #undef IWPP_UN
@@ -1448,7 +1527,7 @@ hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const Queu
 }
 
 template <class T, int CONN>
-static hipError_t queue_phase_t(const Image& im, int kind, const QueueArgs& q, int wpc,
+static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, int wpc,
                                 hipStream_t s) {
     const void* fn;
     if (kind == Q_MUTEX) fn = (const void*)k_queue_mutex<T, CONN>;
@@ -1461,26 +1540,25 @@ static hipError_t queue_phase_t(const Image& im, int kind, const QueueArgs& q, i
     if (per_cu <= 0) return hipErrorInvalidConfiguration;
     if (wpc > 0) per_cu = std::min(per_cu, wpc);
     int grid = per_cu * cu_count();
-    const int nmp = q.nmx * q.nmy;
-    grid = std::max(1, std::min(grid, (nmp + 3) / 4));
-    Image imc = im;
+    grid = std::max(1, std::min(grid, (q.nmp + 3) / 4));
+    Descs dc = ds;
     QueueArgs qc = q;
     if (kind == Q_MUTEX) {
-        hipLaunchKernelGGL((k_queue_mutex<T, CONN>), grid, kWG, lds_bytes, s, imc, qc);
+        hipLaunchKernelGGL((k_queue_mutex<T, CONN>), grid, kWG, lds_bytes, s, dc, qc);
         return hipGetLastError();
     }
-    void* args[] = {&imc, &qc};
+    void* args[] = {&dc, &qc};
     return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, lds_bytes, s);
 }
 
-hipError_t launch_queue_phase(const Image& im, int dtype, int conn, int kind, const QueueArgs& q,
-                              int wpc, hipStream_t s) {
+hipError_t launch_queue_phase(const Descs& ds, int dtype, int conn, int kind,
+                              const QueueArgs& q, int wpc, hipStream_t s) {
     if (dtype == 1) {
-        return conn == 8 ? queue_phase_t<uint8_t, 8>(im, kind, q, wpc, s)
-                         : queue_phase_t<uint8_t, 4>(im, kind, q, wpc, s);
+        return conn == 8 ? queue_phase_t<uint8_t, 8>(ds, kind, q, wpc, s)
+                         : queue_phase_t<uint8_t, 4>(ds, kind, q, wpc, s);
     }
-    return conn == 8 ? queue_phase_t<uint16_t, 8>(im, kind, q, wpc, s)
-                     : queue_phase_t<uint16_t, 4>(im, kind, q, wpc, s);
+    return conn == 8 ? queue_phase_t<uint16_t, 8>(ds, kind, q, wpc, s)
+                     : queue_phase_t<uint16_t, 4>(ds, kind, q, wpc, s);
 }
 
 }  // namespace iwpp

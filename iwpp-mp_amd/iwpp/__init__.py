@@ -71,6 +71,13 @@ SIGNATURES = {
     "iwpp_sweep_cols": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _VP, _VP]),
     "iwpp_sweep_cols_block_width": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I]),
     "iwpp_unstable_megapixels": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _VP, _VP, _VP]),
+    "iwpp_reconstruct_batch": (_I, [_VP, _I, _I, _I, _I, _I, _I, ctypes.POINTER(Options),
+                                    ctypes.POINTER(Stats), _VP, _VP]),
+    "iwpp_batch_graph_create": (_VP, [_VP, _I, _I, _I, _I, _I, _I, ctypes.POINTER(Options),
+                                      _VP, _VP]),
+    "iwpp_batch_graph_launch": (_I, [_VP, _VP]),
+    "iwpp_batch_graph_stats": (_I, [_VP, ctypes.POINTER(Stats), _VP]),
+    "iwpp_batch_graph_destroy": (None, [_VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
     "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
 }
@@ -215,6 +222,86 @@ def reconstruct(marker, mask, connectivity=8, tile=0, queue="uq", schedule="plai
                                    workspace.handle if workspace else None, _stream(stream))
     _raise(rc)
     return st.as_dict()
+
+
+class ImageDesc(ctypes.Structure):
+    _fields_ = [("marker", ctypes.c_void_p), ("mask", ctypes.c_void_p),
+                ("pitch_marker", ctypes.c_size_t), ("pitch_mask", ctypes.c_size_t),
+                ("width", ctypes.c_int), ("height", ctypes.c_int)]
+
+
+def _batch_array(pairs):
+    arr = (ImageDesc * len(pairs))()
+    dtype = pairs[0][0].dtype
+    for k, (mk, ms) in enumerate(pairs):
+        assert mk.h == ms.h and mk.w == ms.w and mk.dtype == ms.dtype == dtype
+        arr[k] = ImageDesc(mk.ptr, ms.ptr, mk.pitch, ms.pitch, mk.w, mk.h)
+    return arr, dtype
+
+
+def reconstruct_batch(pairs, connectivity=8, tile=0, queue="auto", schedule="plain",
+                      options=None, workspace=None, stream=None):
+    """In-place reconstruction of a list of (marker, mask) DeviceImage pairs in
+    one set of launches.  Returns the counters as a dict."""
+    arr, dtype = _batch_array(pairs)
+    st = Stats()
+    opt = options if options is not None else default_options()
+    rc = lib().iwpp_reconstruct_batch(arr, len(pairs), dtype, connectivity, tile,
+                                      QUEUES[queue] if isinstance(queue, str) else queue,
+                                      SCHEDULES[schedule] if isinstance(schedule, str) else schedule,
+                                      ctypes.byref(opt), ctypes.byref(st),
+                                      workspace.handle if workspace else None, _stream(stream))
+    _raise(rc)
+    return st.as_dict()
+
+
+class BatchGraph:
+    """A batch reconstruction captured into a hipGraph (mutex queue variant)."""
+
+    def __init__(self, pairs, workspace, connectivity=8, tile=0, schedule="plain", options=None,
+                 stream=None):
+        arr, dtype = _batch_array(pairs)
+        self._arr = arr
+        self.workspace = workspace
+        # capture needs a stream other than the NULL stream (default priority)
+        self.stream = stream if stream is not None else _torch().cuda.Stream()
+        opt = options if options is not None else default_options()
+        _torch().cuda.synchronize()
+        self.handle = lib().iwpp_batch_graph_create(
+            arr, len(pairs), dtype, connectivity, tile, QUEUES["mutex"],
+            SCHEDULES[schedule] if isinstance(schedule, str) else schedule, ctypes.byref(opt),
+            workspace.handle, self._sid())
+        if not self.handle:
+            msg = lib().iwpp_last_error()
+            raise IWPPError(-2, msg.decode() if msg else "graph capture failed")
+
+    def _sid(self):
+        st = self.stream
+        return st.cuda_stream if hasattr(st, "cuda_stream") else int(st)
+
+    def launch(self):
+        """Replays the graph on the graph's stream (ordered after all work
+        already queued on the current stream)."""
+        torch = _torch()
+        if hasattr(self.stream, "wait_stream"):
+            self.stream.wait_stream(torch.cuda.current_stream())
+        _raise(lib().iwpp_batch_graph_launch(self.handle, self._sid()))
+
+    def stats(self):
+        st = Stats()
+        _raise(lib().iwpp_batch_graph_stats(self.handle, ctypes.byref(st), self._sid()))
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().iwpp_batch_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def clip(marker, mask, stream=None):

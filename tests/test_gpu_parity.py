@@ -225,3 +225,63 @@ def test_config3_full(torch_cuda):
 
 def test_config5_full(torch_cuda):
     _full_case_check(W.c5(), ("mutex", "uq"))
+
+
+def _batch_inputs(rng, shapes, dtype):
+    cases = []
+    for i, (h, w) in enumerate(shapes):
+        kind = ("random", "seeds", "hdome")[i % 3]
+        cases.append(rand_pair(rng, h, w, dtype, kind))
+    return cases
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_batch_random(torch_cuda, queue, conn, dtype):
+    rng = np.random.default_rng(11)
+    shapes = [(1, 1), (1, 300), (300, 1), (64, 64), (65, 129), (200, 333), (640, 257),
+              (77, 1500), (1029, 70), (512, 512)]
+    cases = _batch_inputs(rng, shapes, dtype)
+    for sweeps in (0, 2):
+        say("batch", queue, conn, dtype.__name__, sweeps)
+        pairs = [(up(J), up(I)) for J, I in cases]
+        st = iwpp.reconstruct_batch(pairs, conn, 0, queue, "plain",
+                                    iwpp.default_options(sweep_rounds=sweeps))
+        for (J, I), (dj, _) in zip(cases, pairs):
+            assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn)), (J.shape, st)
+
+
+def test_batch_graph_replays(torch_cuda):
+    rng = np.random.default_rng(12)
+    shapes = [(300, 200), (64, 700), (513, 65), (128, 128), (1, 50)]
+    cases = _batch_inputs(rng, shapes, np.uint8)
+    wants = [oracle.reconstruct(J, I, 8) for J, I in cases]
+    pairs = [(up(J), up(I)) for J, I in cases]
+    inputs = [up(J) for J, _ in cases]
+    ws = iwpp.Workspace()
+    g = iwpp.BatchGraph(pairs, ws, 8, options=iwpp.default_options(sweep_rounds=1))
+    for rep in range(4):
+        say("graph replay", rep)
+        for (dj, _), src in zip(pairs, inputs):
+            dj.buf.copy_(src.buf)
+        # unrelated eager work between replays
+        scratch = torch_cuda.ones(1 << 20, device="cuda").sum()
+        torch_cuda.cuda.synchronize()
+        g.launch()
+        st = g.stats()
+        for (dj, _), want in zip(pairs, wants):
+            assert np.array_equal(dj.numpy(), want), (rep, st)
+        assert float(scratch.item()) == float(1 << 20)
+    g.close()
+
+
+def test_config4_batch(torch_cuda):
+    cases = W.c4(n=48, max_side=1024)
+    wants = [oracle.reconstruct(c.marker, c.mask, 8) for c in cases]
+    for queue in ("mutex", "uq"):
+        say("c4", queue)
+        pairs = [(up(c.marker), up(c.mask)) for c in cases]
+        st = iwpp.reconstruct_batch(pairs, 8, 0, queue)
+        for (dj, _), want, c in zip(pairs, wants, cases):
+            assert np.array_equal(dj.numpy(), want), (c.name, st)
