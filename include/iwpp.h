@@ -154,6 +154,22 @@ int iwpp_batch_graph_launch(iwpp_graph* g, void* stream);
 int iwpp_batch_graph_stats(iwpp_graph* g, iwpp_stats* stats, void* stream);
 void iwpp_batch_graph_destroy(iwpp_graph* g);
 
+/* Autotuner.  Times every (queue variant, sweep rounds 0..4) on these very
+ * images (best of `trials` runs each, HIP events), restores the input markers
+ * afterwards, and remembers the winner for this workload class (dtype,
+ * connectivity, single/batch, log2 pixels, log2 widest side): later calls
+ * with queue = IWPP_QUEUE_AUTO and sweep_rounds = -1 use it.  No reference
+ * counterpart (the reference fixes everything at build time). */
+typedef struct {
+    int queue;
+    int sweep_rounds;
+    int tile;
+    float ms;  /* best measured time of one reconstruction */
+} iwpp_tuning;
+int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int connectivity, int trials,
+                  const iwpp_options* opt, iwpp_tuning* best, iwpp_workspace* ws, void* stream);
+void iwpp_autotune_clear(void);
+
 /* marker = min(marker, mask) (IWPP MP/src/MorphologicalReconstruction.cu:827-836). */
 int iwpp_clip(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask, int width,
               int height, int dtype, void* stream);

@@ -453,6 +453,15 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
     int rc = plan_schedule(n == 1 ? imgs[0].width : maxw, n == 1 ? imgs[0].height : maxh,
                            r.dtype, r.conn, tile, queue, schedule, r.opt.sweep_rounds, &plan);
     if (rc) return fail(IWPP_ERR_ARG, "no schedule for this configuration");
+    {  // measured choices of the autotuner override the heuristic defaults
+        long long px = 0;
+        for (int i = 0; i < n; ++i) px += (long long)imgs[i].width * imgs[i].height;
+        int tq = 0, tsw = 0;
+        if (tune_lookup(tune_key(r.dtype, r.conn, n, px, std::max(maxw, maxh)), &tq, &tsw)) {
+            if (queue == IWPP_QUEUE_AUTO) plan.queue = r.graph ? Q_MUTEX : tq;
+            if (r.opt.sweep_rounds < 0) plan.sweep_rounds = tsw;
+        }
+    }
     r.queue = plan.queue;
     r.st.queue = plan.queue;
     r.st.tile = plan.tile;
@@ -666,6 +675,94 @@ extern "C" void iwpp_batch_graph_destroy(iwpp_graph* g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
 }
+
+extern "C" int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int conn, int trials,
+                             const iwpp_options* opt, iwpp_tuning* best, iwpp_workspace* ws,
+                             void* stream) {
+    g_err.clear();
+    if (!imgs || n <= 0) return fail(IWPP_ERR_ARG, "empty batch");
+    int rc = check_common(dtype, conn, 0, IWPP_QUEUE_AUTO, IWPP_SCHED_PLAIN);
+    if (rc) return rc;
+    long long px = 0;
+    int maxside = 0;
+    size_t snap_bytes = 0;
+    for (int i = 0; i < n; ++i) {
+        rc = check_image(imgs[i].marker, imgs[i].mask, imgs[i].pitch_marker, imgs[i].pitch_mask,
+                         imgs[i].width, imgs[i].height, dtype);
+        if (rc) return rc;
+        px += (long long)imgs[i].width * imgs[i].height;
+        maxside = std::max(maxside, std::max(imgs[i].width, imgs[i].height));
+        snap_bytes += (size_t)imgs[i].width * dtype * imgs[i].height;
+    }
+    if (!ws) ws = default_ws();
+    if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
+    hipStream_t s = (hipStream_t)stream;
+    // snapshot of the input markers (packed rows), restored before every trial
+    uint8_t* snap = nullptr;
+    HIP_TRY(hipMalloc((void**)&snap, std::max<size_t>(snap_bytes, 1)));
+    auto copy_all = [&](bool to_snap) -> hipError_t {
+        size_t off = 0;
+        for (int i = 0; i < n; ++i) {
+            const size_t row = (size_t)imgs[i].width * dtype;
+            hipError_t e = to_snap
+                ? hipMemcpy2DAsync(snap + off, row, imgs[i].marker, imgs[i].pitch_marker, row,
+                                   imgs[i].height, hipMemcpyDeviceToDevice, s)
+                : hipMemcpy2DAsync(imgs[i].marker, imgs[i].pitch_marker, snap + off, row, row,
+                                   imgs[i].height, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return e;
+            off += row * imgs[i].height;
+        }
+        return hipSuccess;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = copy_all(true);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    iwpp_tuning top{IWPP_QUEUE_MUTEX, 0, 0, 1e30f};
+    const int queues[3] = {IWPP_QUEUE_MUTEX, IWPP_QUEUE_DQ, IWPP_QUEUE_UQ};
+    for (int qi = 0; qi < 3 && he == hipSuccess && rc == IWPP_OK; ++qi) {
+        for (int sw = 0; sw <= 4 && he == hipSuccess && rc == IWPP_OK; ++sw) {
+            float best_ms = 1e30f;
+            for (int t = 0; t < std::max(1, trials) && rc == IWPP_OK; ++t) {
+                if ((he = copy_all(false)) != hipSuccess) break;
+                if ((he = hipEventRecord(e0, s)) != hipSuccess) break;
+                Run r{};
+                r.dtype = dtype;
+                r.conn = conn;
+                r.ws = ws;
+                r.s = s;
+                iwpp_options o;
+                iwpp_default_options(&o);
+                if (opt) o = *opt;
+                o.sweep_rounds = sw;
+                rc = reconstruct_run(r, imgs, n, 0, queues[qi], IWPP_SCHED_PLAIN, &o);
+                if (rc) break;
+                if ((he = hipEventRecord(e1, s)) != hipSuccess) break;
+                if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
+                float ms = 0;
+                if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+                best_ms = std::min(best_ms, ms);
+            }
+            if (best_ms < top.ms) top = iwpp_tuning{queues[qi], sw, 0, best_ms};
+        }
+    }
+    if (he == hipSuccess) he = copy_all(false);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(snap);
+    if (rc) return rc;
+    if (he != hipSuccess) return fail(IWPP_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+    SchedulePlan plan;
+    plan_schedule(maxside, maxside, dtype, conn, 0, top.queue, IWPP_SCHED_PLAIN, top.sweep_rounds,
+                  &plan);
+    top.tile = plan.tile;
+    tune_store(tune_key(dtype, conn, n, px, maxside), top.queue, top.sweep_rounds);
+    if (best) *best = top;
+    return IWPP_OK;
+}
+
+extern "C" void iwpp_autotune_clear(void) { tune_clear(); }
 
 extern "C" int iwpp_clip(void* marker, const void* mask, size_t pm, size_t pk, int w, int h,
                          int dtype, void* stream) {

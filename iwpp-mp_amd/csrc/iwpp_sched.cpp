@@ -7,10 +7,50 @@
 #include "iwpp_sched.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "iwpp.h"
 
 namespace iwpp {
+
+namespace {
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune;
+int ilog2(long long v) {
+    int k = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++k;
+    }
+    return k;
+}
+}  // namespace
+
+bool TuneKey::operator<(const TuneKey& o) const {
+    return std::tie(dtype, conn, batch, log2_px, log2_side) <
+           std::tie(o.dtype, o.conn, o.batch, o.log2_px, o.log2_side);
+}
+TuneKey tune_key(int dtype, int conn, int n, long long px, int max_side) {
+    return TuneKey{dtype, conn, n > 1 ? 1 : 0, ilog2(px), ilog2(max_side)};
+}
+void tune_store(const TuneKey& k, int queue, int sweep_rounds) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[k] = {queue, sweep_rounds};
+}
+bool tune_lookup(const TuneKey& k, int* queue, int* sweep_rounds) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(k);
+    if (it == g_tune.end()) return false;
+    *queue = it->second.first;
+    *sweep_rounds = it->second.second;
+    return true;
+}
+void tune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
 
 int plan_schedule(int w, int h, int dtype, int conn, int tile, int queue, int schedule,
                   int sweep_rounds, SchedulePlan* out) {

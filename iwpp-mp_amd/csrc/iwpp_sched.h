@@ -16,4 +16,16 @@ struct SchedulePlan {
 int plan_schedule(int w, int h, int dtype, int conn, int tile, int queue, int schedule,
                   int sweep_rounds, SchedulePlan* out);
 
+// Autotuner cache: the measured best (queue, sweep rounds) for a workload key
+// (dtype, connectivity, single image or batch, log2 of the pixel count, log2 of
+// the widest side).  plan_schedule uses it for queue = AUTO / sweep_rounds < 0.
+struct TuneKey {
+    int dtype, conn, batch, log2_px, log2_side;
+    bool operator<(const TuneKey& o) const;
+};
+TuneKey tune_key(int dtype, int conn, int n, long long px, int max_side);
+void tune_store(const TuneKey& k, int queue, int sweep_rounds);
+bool tune_lookup(const TuneKey& k, int* queue, int* sweep_rounds);
+void tune_clear();
+
 }  // namespace iwpp

@@ -78,6 +78,8 @@ SIGNATURES = {
     "iwpp_batch_graph_launch": (_I, [_VP, _VP]),
     "iwpp_batch_graph_stats": (_I, [_VP, ctypes.POINTER(Stats), _VP]),
     "iwpp_batch_graph_destroy": (None, [_VP]),
+    "iwpp_autotune": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(Options), _VP, _VP, _VP]),
+    "iwpp_autotune_clear": (None, []),
     "iwpp_last_error": (ctypes.c_char_p, []),
     "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
 }
@@ -253,6 +255,24 @@ def reconstruct_batch(pairs, connectivity=8, tile=0, queue="auto", schedule="pla
                                       workspace.handle if workspace else None, _stream(stream))
     _raise(rc)
     return st.as_dict()
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("queue", ctypes.c_int), ("sweep_rounds", ctypes.c_int), ("tile", ctypes.c_int),
+                ("ms", ctypes.c_float)]
+
+
+def autotune(pairs, connectivity=8, trials=2, options=None, workspace=None, stream=None):
+    """Measures every (queue, sweep rounds) on these images, restores the
+    markers, and caches the winner for queue='auto' calls.  Returns it."""
+    arr, dtype = _batch_array(pairs)
+    t = Tuning()
+    opt = options if options is not None else default_options()
+    _raise(lib().iwpp_autotune(arr, len(pairs), dtype, connectivity, trials, ctypes.byref(opt),
+                               ctypes.byref(t), workspace.handle if workspace else None,
+                               _stream(stream)))
+    inv = {v: k for k, v in QUEUES.items()}
+    return {"queue": inv[t.queue], "sweep_rounds": t.sweep_rounds, "tile": t.tile, "ms": t.ms}
 
 
 class BatchGraph:

@@ -285,3 +285,21 @@ def test_config4_batch(torch_cuda):
         st = iwpp.reconstruct_batch(pairs, 8, 0, queue)
         for (dj, _), want, c in zip(pairs, wants, cases):
             assert np.array_equal(dj.numpy(), want), (c.name, st)
+
+
+def test_autotune_restores_markers_and_tunes(torch_cuda):
+    rng = np.random.default_rng(13)
+    cases = _batch_inputs(rng, [(200, 300), (64, 64), (130, 517)], np.uint8)
+    pairs = [(up(J), up(I)) for J, I in cases]
+    before = [dj.numpy() for dj, _ in pairs]
+    say("autotune")
+    best = iwpp.autotune(pairs, 8, 1)
+    assert best["queue"] in ("mutex", "dq", "uq") and 0 <= best["sweep_rounds"] <= 4
+    for (dj, _), b in zip(pairs, before):
+        assert np.array_equal(dj.numpy(), b)
+    st = iwpp.reconstruct_batch(pairs, 8, 0, "auto")
+    assert st["queue"] == {"mutex": 0, "dq": 1, "uq": 2}[best["queue"]]
+    assert st["sweep_rounds"] == best["sweep_rounds"]
+    for (J, I), (dj, _) in zip(cases, pairs):
+        assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, 8))
+    iwpp.lib().iwpp_autotune_clear()
