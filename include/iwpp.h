@@ -71,6 +71,7 @@ typedef struct {
     int queue;               /* queue variant actually used */
     int tile;                /* tile size actually used */
     int sweeps_skipped_tiles;/* tile-rounds skipped by the schedule */
+    int schedule;            /* schedule actually used (LEARNED needs a model) */
     float ms_sweep;          /* device time of the sweep phase (HIP events) */
     float ms_seed;           /* device time of the seeding passes */
     float ms_queue;          /* device time of the queue-phase kernels */
@@ -169,6 +170,52 @@ typedef struct {
 int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int connectivity, int trials,
                   const iwpp_options* opt, iwpp_tuning* best, iwpp_workspace* ws, void* stream);
 void iwpp_autotune_clear(void);
+
+/* Learned schedule (IWPP_SCHED_LEARNED).  No reference counterpart: the
+ * reference hard-codes its sweep/queue split.  A small MLP (16 tile features
+ * -> 64 -> 64 -> 2, ReLU) predicts for each tile of `tile` x `tile` pixels
+ * (0) how many sweep rounds it takes before the queue phase (rounded,
+ * clamped to 0..4) and (1) its seeding priority.  Inference runs with BF16
+ * weights and FP32 accumulation.  The schedule changes only which tiles the
+ * sweeps touch and the order in which megapixels enter the queue, never the
+ * result.  Without a model IWPP_SCHED_LEARNED behaves as IWPP_SCHED_PLAIN.
+ *
+ * Parameter layout (IWPP_MLP_PARAMS floats): W1[64][16] b1[64] W2[64][64]
+ * b2[64] W3[2][64] b3[2]. */
+#define IWPP_MLP_PARAMS 5378
+#define IWPP_MLP_FEATURES 16
+
+/* Installs the model for this process (host FP32 weights, rounded to BF16;
+ * n must be IWPP_MLP_PARAMS).  weights = NULL removes it. */
+int iwpp_sched_set_model(const float* weights, int n);
+
+/* Tile features of one image (device pointers), as a reconstruction with
+ * IWPP_SCHED_LEARNED computes them (there with sub = 4): raw[t][16]
+ * statistics over every `sub`-th row and feat[t][16] the normalised features
+ * the model reads, t over ceil(h/tile) x ceil(w/tile) tiles in row-major
+ * order.  Scratch comes from ws (NULL: the device's default workspace). */
+int iwpp_tile_features(const void* marker, const void* mask, size_t pitch_marker,
+                       size_t pitch_mask, int width, int height, int dtype, int connectivity,
+                       int tile, int sub, float* raw, float* feat, iwpp_workspace* ws,
+                       void* stream);
+
+/* MLP on device buffers.  params: IWPP_MLP_PARAMS values, BF16 (uint16) when
+ * bf16 != 0 else FP32.  y[n][2] = MLP(x[n][16]); acts (may be NULL) receives
+ * the hidden activations h1[64] h2[64] per sample for the backward pass. */
+int iwpp_mlp_forward(const void* params, int bf16, const float* x, int n, float* y, float* acts,
+                     void* stream);
+
+/* Gradient of L = sum_s (lw0 (y0-t0)^2 + lw1 (y1-t1)^2) / (2n) with respect to
+ * the parameters (grad: IWPP_MLP_PARAMS floats), from the forward pass's y
+ * and acts.  work: n * 130 floats of scratch. */
+int iwpp_mlp_backward(const void* params, int bf16, const float* x, const float* acts,
+                      const float* y, const float* target, int n, float lw0, float lw1,
+                      float* work, float* grad, void* stream);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected for step
+ * >= 1) on FP32 master weights w with moments m, v; writes the BF16 copy wbf. */
+int iwpp_mlp_adam(float* w, float* m, float* v, const float* grad, uint16_t* wbf, float lr,
+                  int step, void* stream);
 
 /* marker = min(marker, mask) (IWPP MP/src/MorphologicalReconstruction.cu:827-836). */
 int iwpp_clip(void* marker, const void* mask, size_t pitch_marker, size_t pitch_mask, int width,

@@ -9,6 +9,7 @@
 // rises and never passes the reconstruction, so any partial marker is a valid
 // restart point.
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +20,7 @@
 
 #include "iwpp.h"
 #include "iwpp_internal.h"
+#include "iwpp_learned.h"
 #include "iwpp_sched.h"
 
 using namespace iwpp;
@@ -91,6 +93,18 @@ struct iwpp_workspace {
     size_t descs_n = 0;
     uint32_t* img_of_mp = nullptr;  // global megapixel -> image (device)
     size_t img_of_mp_n = 0;
+    float* feat_part = nullptr; // learned schedule: per-slab partial statistics
+    size_t feat_part_n = 0;
+    float* feat_raw = nullptr;  // raw tile statistics
+    size_t feat_raw_n = 0;
+    float* feat = nullptr;      // normalised tile features
+    size_t feat_n = 0;
+    float* pred = nullptr;      // MLP outputs per tile
+    size_t pred_n = 0;
+    uint32_t* order = nullptr;  // seeding order of the megapixels
+    size_t order_n = 0;
+    uint16_t* model = nullptr;  // BF16 MLP weights on this device
+    int model_version = -1;
 };
 
 extern "C" void iwpp_default_options(iwpp_options* o) {
@@ -132,6 +146,12 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->tiles);
     (void)hipFree(ws->descs);
     (void)hipFree(ws->img_of_mp);
+    (void)hipFree(ws->feat_part);
+    (void)hipFree(ws->feat_raw);
+    (void)hipFree(ws->feat);
+    (void)hipFree(ws->pred);
+    (void)hipFree(ws->order);
+    (void)hipFree(ws->model);
     if (ws->host_ctl) (void)hipHostFree(ws->host_ctl);
     for (hipEvent_t e : ws->ev)
         if (e) (void)hipEventDestroy(e);
@@ -151,6 +171,12 @@ iwpp_workspace* default_ws() {
     if (!g_default_ws[dev]) g_default_ws[dev] = iwpp_workspace_create(0, 0);
     return g_default_ws[dev];
 }
+
+// The learned schedule's model: BF16 weights, versioned so that workspaces
+// refresh their device copy when it changes.
+std::mutex g_model_mu;
+std::vector<uint16_t> g_model;
+int g_model_version = 0;
 
 int check_image(const void* a, const void* b, size_t pa, size_t pb, int w, int h, int dtype) {
     if (!a || !b) return fail(IWPP_ERR_ARG, "null image pointer");
@@ -184,6 +210,10 @@ struct Run {
     int total_rows = 0, total_cw = 0, total_sw = 0, total_mp = 0;
     QueueArgs q{};
     size_t cap = 0;
+    // learned schedule (single image): per-round tile masks, priority order
+    const uint8_t* round_masks = nullptr;
+    int sched_tile = 0, sched_ntx = 0, sched_nt = 0;
+    const uint32_t* order = nullptr;
 };
 
 int ensure_events(iwpp_workspace* ws) {
@@ -305,7 +335,17 @@ int sweep_phase(Run& r, int rounds) {
             HIP_TRY(launch_clip(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.s));
         return IWPP_OK;
     }
+    if (r.round_masks) {  // blocks may be skipped: start the scratch plane at 0
+        const ImgDesc& d = r.h[0];
+        HIP_TRY(launch_fill_u32((uint32_t*)d.P, 0u, d.pp * d.H / 4, r.s));
+    }
     for (int k = 0; k < rounds; ++k) {
+        if (r.round_masks) {
+            const TileMask tm{r.round_masks + (size_t)k * r.sched_nt, r.sched_tile, r.sched_tile,
+                              r.sched_ntx};
+            r.h[0].tm = tm;
+            r.ds.one.tm = tm;
+        }
         if (r.ds.n == 1) {
             const ImgDesc& d = r.h[0];
             HIP_TRY(launch_row_pass(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, k == 0, d.tm,
@@ -355,6 +395,7 @@ int setup_queue(Run& r) {
     q.max_rounds = 1 << 30;
     q.timeout_ticks = timeout_ticks(r.opt.timeout_s > 0 ? r.opt.timeout_s : 60.0);
     q.trace = r.opt.trace_phases ? 1 : 0;
+    q.order = r.order;
     r.cap = cap;
     return IWPP_OK;
 }
@@ -438,6 +479,91 @@ int check_common(int dtype, int conn, int tile, int queue, int schedule) {
     return IWPP_OK;
 }
 
+// Learned schedule for one image: tile features -> MLP (BF16) -> per tile
+// k = sweep rounds before the queue phase (0..4) and a priority.  Round r of
+// the sweep phase touches the tiles with k > r; the queue is seeded tile by
+// tile in decreasing priority.  Without a model the plain plan is kept.
+int learned_plan(Run& r, SchedulePlan& plan) {
+    iwpp_workspace* ws = r.ws;
+    std::vector<uint16_t> model;
+    int version = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_model_mu);
+        model = g_model;
+        version = g_model_version;
+    }
+    if (model.empty()) return IWPP_OK;  // fallback: the plain heuristic
+    const ImgDesc& d = r.h[0];
+    const int tile = plan.tile;
+    const int ntx = (d.W + tile - 1) / tile, nty = (d.H + tile - 1) / tile, nt = ntx * nty;
+    if (ws->model_version != version) {
+        if (!ws->model) HIP_TRY(hipMalloc((void**)&ws->model, kMlpParams * sizeof(uint16_t)));
+        HIP_TRY(hipMemcpyAsync(ws->model, model.data(), kMlpParams * sizeof(uint16_t),
+                               hipMemcpyHostToDevice, r.s));
+        ws->model_version = version;
+    }
+    HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
+                 (size_t)nt * features_slabs(tile, kFeatureSub) * 16));
+    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)nt * 16));
+    HIP_TRY(grow(ws->feat, ws->feat_n, (size_t)nt * 16));
+    HIP_TRY(grow(ws->pred, ws->pred_n, (size_t)nt * 2));
+    HIP_TRY(launch_tile_features(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.conn, tile,
+                                 kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat, r.s));
+    HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, nt, ws->pred, nullptr, r.s));
+    std::vector<float> pred((size_t)nt * 2);
+    HIP_TRY(hipMemcpyAsync(pred.data(), ws->pred, pred.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, r.s));
+    HIP_TRY(hipStreamSynchronize(r.s));
+    std::vector<int> k(nt);
+    int rounds = 0;
+    for (int t = 0; t < nt; ++t) {
+        const float v = pred[(size_t)t * 2];
+        k[t] = std::isfinite(v) ? (int)std::lround(std::min(4.f, std::max(0.f, v))) : 0;
+        rounds = std::max(rounds, k[t]);
+    }
+    plan.sweep_rounds = rounds;
+    r.st.schedule = IWPP_SCHED_LEARNED;
+    int skipped = 0;
+    for (int t = 0; t < nt; ++t) skipped += rounds - k[t];
+    r.st.sweeps_skipped_tiles = skipped;
+    if (nt == 1) return IWPP_OK;  // one tile: uniform rounds, nothing to order
+    std::vector<uint8_t> masks;   // per-round tile masks, only when k varies
+    if (skipped > 0) {
+        masks.resize((size_t)rounds * nt);
+        for (int rr = 0; rr < rounds; ++rr)
+            for (int t = 0; t < nt; ++t) masks[(size_t)rr * nt + t] = k[t] > rr ? 1 : 0;
+    }
+    std::vector<int> tiles(nt);  // seeding order: tiles by decreasing priority
+    for (int t = 0; t < nt; ++t) tiles[t] = t;
+    std::stable_sort(tiles.begin(), tiles.end(), [&](int a, int b) {
+        return pred[(size_t)a * 2 + 1] > pred[(size_t)b * 2 + 1];
+    });
+    std::vector<uint32_t> order;
+    order.reserve((size_t)d.nmx * d.nmy);
+    const int mpt = tile / kMP;  // megapixels per tile side
+    for (int t : tiles) {
+        const int tx = t % ntx, ty = t / ntx;
+        for (int my = ty * mpt; my < std::min(d.nmy, (ty + 1) * mpt); ++my)
+            for (int mx = tx * mpt; mx < std::min(d.nmx, (tx + 1) * mpt); ++mx)
+                order.push_back((uint32_t)(my * d.nmx + mx));
+    }
+    HIP_TRY(grow(ws->order, ws->order_n, order.size()));
+    HIP_TRY(hipMemcpyAsync(ws->order, order.data(), order.size() * 4, hipMemcpyHostToDevice,
+                           r.s));
+    if (!masks.empty()) {
+        HIP_TRY(grow(ws->tiles, ws->tiles_n, masks.size()));
+        HIP_TRY(hipMemcpyAsync(ws->tiles, masks.data(), masks.size(), hipMemcpyHostToDevice,
+                               r.s));
+        r.round_masks = ws->tiles;
+        r.sched_tile = tile;
+        r.sched_ntx = ntx;
+        r.sched_nt = nt;
+    }
+    HIP_TRY(hipStreamSynchronize(r.s));  // the host vectors go out of scope
+    r.order = ws->order;
+    return IWPP_OK;
+}
+
 // Shared body of the single-image and batch entry points.  With graph = true
 // only the launches are issued (nothing synchronises, no restart).
 int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, int schedule,
@@ -465,10 +591,15 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
     r.queue = plan.queue;
     r.st.queue = plan.queue;
     r.st.tile = plan.tile;
+    r.st.schedule = IWPP_SCHED_PLAIN;
     if (r.graph && r.queue != Q_MUTEX)
         return fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
     rc = setup_images(r, imgs, n);
     if (rc) return rc;
+    if (schedule == IWPP_SCHED_LEARNED && n == 1 && !r.graph) {
+        rc = learned_plan(r, plan);
+        if (rc) return rc;
+    }
     if (r.graph) {
         rc = setup_queue(r);
         if (rc) return rc;
@@ -763,6 +894,74 @@ extern "C" int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int conn,
 }
 
 extern "C" void iwpp_autotune_clear(void) { tune_clear(); }
+
+extern "C" int iwpp_sched_set_model(const float* weights, int n) {
+    g_err.clear();
+    static_assert(kMlpParams == IWPP_MLP_PARAMS && kMlpIn == IWPP_MLP_FEATURES, "MLP shape");
+    if (weights && n != kMlpParams) return fail(IWPP_ERR_ARG, "model needs %d weights", kMlpParams);
+    std::vector<uint16_t> bf;
+    if (weights) {
+        bf.resize(kMlpParams);
+        for (int i = 0; i < kMlpParams; ++i) {
+            if (!std::isfinite(weights[i])) return fail(IWPP_ERR_ARG, "weight %d not finite", i);
+            bf[i] = f32_to_bf16(weights[i]);
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_model_mu);
+    g_model.swap(bf);
+    ++g_model_version;
+    return IWPP_OK;
+}
+
+extern "C" int iwpp_tile_features(const void* marker, const void* mask, size_t pm, size_t pk,
+                                  int w, int h, int dtype, int conn, int tile, int sub,
+                                  float* raw, float* feat, iwpp_workspace* ws, void* stream) {
+    g_err.clear();
+    int rc = check_image(marker, mask, pm, pk, w, h, dtype);
+    if (rc) return rc;
+    if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
+    if (tile < kMP || tile % kMP) return fail(IWPP_ERR_ARG, "tile must be a multiple of 64");
+    if (sub < 1) return fail(IWPP_ERR_ARG, "sub must be >= 1");
+    if (!raw || !feat) return fail(IWPP_ERR_ARG, "null output");
+    if (!ws) ws = default_ws();
+    if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
+    const size_t nt = (size_t)((w + tile - 1) / tile) * ((h + tile - 1) / tile);
+    HIP_TRY(grow(ws->feat_part, ws->feat_part_n, nt * features_slabs(tile, sub) * 16));
+    HIP_TRY(launch_tile_features(Image{(uint8_t*)marker, (const uint8_t*)mask, pm, pk, w, h},
+                                 dtype, conn, tile, sub, ws->feat_part, raw, feat,
+                                 (hipStream_t)stream));
+    return IWPP_OK;
+}
+
+extern "C" int iwpp_mlp_forward(const void* params, int bf16, const float* x, int n, float* y,
+                                float* acts, void* stream) {
+    g_err.clear();
+    if (!params || n < 0 || (n > 0 && (!x || !y))) return fail(IWPP_ERR_ARG, "bad MLP arguments");
+    HIP_TRY(launch_mlp_forward(params, bf16 != 0, x, n, y, acts, (hipStream_t)stream));
+    return IWPP_OK;
+}
+
+extern "C" int iwpp_mlp_backward(const void* params, int bf16, const float* x, const float* acts,
+                                 const float* y, const float* t, int n, float lw0, float lw1,
+                                 float* work, float* grad, void* stream) {
+    g_err.clear();
+    if (!params || !grad || n <= 0 || !x || !acts || !y || !t || !work)
+        return fail(IWPP_ERR_ARG, "bad MLP arguments");
+    float* d3 = work;
+    float* d2 = d3 + (size_t)n * 2;
+    float* d1 = d2 + (size_t)n * 64;
+    HIP_TRY(launch_mlp_backward(params, bf16 != 0, x, acts, y, t, n, lw0, lw1, d3, d2, d1, grad,
+                                (hipStream_t)stream));
+    return IWPP_OK;
+}
+
+extern "C" int iwpp_mlp_adam(float* w, float* m, float* v, const float* grad, uint16_t* wbf,
+                             float lr, int step, void* stream) {
+    g_err.clear();
+    if (!w || !m || !v || !grad || !wbf || step < 1) return fail(IWPP_ERR_ARG, "bad Adam arguments");
+    HIP_TRY(launch_adam(w, m, v, grad, wbf, lr, step, (hipStream_t)stream));
+    return IWPP_OK;
+}
 
 extern "C" int iwpp_clip(void* marker, const void* mask, size_t pm, size_t pk, int w, int h,
                          int dtype, void* stream) {

@@ -96,6 +96,7 @@ struct QueueArgs {
     int max_rounds;     // bsp round limit
     unsigned long long timeout_ticks;  // wall-clock ticks a wait may last
     int trace;          // accumulate the phase timers
+    const uint32_t* order;  // seeding order of the megapixels (nullptr = index order)
 };
 
 enum QueueKind : int { Q_MUTEX = 0, Q_DQ = 1, Q_UQ = 2 };

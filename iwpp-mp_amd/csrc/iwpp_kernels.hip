@@ -202,6 +202,16 @@ DEV bool tile_on(const TileMask& tm, int x, int y) {
     return tm.active == nullptr || tm.active[(size_t)(y / tm.th) * tm.ntx + x / tm.tw] != 0;
 }
 
+// Whether any tile under pixels [x0, x1) of row y is on (x0 < x1).  The same
+// value in every lane when the arguments are.
+DEV bool span_on(const TileMask& tm, int x0, int x1, int y) {
+    if (tm.active == nullptr) return true;
+    const uint8_t* row = tm.active + (size_t)(y / tm.th) * tm.ntx;
+    for (int t = x0 / tm.tw; t <= (x1 - 1) / tm.tw; ++t)
+        if (row[t]) return true;
+    return false;
+}
+
 // ------------------------------------------------------------ descriptors ---
 DEV const ImgDesc& desc(const Descs& a, int i) { return a.d ? a.d[i] : a.one; }
 // Image owning global work item `idx` for a prefix field (binary search).
@@ -273,9 +283,19 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
             const int o = (ch * NT + tid) * VB;
+            dch[ch] = false;
+            // A chunk whose tiles are all off is neither read nor written:
+            // zero marker and mask make it a wall (carry 0, a lower bound).
+            // The clipping round reads every chunk.
+            const int cx0 = ch * NT * VB / (int)sizeof(T);
+            if (!CLIP && tm.active &&
+                (cx0 >= im.W || !span_on(tm, cx0, min(im.W, cx0 + NT * VB / (int)sizeof(T)), y))) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i) jv[ch][i] = iv[ch][i] = 0;
+                continue;
+            }
             V::load(jr, o, rowbytes, jv[ch]);
             V::load(ir, o, rowbytes, iv[ch]);
-            dch[ch] = false;
             if (CLIP) {
                 uint32_t a[NPX], b[NPX];
                 V::unpack(jv[ch], a);
@@ -407,6 +427,15 @@ DEV void row_stream(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2
         for (int n = 0; n < nch; ++n) {
             const int ch = dir > 0 ? n : nch - 1 - n;
             const int o = (ch * kWG + tid) * VB;
+            if (!clip && tm.active) {
+                // all tiles of the chunk off: skip it, restart the carry at 0
+                // (a lower bound of the marker), touch no LDS
+                const int cx0 = ch * kWG * VB / (int)sizeof(T);
+                if (!span_on(tm, cx0, min(im.W, cx0 + kWG * VB / (int)sizeof(T)), y)) {
+                    carry = 0;
+                    continue;
+                }
+            }
             uint32_t jw[NW], iw[NW], v[NPX], c[NPX], ov[NPX];
             V::load(jr, o, rowbytes, jw);
             V::load(ir, o, rowbytes, iw);
@@ -517,6 +546,12 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
     const int xs1 = (xb0 + 64 * VB) / (int)sizeof(T); // first pixel right of the strip
     const int y0 = seg * R, y1 = min(H, y0 + R);
     const bool on = o < rowbytes ? tile_on(tm, o / (int)sizeof(T), y0) : false;
+    // With a tile schedule, a block whose tiles are all off (at its first row)
+    // is skipped in both directions of a round (the same test in both), so its
+    // marker keeps its value; blocks reading its rows from the scratch plane
+    // then see an older value of the same pixels, a valid lower bound (the
+    // plane is zeroed before the first scheduled round).
+    if (tm.active && !wave_any(on)) return;
     uint32_t prev[NPX];
     {
         const int yh = DIR > 0 ? y0 - 1 : y1;
@@ -1036,7 +1071,8 @@ __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nm
     const int lane = lane_id();
     for (int base = (blockIdx.x * kWG + threadIdx.x) & ~63; base < nmp;
          base += gridDim.x * kWG) {
-        const int m = base + lane;
+        const int pos_in = base + lane;
+        const int m = pos_in < nmp ? (q.order ? (int)q.order[pos_in] : pos_in) : nmp;
         const bool f = m < nmp && flags[m] != 0;
         const uint64_t b = ballot(f);
         const uint32_t n = (uint32_t)__popcll(b);

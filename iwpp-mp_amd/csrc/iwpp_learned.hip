@@ -1,0 +1,403 @@
+// Learned schedule: per-tile features, and an MLP 16-64-64-2 (ReLU) that maps
+// them to (sweep rounds before the queue phase, tile priority).  Inference
+// uses BF16 weights with FP32 accumulation; training keeps FP32 master
+// weights, runs the forward pass with the BF16-rounded weights and applies
+// Adam.  The schedule only chooses which tiles the sweep rounds touch and
+// the order in which megapixels enter the queue, so reconstruction results
+// do not depend on it.
+#include "iwpp_internal.h"
+#include "iwpp_learned.h"
+
+namespace iwpp {
+
+#define DEV static __device__ __forceinline__
+
+// ------------------------------------------------------------- features ----
+// Raw per-tile statistics over every `sub`-th row of the tile (from its first):
+//  0 sum I, 1 sum J, 2 sum (I-J), 3 sum (I-J)^2, 4 max(0, max (I-J)),
+//  5 count J==I, 6 count J==0, 7 count seeds (J==I, J>0), 8 max J,
+//  9 sum |I(x+1)-I(x)| over pixel pairs (x even, x+1 in the tile),
+//  10..13 border inflow top/bottom/left/right: sum over every border pixel p
+//  of max(0, min(J(q), I(p)) - J(p)) with q the pixel across the border,
+//  14 pixels sampled, 15 max I.
+// Work split: one workgroup per (tile, slab of kSlab sampled rows); partial
+// statistics go to part[tile][slab][16] and a second kernel combines them.
+// Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
+constexpr int kSlab = 32;
+constexpr int kStats = 16;
+
+DEV bool is_max_stat(int k) { return k == 4 || k == 8 || k == 15; }
+
+DEV float wave_reduce(float v, bool mx) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const float o = __shfl_xor(v, 32 >> i, 64);
+        v = mx ? fmaxf(v, o) : v + o;
+    }
+    return v;
+}
+
+template <class T, int VB>
+__global__ __launch_bounds__(256) void k_tile_features_part(Image im, int tile, int ntx, int sub,
+                                                             int nslab, float* part) {
+    constexpr int NPX = VB / (int)sizeof(T), PPW = 4 / (int)sizeof(T);
+    constexpr int BITS = 8 * (int)sizeof(T);
+    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
+    const int t = blockIdx.x / nslab, sl = blockIdx.x % nslab;
+    const int tx = t % ntx, ty = t / ntx;
+    const int x0 = tx * tile, y0 = ty * tile;
+    const int x1 = min(im.W, x0 + tile), y1 = min(im.H, y0 + tile);
+    const int bx0 = x0 * (int)sizeof(T), bx1 = x1 * (int)sizeof(T);
+    const int rowbytes = im.W * (int)sizeof(T);
+    const int nvec = (bx1 - bx0 + VB - 1) / VB;
+    float acc[kStats];
+#pragma unroll
+    for (int k = 0; k < kStats; ++k) acc[k] = 0.f;
+    const int items = kSlab * nvec;
+    for (int i = threadIdx.x; i < items; i += blockDim.x) {
+        const int r = i / nvec, v = i % nvec;
+        const int y = y0 + (sl * kSlab + r) * sub;
+        if (y >= y1) continue;
+        const int o = bx0 + v * VB;  // VB-aligned (tiles start at multiples of 64 px)
+        uint32_t jw[VB / 4], iw[VB / 4];
+        const uint8_t* jr = im.J + (size_t)y * im.pj + o;
+        const uint8_t* ir = im.I + (size_t)y * im.pi + o;
+        if (o + VB > rowbytes) {  // row end: pixel by pixel, nothing past the row
+#pragma unroll
+            for (int w = 0; w < VB / 4; ++w) jw[w] = iw[w] = 0;
+#pragma unroll
+            for (int k = 0; k < NPX; ++k) {
+                if (o + (k + 1) * (int)sizeof(T) <= rowbytes) {
+                    jw[k / PPW] |= (uint32_t)*reinterpret_cast<const T*>(jr + k * sizeof(T)) << (BITS * (k % PPW));
+                    iw[k / PPW] |= (uint32_t)*reinterpret_cast<const T*>(ir + k * sizeof(T)) << (BITS * (k % PPW));
+                }
+            }
+        } else if constexpr (VB == 16) {
+            const uint4 a = *reinterpret_cast<const uint4*>(jr);
+            const uint4 b = *reinterpret_cast<const uint4*>(ir);
+            jw[0] = a.x; jw[1] = a.y; jw[2] = a.z; jw[3] = a.w;
+            iw[0] = b.x; iw[1] = b.y; iw[2] = b.z; iw[3] = b.w;
+        } else {
+            jw[0] = *reinterpret_cast<const uint32_t*>(jr);
+            iw[0] = *reinterpret_cast<const uint32_t*>(ir);
+        }
+        const int xv = o / (int)sizeof(T);  // first pixel of the vector
+        float prev_i = 0.f;
+#pragma unroll
+        for (int k = 0; k < NPX; ++k) {
+            if (xv + k >= x1) continue;  // past the tile (uniform per vector)
+            const float j = (float)((jw[k / PPW] >> (BITS * (k % PPW))) & PM);
+            const float c = (float)((iw[k / PPW] >> (BITS * (k % PPW))) & PM);
+            const float g = c - j;
+            acc[0] += c;
+            acc[1] += j;
+            acc[2] += g;
+            acc[3] += g * g;
+            acc[4] = fmaxf(acc[4], g);
+            acc[5] += (j == c);
+            acc[6] += (j == 0.f);
+            acc[7] += (j == c && j > 0.f);
+            acc[8] = fmaxf(acc[8], j);
+            if (k & 1) acc[9] += fabsf(c - prev_i);  // pair (x-1, x), x-1 even
+            prev_i = c;
+            acc[14] += 1.f;
+            acc[15] = fmaxf(acc[15], c);
+        }
+    }
+    // border inflow: rows y0 / y1-1 in the first / last slab, the left and
+    // right columns over this slab's band of image rows
+    auto J = [&](int x, int y) -> float {
+        return (float)*reinterpret_cast<const T*>(im.J + (size_t)y * im.pj + (size_t)x * sizeof(T));
+    };
+    auto I = [&](int x, int y) -> float {
+        return (float)*reinterpret_cast<const T*>(im.I + (size_t)y * im.pi + (size_t)x * sizeof(T));
+    };
+    if (sl == 0) {
+        for (int x = x0 + (int)threadIdx.x; x < x1; x += blockDim.x) {
+            if (y0 > 0) acc[10] += fmaxf(0.f, fminf(J(x, y0 - 1), I(x, y0)) - J(x, y0));
+            if (y1 < im.H) acc[11] += fmaxf(0.f, fminf(J(x, y1), I(x, y1 - 1)) - J(x, y1 - 1));
+        }
+    }
+    const int band0 = y0 + sl * kSlab * sub;
+    const int band1 = sl == nslab - 1 ? y1 : min(y1, band0 + kSlab * sub);
+    for (int y = band0 + (int)threadIdx.x; y < band1; y += blockDim.x) {
+        if (x0 > 0) acc[12] += fmaxf(0.f, fminf(J(x0 - 1, y), I(x0, y)) - J(x0, y));
+        if (x1 < im.W) acc[13] += fmaxf(0.f, fminf(J(x1, y), I(x1 - 1, y)) - J(x1 - 1, y));
+    }
+    __shared__ float red[kStats][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kStats; ++k) {
+        const float v = wave_reduce(acc[k], is_max_stat(k));
+        if (lane == 0) red[k][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kStats) {
+        const int k = threadIdx.x;
+        const bool mx = is_max_stat(k);
+        float v = red[k][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = mx ? fmaxf(v, red[k][w]) : v + red[k][w];
+        part[((size_t)t * nslab + sl) * kStats + k] = v;
+    }
+}
+
+// raw[t][k] from the slab partials (one thread per tile and statistic).
+__global__ __launch_bounds__(256) void k_tile_features_combine(const float* part, int ntiles,
+                                                                int nslab, float* raw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntiles * kStats) return;
+    const int t = i / kStats, k = i % kStats;
+    const bool mx = is_max_stat(k);
+    float v = part[((size_t)t * nslab) * kStats + k];
+    for (int s = 1; s < nslab; ++s) {
+        const float o = part[((size_t)t * nslab + s) * kStats + k];
+        v = mx ? fmaxf(v, o) : v + o;
+    }
+    raw[i] = v;
+}
+
+// Normalised features from raw statistics.  M = max mask value over all tiles
+// (0 -> 1); tile_px = sampled pixels of a full tile.  Features 1 and 14 carry
+// the connectivity (8 -> 1, 4 -> 0) and the pixel type (uint16 -> 1, uint8 -> 0).
+DEV void normalise(const float* r, float M, float tile_px, int tile, int dtype, int conn,
+                   float* f) {
+    const float n = fmaxf(r[14], 1.f);
+    const float inv = 1.f / fmaxf(M, 1.f);
+    f[0] = r[0] / n * inv;
+    f[1] = conn == 8 ? 1.f : 0.f;
+    f[2] = r[2] / n * inv;
+    f[3] = sqrtf(r[3] / n) * inv;
+    f[4] = r[4] * inv;
+    f[5] = r[5] / n;
+    f[6] = r[6] / n;
+    f[7] = r[7] / n * 100.f;
+    f[8] = r[8] * inv;
+    f[9] = r[9] / n * inv * 10.f;
+    const float side = (float)tile;
+    f[10] = r[10] / side * inv;
+    f[11] = r[11] / side * inv;
+    f[12] = r[12] / side * inv;
+    f[13] = r[13] / side * inv;
+    f[14] = dtype == 2 ? 1.f : 0.f;
+    f[15] = r[14] / fmaxf(tile_px, 1.f);
+}
+
+__global__ __launch_bounds__(256) void k_features_normalise(const float* raw, int ntiles,
+                                                             int tile, int sub, int dtype,
+                                                             int conn, float* feat) {
+    __shared__ float smax[256];
+    float m = 0.f;
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x) m = fmaxf(m, raw[(size_t)t * 16 + 15]);
+    smax[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const float M = smax[0];
+    const float tile_px = (float)tile * (float)((tile + sub - 1) / sub);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
+        normalise(raw + (size_t)t * 16, M, tile_px, tile, dtype, conn, feat + (size_t)t * 16);
+}
+
+// ------------------------------------------------------------------ MLP ----
+// Parameter layout (kMlpParams floats): W1[64][16] b1[64] W2[64][64] b2[64]
+// W3[2][64] b3[2].
+constexpr int O_W1 = 0, O_B1 = O_W1 + 64 * 16, O_W2 = O_B1 + 64, O_B2 = O_W2 + 64 * 64,
+              O_W3 = O_B2 + 64, O_B3 = O_W3 + 2 * 64;
+static_assert(O_B3 + 2 == kMlpParams, "MLP layout");
+
+// One thread per sample; weights in LDS as float (from BF16 or FP32).
+// acts (optional, training): per sample h1[64] h2[64] pre-activations' sign
+// is recovered from h (ReLU).
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_mlp_forward(const void* params, const float* x, int n,
+                                                      float* y, float* acts) {
+    __shared__ float w[kMlpParams];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x)
+        w[i] = BF16 ? bf16_to_f32(reinterpret_cast<const uint16_t*>(params)[i])
+                    : reinterpret_cast<const float*>(params)[i];
+    __syncthreads();
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        float in[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) in[k] = x[(size_t)s * 16 + k];
+        float h1[64];
+#pragma unroll 8
+        for (int o = 0; o < 64; ++o) {
+            float a = w[O_B1 + o];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a = fmaf(w[O_W1 + o * 16 + k], in[k], a);
+            h1[o] = fmaxf(a, 0.f);
+        }
+        float y0 = w[O_B3], y1 = w[O_B3 + 1];
+        for (int o = 0; o < 64; ++o) {
+            float a = w[O_B2 + o];
+#pragma unroll
+            for (int k = 0; k < 64; ++k) a = fmaf(w[O_W2 + o * 64 + k], h1[k], a);
+            const float h2 = fmaxf(a, 0.f);
+            y0 = fmaf(w[O_W3 + o], h2, y0);
+            y1 = fmaf(w[O_W3 + 64 + o], h2, y1);
+            if (acts) acts[(size_t)s * 128 + 64 + o] = h2;
+        }
+        if (acts) {
+#pragma unroll 8
+            for (int k = 0; k < 64; ++k) acts[(size_t)s * 128 + k] = h1[k];
+        }
+        y[(size_t)s * 2] = y0;
+        y[(size_t)s * 2 + 1] = y1;
+    }
+}
+
+// Backward per sample for the weighted squared loss
+//   L = sum_s sum_o lw[o] * (y[s][o] - t[s][o])^2 / (2 n)
+// Writes d3[s][2], d2[s][64] (after ReLU'), d1[s][64].
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_mlp_backward(const void* params, const float* acts,
+                                                       const float* y, const float* t, int n,
+                                                       float lw0, float lw1, float* d3, float* d2,
+                                                       float* d1) {
+    __shared__ float w[kMlpParams];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x)
+        w[i] = BF16 ? bf16_to_f32(reinterpret_cast<const uint16_t*>(params)[i])
+                    : reinterpret_cast<const float*>(params)[i];
+    __syncthreads();
+    const float invn = 1.f / (float)n;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const float e0 = (y[(size_t)s * 2] - t[(size_t)s * 2]) * lw0 * invn;
+        const float e1 = (y[(size_t)s * 2 + 1] - t[(size_t)s * 2 + 1]) * lw1 * invn;
+        d3[(size_t)s * 2] = e0;
+        d3[(size_t)s * 2 + 1] = e1;
+        float g2[64];
+#pragma unroll 8
+        for (int k = 0; k < 64; ++k) {
+            const float h2 = acts[(size_t)s * 128 + 64 + k];
+            const float g = w[O_W3 + k] * e0 + w[O_W3 + 64 + k] * e1;
+            g2[k] = h2 > 0.f ? g : 0.f;
+            d2[(size_t)s * 64 + k] = g2[k];
+        }
+        for (int j = 0; j < 64; ++j) {
+            float g = 0.f;
+#pragma unroll
+            for (int k = 0; k < 64; ++k) g = fmaf(w[O_W2 + k * 64 + j], g2[k], g);
+            const float h1 = acts[(size_t)s * 128 + j];
+            d1[(size_t)s * 64 + j] = h1 > 0.f ? g : 0.f;
+        }
+    }
+}
+
+// Parameter gradients: one thread per parameter, summing over samples.
+__global__ __launch_bounds__(256) void k_mlp_grads(const float* x, const float* acts,
+                                                    const float* d3, const float* d2,
+                                                    const float* d1, int n, float* grad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float g = 0.f;
+    if (p < O_B1) {
+        const int o = p / 16, k = p % 16;
+        for (int s = 0; s < n; ++s) g = fmaf(d1[(size_t)s * 64 + o], x[(size_t)s * 16 + k], g);
+    } else if (p < O_W2) {
+        const int o = p - O_B1;
+        for (int s = 0; s < n; ++s) g += d1[(size_t)s * 64 + o];
+    } else if (p < O_B2) {
+        const int o = (p - O_W2) / 64, k = (p - O_W2) % 64;
+        for (int s = 0; s < n; ++s) g = fmaf(d2[(size_t)s * 64 + o], acts[(size_t)s * 128 + k], g);
+    } else if (p < O_W3) {
+        const int o = p - O_B2;
+        for (int s = 0; s < n; ++s) g += d2[(size_t)s * 64 + o];
+    } else if (p < O_B3) {
+        const int o = (p - O_W3) / 64, k = (p - O_W3) % 64;
+        for (int s = 0; s < n; ++s)
+            g = fmaf(d3[(size_t)s * 2 + o], acts[(size_t)s * 128 + 64 + k], g);
+    } else {
+        const int o = p - O_B3;
+        for (int s = 0; s < n; ++s) g += d3[(size_t)s * 2 + o];
+    }
+    grad[p] = g;
+}
+
+// Adam on the FP32 master weights; refreshes the BF16 copy.
+__global__ __launch_bounds__(256) void k_adam(float* w, float* m, float* v, const float* grad,
+                                               uint16_t* wbf, float lr, float b1, float b2,
+                                               float eps, float c1, float c2) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float g = grad[p];
+    const float mp = b1 * m[p] + (1.f - b1) * g;
+    const float vp = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mp;
+    v[p] = vp;
+    const float step = lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+    w[p] -= step;
+    wbf[p] = f32_to_bf16(w[p]);
+}
+
+__global__ void k_to_bf16(const float* w, uint16_t* wbf, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) wbf[p] = f32_to_bf16(w[p]);
+}
+
+// ------------------------------------------------------------ launchers ----
+int features_slabs(int tile, int sub) { return ((tile + sub - 1) / sub + kSlab - 1) / kSlab; }
+
+hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, int sub,
+                                float* part, float* raw, float* feat, hipStream_t s) {
+    const int ntx = (im.W + tile - 1) / tile, nty = (im.H + tile - 1) / tile;
+    const int nt = ntx * nty, nslab = features_slabs(tile, sub);
+    const bool v16 = (uintptr_t)im.J % 16 == 0 && (uintptr_t)im.I % 16 == 0 && im.pj % 16 == 0 &&
+                     im.pi % 16 == 0;
+    const int grid = nt * nslab;
+    if (dtype == 1) {
+        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint8_t, 16>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
+        else hipLaunchKernelGGL((k_tile_features_part<uint8_t, 4>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
+    } else {
+        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint16_t, 16>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
+        else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
+    }
+    hipLaunchKernelGGL(k_tile_features_combine, (nt * kStats + 255) / 256, 256, 0, s, part, nt,
+                       nslab, raw);
+    hipLaunchKernelGGL(k_features_normalise, std::max(1, (nt + 255) / 256), 256, 0, s, raw, nt,
+                       tile, sub, dtype, conn, feat);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const void* params, bool bf16, const float* x, int n, float* y,
+                              float* acts, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int grid = std::min(1024, (n + 255) / 256);
+    if (bf16) hipLaunchKernelGGL(k_mlp_forward<true>, grid, 256, 0, s, params, x, n, y, acts);
+    else hipLaunchKernelGGL(k_mlp_forward<false>, grid, 256, 0, s, params, x, n, y, acts);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward(const void* params, bool bf16, const float* x, const float* acts,
+                               const float* y, const float* t, int n, float lw0, float lw1,
+                               float* d3, float* d2, float* d1, float* grad, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int grid = std::min(1024, (n + 255) / 256);
+    if (bf16)
+        hipLaunchKernelGGL(k_mlp_backward<true>, grid, 256, 0, s, params, acts, y, t, n, lw0, lw1,
+                           d3, d2, d1);
+    else
+        hipLaunchKernelGGL(k_mlp_backward<false>, grid, 256, 0, s, params, acts, y, t, n, lw0,
+                           lw1, d3, d2, d1);
+    hipLaunchKernelGGL(k_mlp_grads, (kMlpParams + 255) / 256, 256, 0, s, x, acts, d3, d2, d1, n,
+                       grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* w, float* m, float* v, const float* grad, uint16_t* wbf, float lr,
+                       int step, hipStream_t s) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, (kMlpParams + 255) / 256, 256, 0, s, w, m, v, grad, wbf, lr, b1,
+                       b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float* w, uint16_t* wbf, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_bf16, (n + 255) / 256, 256, 0, s, w, wbf, n);
+    return hipGetLastError();
+}
+
+}  // namespace iwpp

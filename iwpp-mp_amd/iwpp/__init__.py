@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("tile_visits", ctypes.c_longlong), ("seeded", ctypes.c_longlong),
                 ("sweep_rounds", ctypes.c_int), ("queue_rounds", ctypes.c_int),
                 ("restarts", ctypes.c_int), ("queue", ctypes.c_int), ("tile", ctypes.c_int),
-                ("sweeps_skipped_tiles", ctypes.c_int), ("ms_sweep", ctypes.c_float),
+                ("sweeps_skipped_tiles", ctypes.c_int), ("schedule", ctypes.c_int),
+                ("ms_sweep", ctypes.c_float),
                 ("ms_seed", ctypes.c_float), ("ms_queue", ctypes.c_float),
                 ("us_wait", ctypes.c_double), ("us_fetch", ctypes.c_double),
                 ("us_converge", ctypes.c_double), ("us_store", ctypes.c_double),
@@ -80,6 +81,12 @@ SIGNATURES = {
     "iwpp_batch_graph_destroy": (None, [_VP]),
     "iwpp_autotune": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(Options), _VP, _VP, _VP]),
     "iwpp_autotune_clear": (None, []),
+    "iwpp_sched_set_model": (_I, [_VP, _I]),
+    "iwpp_tile_features": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP]),
+    "iwpp_mlp_forward": (_I, [_VP, _I, _VP, _I, _VP, _VP, _VP]),
+    "iwpp_mlp_backward": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, ctypes.c_float, ctypes.c_float,
+                               _VP, _VP, _VP]),
+    "iwpp_mlp_adam": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.c_float, _I, _VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
     "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
 }
@@ -114,6 +121,15 @@ def lib():
     """The product library (loads it on first use; raises if it is not built)."""
     global _LIB
     if _LIB is None:
+        # PyTorch bundles its own HIP runtime with the same soname
+        # (libamdhip64.so.7) as the system one libiwpp.so links against; the
+        # first one loaded serves the whole process.  Load torch's first so
+        # that torch keeps the runtime it was built with (the other order
+        # leaves torch without devices) and both share streams and memory.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _LIB = _bind(LIB_PATH, SIGNATURES)
     return _LIB
 
@@ -322,6 +338,67 @@ class BatchGraph:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------ learned schedule ----
+MLP_PARAMS = 5378
+MLP_FEATURES = 16
+
+
+def set_model(weights):
+    """Install the learned schedule's model (FP32 numpy array of MLP_PARAMS
+    values, rounded to BF16 by the library), or remove it with None."""
+    if weights is None:
+        _raise(lib().iwpp_sched_set_model(None, 0))
+        return
+    w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+    _raise(lib().iwpp_sched_set_model(w.ctypes.data, w.size))
+
+
+def tile_features(marker, mask, tile, connectivity=8, sub=4, workspace=None, stream=None):
+    """(raw, feat): torch float32 device tensors of shape (tiles, 16)."""
+    torch = _torch()
+    ntx = (marker.w + tile - 1) // tile
+    nty = (marker.h + tile - 1) // tile
+    dev = marker.buf.device
+    raw = torch.empty((nty * ntx, MLP_FEATURES), dtype=torch.float32, device=dev)
+    feat = torch.empty_like(raw)
+    _raise(lib().iwpp_tile_features(marker.ptr, mask.ptr, marker.pitch, mask.pitch, marker.w,
+                                    marker.h, marker.dtype, connectivity, tile, sub,
+                                    raw.data_ptr(), feat.data_ptr(),
+                                    workspace.handle if workspace else None, _stream(stream)))
+    return raw, feat
+
+
+def mlp_forward(params, x, bf16, acts=False, stream=None):
+    """y (n, 2) [and the hidden activations (n, 128)] of the MLP on device
+    tensors: params float32 (FP32 weights) or int16/uint16 BF16 bit patterns."""
+    torch = _torch()
+    n = x.shape[0]
+    y = torch.empty((n, 2), dtype=torch.float32, device=x.device)
+    a = torch.empty((n, 128), dtype=torch.float32, device=x.device) if acts else None
+    _raise(lib().iwpp_mlp_forward(params.data_ptr(), 1 if bf16 else 0, x.data_ptr(), n,
+                                  y.data_ptr(), a.data_ptr() if acts else None,
+                                  _stream(stream)))
+    return (y, a) if acts else y
+
+
+def mlp_backward(params, x, acts, y, target, bf16, lw=(1.0, 1.0), stream=None):
+    """Gradient (MLP_PARAMS float32 device tensor) of the weighted squared loss."""
+    torch = _torch()
+    n = x.shape[0]
+    work = torch.empty((n * 130,), dtype=torch.float32, device=x.device)
+    grad = torch.empty((MLP_PARAMS,), dtype=torch.float32, device=x.device)
+    _raise(lib().iwpp_mlp_backward(params.data_ptr(), 1 if bf16 else 0, x.data_ptr(),
+                                   acts.data_ptr(), y.data_ptr(), target.data_ptr(), n,
+                                   float(lw[0]), float(lw[1]), work.data_ptr(), grad.data_ptr(),
+                                   _stream(stream)))
+    return grad
+
+
+def mlp_adam(w, m, v, grad, wbf, lr, step, stream=None):
+    _raise(lib().iwpp_mlp_adam(w.data_ptr(), m.data_ptr(), v.data_ptr(), grad.data_ptr(),
+                               wbf.data_ptr(), float(lr), int(step), _stream(stream)))
 
 
 def clip(marker, mask, stream=None):

@@ -1,0 +1,327 @@
+"""Learned schedule: traces generated by the exact method, labels, training on
+the GPU with the library's own kernels, and the model file.
+
+The model (MLP 16-64-64-2, ReLU) maps the 16 features of a tile to
+  y0 = sweep rounds the tile takes before the queue phase (rounded, 0..4),
+  y1 = the tile's seeding priority (higher enters the queue first).
+The library applies it (iwpp_reconstruct_ex with IWPP_SCHED_LEARNED); the
+schedule never changes the result, only the work done to reach it.
+
+Traces.  For one training image, with the schedule's tile size:
+  * the features of every tile on the input marker (what the library sees),
+  * d_t(k): megapixels of tile t whose marker still differs from the
+    reconstruction after k full sweep rounds, k = 0..K, using the library's
+    own row and column passes (the same kernels and column blocking as a
+    reconstruction); the queue phase visits each of them at least once,
+  * the tile's mean reconstructed value.
+Labels.  For every training image the whole reconstruction is timed with
+k = 0..4 uniform sweep rounds; every tile of the image gets the fastest k,
+and priority_t = mean reconstructed value of t / max mask value, so tiles
+whose flood carries the highest values are seeded first (their propagation
+overwrites lower floods, which would otherwise be propagated and then
+overtaken).  The reference has no counterpart; its schedule is fixed
+(IWPP MP/src/MorphologicalReconstruction.cu:785-806 seeds every megapixel).
+
+Model file: iwpp-mp_amd/model/sched_mlp.f32 (5378 little-endian float32,
+layout W1[64][16] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]) and
+sched_mlp.json (training metadata).  C callers read the .f32 file and pass
+it to iwpp_sched_set_model.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import (MLP_FEATURES, MLP_PARAMS, MP_SIZE, DeviceImage, Workspace, clip, default_options,
+               mlp_adam, mlp_backward, mlp_forward, reconstruct, set_model, sweep_cols,
+               sweep_rows, tile_features)
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL_DIR = os.path.join(_PKG, "model")
+MODEL_PATH = os.path.join(MODEL_DIR, "sched_mlp.f32")
+META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
+
+MAX_ROUNDS = 4
+FEATURE_SUB = 4  # row subsampling of the features (the library uses the same)
+
+
+def schedule_tile(width, height):
+    """Tile size of the schedule for an image (the library's plan: 2048 when
+    the short side is at least 8192, else 1024)."""
+    return 2048 if min(width, height) >= 8192 else 1024
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# ---------------------------------------------------------------- model -----
+def init_params(seed=0):
+    """He-normal weights, zero biases (float32, MLP_PARAMS values)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_PARAMS, dtype=np.float32)
+    o = 0
+    for fan_out, fan_in in ((64, 16), (64, 64), (2, 64)):
+        p[o:o + fan_out * fan_in] = rng.normal(0, np.sqrt(2.0 / fan_in), fan_out * fan_in)
+        o += fan_out * fan_in + fan_out
+    return p
+
+
+def bf16_bits(w):
+    """float32 -> bfloat16 bit patterns, round to nearest even (NaN kept quiet);
+    the same rule as the library's iwpp_sched_set_model."""
+    u = np.ascontiguousarray(w, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = ((u & 0x7F800000) == 0x7F800000) & ((u & 0x007FFFFF) != 0)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    return np.where(nan, ((u >> 16) | 0x40) & 0xFFFF, r).astype(np.uint16)
+
+
+def load_model(path=None):
+    path = path or MODEL_PATH
+    w = np.fromfile(path, dtype="<f4")
+    if w.size != MLP_PARAMS:
+        raise ValueError(f"{path}: {w.size} values, expected {MLP_PARAMS}")
+    return w.astype(np.float32)
+
+
+def install(path=None):
+    """Load the model file and install it in the library; returns the weights."""
+    w = load_model(path)
+    set_model(w)
+    return w
+
+
+def save_model(w, meta, path=None, meta_path=None):
+    path = path or MODEL_PATH
+    meta_path = meta_path or (os.path.splitext(path)[0] + ".json")
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.asarray(w, dtype="<f4").tofile(path)
+    with open(meta_path, "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+
+
+# --------------------------------------------------------------- traces -----
+def _tile_sums(flags, tile):
+    """Per-tile sums of a (nmy, nmx) megapixel array, tiles in row-major order."""
+    m = tile // MP_SIZE
+    nmy, nmx = flags.shape
+    nty, ntx = -(-nmy // m), -(-nmx // m)
+    pad = np.zeros((nty * m, ntx * m), dtype=np.int64)
+    pad[:nmy, :nmx] = flags
+    return pad.reshape(nty, m, ntx, m).sum(axis=(1, 3)).ravel()
+
+
+def _tile_pixels(w, h, tile):
+    xs = np.minimum(tile, w - np.arange(0, w, tile))
+    ys = np.minimum(tile, h - np.arange(0, h, tile))
+    return np.outer(ys, xs).ravel()
+
+
+def _dirty_megapixels(J, F):
+    """(nmy, nmx) int64: megapixels where device images J and F differ."""
+    torch = _torch()
+    rb = J.w * J.dtype
+    nmy, nmx = -(-J.h // MP_SIZE), -(-J.w // MP_SIZE)
+    d = torch.zeros((nmy * MP_SIZE, nmx * MP_SIZE * J.dtype), dtype=torch.bool,
+                    device=J.buf.device)
+    d[:J.h, :rb] = J.buf[:, :rb] != F.buf[:, :rb]
+    d = d.view(nmy, MP_SIZE, nmx, MP_SIZE * J.dtype).any(dim=3).any(dim=1)
+    return d.cpu().numpy().astype(np.int64)
+
+
+def trace(mask, marker, connectivity, rounds=MAX_ROUNDS, tile=None):
+    """Trace of one image (numpy mask/marker): dict with
+    features (T, 16) of the input marker; dirty (T, rounds+1): megapixels of
+    tile t whose marker still differs from the reconstruction after k sweep
+    rounds (each one the queue phase has to visit at least once); pixels (T,);
+    final_mean (T,); maxval; tile."""
+    torch = _torch()
+    h, w = mask.shape
+    tile = tile or schedule_tile(w, h)
+    I = DeviceImage.from_numpy(mask)
+    J = DeviceImage.from_numpy(marker)
+    F = DeviceImage.from_numpy(marker)
+    reconstruct(F, I, connectivity=connectivity, queue="mutex")
+    _, feat = tile_features(J, I, tile, connectivity, FEATURE_SUB)
+    feats = feat.cpu().numpy().astype(np.float32)
+    P = DeviceImage(h, w, I.dtype)
+    clip(J, I)
+    dirty = []
+    for k in range(rounds + 1):
+        dirty.append(_tile_sums(_dirty_megapixels(J, F), tile))
+        if k < rounds:
+            sweep_rows(J, I)
+            sweep_cols(J, P, I, connectivity, +1, 0)
+            sweep_cols(P, J, I, connectivity, -1, 0)
+    torch.cuda.synchronize()
+    final = F.numpy().astype(np.float64)
+    nty, ntx = -(-h // tile), -(-w // tile)
+    pad = np.zeros((nty * tile, ntx * tile))
+    pad[:h, :w] = final
+    sums = pad.reshape(nty, tile, ntx, tile).sum(axis=(1, 3)).ravel()
+    px = _tile_pixels(w, h, tile)
+    return {"features": feats, "dirty": np.stack(dirty, 1), "pixels": px,
+            "final_mean": sums / px, "maxval": float(mask.max()), "tile": tile}
+
+
+def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=3, workspace=None):
+    """Wall-clock ms of a whole reconstruction (launches, host work and
+    synchronisation included) with k uniform sweep rounds, k = 0..rounds,
+    best of `trials`, on the GPU."""
+    torch = _torch()
+    I = DeviceImage.from_numpy(mask)
+    J0 = DeviceImage.from_numpy(marker)
+    J = DeviceImage(mask.shape[0], mask.shape[1], I.dtype)
+    out = np.full(rounds + 1, np.inf)
+    for k in range(rounds + 1):
+        opt = default_options(sweep_rounds=k)
+        for _ in range(trials):
+            J.buf.copy_(J0.buf)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reconstruct(J, I, connectivity=conn, queue="mutex", options=opt, workspace=workspace)
+            torch.cuda.synchronize()
+            out[k] = min(out[k], (time.perf_counter() - t0) * 1e3)
+    return out
+
+
+def labels(tr, timings):
+    """(k*, priority) per tile: k* = the number of uniform sweep rounds with
+    the shortest measured reconstruction of the tile's image; priority = the
+    tile's mean reconstructed value over the image's largest mask value."""
+    k = float(np.argmin(timings))
+    prio = (tr["final_mean"] / max(tr["maxval"], 1.0)).astype(np.float32)
+    return np.stack([np.full(prio.shape, k, np.float32), prio], 1)
+
+
+# ------------------------------------------------------------- training -----
+def train(X, Y, steps=3000, lr=2e-3, lw=(1.0, 4.0), seed=0, log_every=100, device="cuda"):
+    """Full-batch Adam on the GPU with the library's kernels: BF16 forward,
+    backward through the BF16 weights into FP32 master weights.
+    Returns (weights float32, history [(step, loss)])."""
+    torch = _torch()
+    n = X.shape[0]
+    x = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).to(device)
+    t = torch.from_numpy(np.ascontiguousarray(Y, dtype=np.float32)).to(device)
+    w0 = init_params(seed)
+    w = torch.from_numpy(w0).to(device)
+    m = torch.zeros_like(w)
+    v = torch.zeros_like(w)
+    wbf = torch.from_numpy(bf16_bits(w0).view(np.int16)).to(device)
+    lwt = torch.tensor(lw, dtype=torch.float32, device=device)
+    hist = []
+    for step in range(1, steps + 1):
+        y, acts = mlp_forward(wbf, x, bf16=True, acts=True)
+        if step == 1 or step % log_every == 0 or step == steps:
+            L = float(((y - t) ** 2 * lwt).sum() / (2 * n))
+            hist.append((step - 1, L))
+        g = mlp_backward(wbf, x, acts, y, t, bf16=True, lw=lw)
+        mlp_adam(w, m, v, g, wbf, lr, step)
+    torch.cuda.synchronize()
+    return w.cpu().numpy().astype(np.float32), hist
+
+
+def training_cases(counts=None, seed_base=None):
+    """Training images: the generators of the first four workloads (h-dome
+    1024^2; 0.1% seeds 4096^2; uint16 h-dome with disks 16384^2, 4-connected;
+    the batch of mixed sizes) with seeds those workloads do not use, plus the
+    same seeds / h-dome generators at 8192^2 with other seed densities, blur
+    widths and disk counts.  Returns (name, mask, marker, connectivity) tuples."""
+    from . import U8, U16, synth_mask, synth_marker_hdome, synth_marker_seeds
+    from . import workloads as wl
+    base = seed_base if seed_base is not None else wl.S0 + 1000
+    n = {"c1": 4, "c2": 6, "c3": 2, "c4": 96, "seeds8k": 6, "hdome8k": 3}
+    n.update(counts or {})
+    out = []
+    for i in range(n["c1"]):
+        c = wl.c1(seed=base + 10 + i)
+        out.append((c.name, c.mask, c.marker, c.connectivity))
+    for i in range(n["c2"]):
+        c = wl.c2(seed=base + 20 + i)
+        out.append((c.name, c.mask, c.marker, c.connectivity))
+    for i in range(n["c3"]):
+        c = wl.c3(seed=base + 30 + i)
+        out.append((c.name, c.mask, c.marker, c.connectivity))
+    if n["c4"]:
+        for c in wl.c4(n=n["c4"], seed=base + 40):
+            out.append((c.name, c.mask, c.marker, c.connectivity))
+    # (ppm, sigma, disks) of the 8192^2 seeds images, cycled
+    seeds_params = [(1000, 8.0, 0), (300, 16.0, 100), (100, 32.0, 250), (30, 32.0, 250),
+                    (100, 16.0, 0), (300, 32.0, 250)]
+    for i in range(n["seeds8k"]):
+        ppm, sigma, disks = seeds_params[i % len(seeds_params)]
+        sd = base + 60 + i
+        mask = synth_mask(8192, 8192, U8, 255, sigma, disks, 50, 400, sd)
+        marker = synth_marker_seeds(mask, wl.seeds_count(mask.size, ppm), sd)
+        out.append((f"seeds_8192_u8_ppm{ppm}_s{int(sigma)}", mask, marker, 8))
+    hdome_params = [(50, 16.0, 50, U16, 4095, 4), (30, 8.0, 0, U8, 255, 8), (20, 32.0, 100, U8, 255, 8)]
+    for i in range(n["hdome8k"]):
+        h, sigma, disks, dt, top, conn = hdome_params[i % len(hdome_params)]
+        sd = base + 80 + i
+        mask = synth_mask(8192, 8192, dt, top, sigma, disks, 50, 400, sd)
+        out.append((f"hdome_8192_{'u16' if dt == U16 else 'u8'}_h{h}", mask,
+                    synth_marker_hdome(mask, h), conn))
+    return out
+
+
+def build_dataset(cases, rounds=MAX_ROUNDS, log=None):
+    """Features and labels of every tile of `cases` (traces and timings on
+    the GPU).  Returns X, Y and per-image information."""
+    from . import Workspace as _W
+    ws = _W()
+    X, Y, info = [], [], []
+    for name, mask, marker, conn in cases:
+        t0 = time.time()
+        tr = trace(mask, marker, conn, rounds)
+        tm = image_timings(mask, marker, conn, rounds, workspace=ws)
+        y = labels(tr, tm)
+        X.append(tr["features"])
+        Y.append(y)
+        info.append({"image": name, "tiles": int(y.shape[0]), "ms_by_rounds": tm.round(4).tolist(),
+                     "k": int(y[0, 0])})
+        if log:
+            log(f"trace {name}: {tr['features'].shape[0]} tiles, dirty "
+                f"{tr['dirty'].sum(0).tolist()}, ms by rounds {np.round(tm, 3).tolist()}, "
+                f"{time.time() - t0:.1f} s")
+    ws.close()
+    return np.concatenate(X), np.concatenate(Y), info
+
+
+def main(argv=None):
+    """Train the schedule model on the GPU and write the model files:
+        python -m iwpp.learned [--out DIR] [--steps N] [--lr LR]"""
+    import argparse
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", default=MODEL_DIR)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--c4", type=int, default=96)
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    t0 = time.time()
+    cases = training_cases({"c4": args.c4})
+    log(f"{len(cases)} training images generated in {time.time() - t0:.1f} s")
+    X, Y, info = build_dataset(cases, log=lambda m: None if " c4_" in m else log(m))
+    log(f"dataset: {X.shape[0]} tiles, k* histogram "
+        f"{np.bincount(Y[:, 0].astype(int), minlength=MAX_ROUNDS + 1).tolist()}")
+    w, hist = train(X, Y, steps=args.steps, lr=args.lr)
+    for s, L in hist[:: max(1, len(hist) // 10)] + hist[-1:]:
+        log(f"step {s}: loss {L:.5f}")
+    meta = {"params": MLP_PARAMS, "features": MLP_FEATURES, "feature_sub": FEATURE_SUB,
+            "max_rounds": MAX_ROUNDS, "samples": int(X.shape[0]), "images": info,
+            "steps": args.steps, "lr": args.lr, "loss_first": hist[0][1],
+            "loss_last": hist[-1][1],
+            "label_k_histogram": np.bincount(Y[:, 0].astype(int),
+                                             minlength=MAX_ROUNDS + 1).tolist()}
+    save_model(w, meta, os.path.join(args.out, "sched_mlp.f32"))
+    np.savez_compressed(os.path.join(args.out, "sched_dataset.npz"), X=X, Y=Y)
+    log(f"model written to {args.out} ({time.time() - t0:.1f} s)")
+
+
+if __name__ == "__main__":
+    main()

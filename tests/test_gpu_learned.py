@@ -1,0 +1,226 @@
+"""GPU tests of the learned schedule: the feature, MLP forward / backward and
+Adam kernels against the CPU oracle (oracle/mlp.py), training on the GPU, and
+reconstructions under learned schedules bit-exact against the reconstruction
+oracle.  Floating point tolerances are stated per test."""
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import iwpp
+import oracle
+from iwpp import learned
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_case(rng, h, w, dtype, kind):
+    top = 255 if dtype == np.uint8 else 4095
+    mask = rng.integers(0, top + 1, (h, w)).astype(dtype)
+    if kind == "seeds":
+        marker = np.zeros_like(mask)
+        idx = rng.choice(h * w, max(1, h * w // 500), replace=False)
+        marker.flat[idx] = mask.flat[idx]
+    else:
+        marker = np.maximum(mask.astype(np.int64) - top // 5, 0).astype(dtype)
+    return marker, mask
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("shape,tile,sub,pitch", [((300, 517), 128, 3, None),
+                                                  ((1100, 700), 256, 4, None),
+                                                  ((64, 64), 64, 1, None),
+                                                  ((97, 301), 64, 2, "odd"),
+                                                  ((2100, 4500), 2048, 4, None)])
+def test_tile_features(torch_cuda, dtype, shape, tile, sub, pitch):
+    say("features", dtype.__name__, shape, tile, sub, pitch)
+    rng = np.random.default_rng(11)
+    top = 255 if dtype == np.uint8 else 4095
+    mask = rng.integers(0, top + 1, shape).astype(dtype)
+    marker = rng.integers(0, top + 1, shape).astype(dtype)  # unclipped: negative gaps too
+    marker[rng.random(shape) < 0.1] = 0
+    eq = rng.random(shape) < 0.05
+    marker[eq] = mask[eq]
+    p = None
+    if pitch == "odd":  # 4-byte aligned rows that are not 16-byte aligned
+        p = (shape[1] * mask.itemsize + 3) // 4 * 4
+        p = p + 4 if p % 16 == 0 else p
+    J, I = iwpp.DeviceImage.from_numpy(marker, p), iwpp.DeviceImage.from_numpy(mask, p)
+    conn = 8 if tile != 256 else 4
+    raw, feat = iwpp.tile_features(J, I, tile, conn, sub)
+    torch_cuda.cuda.synchronize()
+    want_raw = M.tile_features_raw(marker, mask, tile, sub)
+    # float32 sums of at most tile*tile/sub integers: relative 1e-5
+    np.testing.assert_allclose(raw.cpu().numpy(), want_raw, rtol=1e-5, atol=1e-3)
+    np.testing.assert_allclose(feat.cpu().numpy(),
+                               M.normalise(want_raw, tile, sub, dtype, conn),
+                               rtol=1e-5, atol=1e-6)
+    assert np.array_equal(J.numpy(), marker)  # features do not touch the marker
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = learned.init_params(seed)
+    p[M.O_B1:M.O_W2] = rng.normal(0, 0.1, 64)
+    p[M.O_B2:M.O_W3] = rng.normal(0, 0.1, 64)
+    p[M.O_B3:] = rng.normal(0, 0.5, 2)
+    return p.astype(np.float32)
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("bf16", [True, False])
+@pytest.mark.parametrize("n", [1, 255, 1000, 70000])
+def test_mlp_forward(torch_cuda, bf16, n):
+    say("mlp_forward", bf16, n)
+    torch = torch_cuda
+    rng = np.random.default_rng(n)
+    p = _params(n)
+    x = rng.uniform(0, 1.5, (n, 16)).astype(np.float32)
+    pd = _dev(torch, learned.bf16_bits(p).view(np.int16) if bf16 else p)
+    y, acts = iwpp.mlp_forward(pd, _dev(torch, x), bf16, acts=True)
+    torch.cuda.synchronize()
+    pw = M.bf16_round(p) if bf16 else p
+    want, h1, h2 = M.forward(pw, x)
+    # FP32 accumulation of at most 65 products vs float64: |err| <= 1e-5 * scale
+    scale = 1.0 + np.abs(want).max()
+    np.testing.assert_allclose(y.cpu().numpy(), want, rtol=0, atol=1e-5 * scale)
+    a = acts.cpu().numpy()
+    np.testing.assert_allclose(a[:, :64], h1, rtol=0, atol=1e-5 * (1 + h1.max()))
+    np.testing.assert_allclose(a[:, 64:], h2, rtol=0, atol=1e-5 * (1 + h2.max()))
+
+
+@pytest.mark.parametrize("bf16", [True, False])
+@pytest.mark.parametrize("n", [7, 3000])
+def test_mlp_backward(torch_cuda, bf16, n):
+    say("mlp_backward", bf16, n)
+    torch = torch_cuda
+    rng = np.random.default_rng(100 + n)
+    p = _params(n + 1)
+    x = rng.uniform(0, 1.5, (n, 16)).astype(np.float32)
+    t = rng.uniform(0, 4, (n, 2)).astype(np.float32)
+    lw = (1.0, 3.0)
+    pd = _dev(torch, learned.bf16_bits(p).view(np.int16) if bf16 else p)
+    xd = _dev(torch, x)
+    y, acts = iwpp.mlp_forward(pd, xd, bf16, acts=True)
+    g = iwpp.mlp_backward(pd, xd, acts, y, _dev(torch, t), bf16, lw)
+    torch.cuda.synchronize()
+    pw = M.bf16_round(p) if bf16 else p
+    want = M.backward(pw, x, t, lw)
+    got = g.cpu().numpy().astype(np.float64)
+    # FP32 sums over n samples vs float64: 1e-4 of the largest gradient
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-4 * np.abs(want).max())
+    # and the gradient points downhill: a small step lowers the oracle loss
+    L0 = M.loss(pw, x, t, lw)
+    L1 = M.loss(pw - 1e-3 * got / np.linalg.norm(got), x, t, lw)
+    assert L1 < L0
+
+
+def test_mlp_adam(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(M.N_PARAMS).astype(np.float32)
+    wd, md, vd = _dev(torch, w.copy()), torch.zeros(M.N_PARAMS, device="cuda"), \
+        torch.zeros(M.N_PARAMS, device="cuda")
+    wbf = torch.zeros(M.N_PARAMS, dtype=torch.int16, device="cuda")
+    ww, mm, vv = w.astype(np.float64), np.zeros(M.N_PARAMS), np.zeros(M.N_PARAMS)
+    for step in range(1, 5):
+        g = rng.standard_normal(M.N_PARAMS).astype(np.float32)
+        iwpp.mlp_adam(wd, md, vd, _dev(torch, g), wbf, 1e-2, step)
+        ww, mm, vv = M.adam(ww, mm, vv, g.astype(np.float64), 1e-2, step)
+    torch.cuda.synchronize()
+    got = wd.cpu().numpy()
+    np.testing.assert_allclose(got, ww, rtol=0, atol=1e-5)
+    assert np.array_equal(wbf.cpu().numpy().view(np.uint16), M.bf16_bits(got))
+
+
+def test_training_reduces_loss(torch_cuda):
+    # targets from a random teacher network: the loss must fall by 10x
+    rng = np.random.default_rng(9)
+    teacher = _params(77)
+    X = rng.uniform(0, 1, (4000, 16)).astype(np.float32)
+    Y = M.forward(M.bf16_round(teacher), X)[0].astype(np.float32)
+    w, hist = learned.train(X, Y, steps=600, lr=3e-3, lw=(1.0, 1.0), log_every=100)
+    say("training loss", hist[0], hist[-1])
+    assert hist[-1][1] < 0.1 * hist[0][1]
+    # the returned master weights reproduce the last loss through the oracle
+    assert M.loss(M.bf16_round(w), X, Y) < 0.15 * hist[0][1]
+
+
+@pytest.fixture
+def model_guard():
+    yield
+    iwpp.set_model(None)
+
+
+def _model_with_bias(seed, k_bias, prio_scale=1.0):
+    p = _params(seed)
+    p[M.O_W3:M.O_B3] *= 0.3
+    p[M.O_B3] = k_bias
+    p[M.O_W3 + 64:M.O_B3] *= prio_scale
+    return p
+
+
+MODELS = [("zero_rounds", -5.0), ("four_rounds", 9.0), ("mixed_a", 1.2), ("mixed_b", 2.4)]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("queue", ["mutex", "dq", "uq"])
+def test_learned_reconstruction_exact(torch_cuda, model_guard, dtype, conn, queue):
+    rng = np.random.default_rng(zlib.crc32(f"{dtype.__name__}/{conn}/{queue}".encode()))
+    for mi, (name, kb) in enumerate(MODELS):
+        iwpp.set_model(_model_with_bias(mi + 3, kb))
+        for shape, tile, kind in (((300, 517), 64, "seeds"), ((700, 390), 128, "hdome"),
+                                  ((129, 1000), 64, "seeds")):
+            say("learned", name, dtype.__name__, conn, queue, shape, tile, kind)
+            marker, mask = rand_case(rng, *shape, dtype, kind)
+            want = oracle.reconstruct(marker, mask, conn)
+            J, I = iwpp.DeviceImage.from_numpy(marker), iwpp.DeviceImage.from_numpy(mask)
+            st = iwpp.reconstruct(J, I, connectivity=conn, tile=tile, queue=queue,
+                                  schedule="learned")
+            assert st["schedule"] == iwpp.SCHEDULES["learned"]
+            got = J.numpy()
+            assert np.array_equal(got, want), (name, shape, tile, kind, int((got != want).sum()))
+
+
+def test_learned_without_model_is_plain(torch_cuda, model_guard):
+    iwpp.set_model(None)
+    rng = np.random.default_rng(3)
+    marker, mask = rand_case(rng, 200, 300, np.uint8, "seeds")
+    J, I = iwpp.DeviceImage.from_numpy(marker), iwpp.DeviceImage.from_numpy(mask)
+    st = iwpp.reconstruct(J, I, connectivity=8, schedule="learned")
+    assert st["schedule"] == iwpp.SCHEDULES["plain"]
+    assert np.array_equal(J.numpy(), oracle.reconstruct(marker, mask, 8))
+
+
+def test_set_model_rejects_bad_input(torch_cuda, model_guard):
+    short = np.zeros(10, np.float32)
+    assert iwpp.lib().iwpp_sched_set_model(short.ctypes.data, short.size) == -1
+    bad = np.zeros(M.N_PARAMS, np.float32)
+    bad[7] = np.nan
+    with pytest.raises(iwpp.IWPPError):
+        iwpp.set_model(bad)
+
+
+def test_committed_model_exact_on_scaled_workloads(torch_cuda, model_guard):
+    # the shipped model on 4096^2 images of the seeds and h-dome generators
+    learned.install()
+    from iwpp import workloads as W
+    for case in (W.c2(side=4096, seed=W.S0 + 777), W.c1(side=2048, seed=W.S0 + 778)):
+        say("committed model", case.name)
+        want = oracle.reconstruct(case.marker, case.mask, case.connectivity)
+        J, I = iwpp.DeviceImage.from_numpy(case.marker), iwpp.DeviceImage.from_numpy(case.mask)
+        st = iwpp.reconstruct(J, I, connectivity=case.connectivity, queue="mutex",
+                              schedule="learned")
+        assert st["schedule"] == iwpp.SCHEDULES["learned"]
+        assert np.array_equal(J.numpy(), want)
