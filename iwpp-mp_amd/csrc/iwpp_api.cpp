@@ -480,9 +480,10 @@ int check_common(int dtype, int conn, int tile, int queue, int schedule) {
 }
 
 // Learned schedule for one image: tile features -> MLP (BF16) -> per tile
-// k = sweep rounds before the queue phase (0..4) and a priority.  Round r of
-// the sweep phase touches the tiles with k > r; the queue is seeded tile by
-// tile in decreasing priority.  Without a model the plain plan is kept.
+// a predicted number of sweep rounds before the queue phase (0..4) and a
+// priority.  Round r of the sweep phase touches the tiles scheduled for more
+// than r rounds; the queue is seeded tile by tile in decreasing priority.
+// Without a model the plain plan is kept.
 int learned_plan(Run& r, SchedulePlan& plan) {
     iwpp_workspace* ws = r.ws;
     std::vector<uint16_t> model;
@@ -514,12 +515,26 @@ int learned_plan(Run& r, SchedulePlan& plan) {
     HIP_TRY(hipMemcpyAsync(pred.data(), ws->pred, pred.size() * sizeof(float),
                            hipMemcpyDeviceToHost, r.s));
     HIP_TRY(hipStreamSynchronize(r.s));
-    std::vector<int> k(nt);
-    int rounds = 0;
+    // The image runs R = round(area-weighted mean of the predicted rounds)
+    // rounds; a tile whose own prediction is at least two rounds lower skips
+    // the rounds from its prediction on.
+    auto clamp_rounds = [](float v) {
+        return std::isfinite(v) ? (int)std::lround(std::min(4.f, std::max(0.f, v))) : 0;
+    };
+    double wsum = 0, wy = 0;
     for (int t = 0; t < nt; ++t) {
+        const int tx = t % ntx, ty = t / ntx;
+        const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
+                            (double)(std::min(d.H, (ty + 1) * tile) - ty * tile);
         const float v = pred[(size_t)t * 2];
-        k[t] = std::isfinite(v) ? (int)std::lround(std::min(4.f, std::max(0.f, v))) : 0;
-        rounds = std::max(rounds, k[t]);
+        wy += area * (std::isfinite(v) ? std::min(4.f, std::max(0.f, v)) : 0.f);
+        wsum += area;
+    }
+    const int rounds = clamp_rounds((float)(wy / wsum));
+    std::vector<int> k(nt);
+    for (int t = 0; t < nt; ++t) {
+        const int kt = clamp_rounds(pred[(size_t)t * 2]);
+        k[t] = kt <= rounds - 2 ? kt : rounds;
     }
     plan.sweep_rounds = rounds;
     r.st.schedule = IWPP_SCHED_LEARNED;

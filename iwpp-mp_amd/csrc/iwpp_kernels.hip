@@ -15,6 +15,17 @@ namespace iwpp {
 
 #define DEV static __device__ __forceinline__
 
+// Every pointer the kernels dereference outside LDS points to global memory.
+// Saying so (address space 1) makes the compiler emit global_* rather than
+// flat_* instructions for pointers it reads from descriptors; flat accesses
+// also count on the LDS/scalar counter and so serialise the memory waits.
+#define GAS __attribute__((address_space(1)))
+template <class T>
+DEV GAS T* gp(T* p) { return (GAS T*)p; }
+template <class T>
+DEV const GAS T* gp(const T* p) { return (const GAS T*)p; }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16-byte access
+
 // ---------------------------------------------------------------- atomics ---
 // Inside the queue kernels every access to shared mutable data -- marker
 // words, queue entries, counters, flags, state words -- is a relaxed
@@ -29,33 +40,33 @@ namespace iwpp {
 // gfx950, writes back / invalidates the whole XCD L2 each time; measured, that
 // was the dominant cost of a visit.  The mask is read-only and uses plain loads.
 DEV uint32_t a_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV void a_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_add(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_or(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_or(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_and(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_and(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_xchg(uint32_t* p, uint32_t v) {
-    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_exchange(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV uint32_t a_cas(uint32_t* p, uint32_t expected, uint32_t desired) {
-    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+    __hip_atomic_compare_exchange_strong(gp(p), &expected, desired, __ATOMIC_RELAXED,
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return expected;  // the value seen before the exchange
 }
 DEV unsigned long long a_load64(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV void a_store64(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV void mem_wait() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -64,11 +75,11 @@ DEV void mem_wait() {
 }
 // Device-coherent 32-bit accesses to the marker and the queues.
 DEV uint32_t ld_coh(const void* p) {
-    return __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+    return __hip_atomic_load(gp(reinterpret_cast<const uint32_t*>(p)), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 DEV void st_coh(void* p, uint32_t v) {
-    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), v, __ATOMIC_RELAXED,
+    __hip_atomic_store(gp(reinterpret_cast<uint32_t*>(p)), v, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -148,41 +159,42 @@ struct VecIO {
     static constexpr int BITS = 8 * (int)sizeof(T);
     static constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
 
+    // Every lane loads unconditionally (lanes past the row end re-read the
+    // row's first vector) and clears the bytes beyond the row afterwards.
+    // With no divergent branch around the load, a wave's loads of successive
+    // rows stay in flight together; behind a branch the compiler waits for
+    // each one at the join.  A vector never reaches past round_up(rowbytes,
+    // VB), which the pitch covers (VB = 16 is used only for 16-byte pitches).
     DEV void load(const uint8_t* row, int o, int rowbytes, uint32_t (&w)[NW]) {
-        if (o + VB <= rowbytes) {
-            if constexpr (VB == 16) {
-                const uint4 t = *reinterpret_cast<const uint4*>(row + o);
-                w[0] = t.x; w[1] = t.y; w[2] = t.z; w[3] = t.w;
-            } else {
-                w[0] = *reinterpret_cast<const uint32_t*>(row + o);
-            }
+        const int oc = o < rowbytes ? o : 0;
+        if constexpr (VB == 16) {
+            const u32x4 t = *gp(reinterpret_cast<const u32x4*>(row + oc));
+            w[0] = t.x; w[1] = t.y; w[2] = t.z; w[3] = t.w;
         } else {
+            w[0] = *gp(reinterpret_cast<const uint32_t*>(row + oc));
+        }
+        const int valid = rowbytes - o;  // bytes of this vector inside the row
 #pragma unroll
-            for (int i = 0; i < NW; ++i) w[i] = 0;
-            if (o < rowbytes) {
-#pragma unroll
-                for (int k = 0; k < NPX; ++k) {
-                    const int ob = o + k * (int)sizeof(T);
-                    if (ob < rowbytes)
-                        w[k / PPW] |= (uint32_t)(*reinterpret_cast<const T*>(row + ob))
-                                      << (BITS * (k % PPW));
-                }
-            }
+        for (int i = 0; i < NW; ++i) {
+            const int vb = valid - 4 * i;
+            const uint32_t keep = vb >= 4 ? 0xFFFFFFFFu
+                                : vb <= 0 ? 0u : (0xFFFFFFFFu >> (8 * (4 - vb)));
+            w[i] &= keep;
         }
     }
     DEV void store(uint8_t* row, int o, int rowbytes, const uint32_t (&w)[NW]) {
         if (o + VB <= rowbytes) {
             if constexpr (VB == 16) {
-                *reinterpret_cast<uint4*>(row + o) = make_uint4(w[0], w[1], w[2], w[3]);
+                *gp(reinterpret_cast<u32x4*>(row + o)) = u32x4{w[0], w[1], w[2], w[3]};
             } else {
-                *reinterpret_cast<uint32_t*>(row + o) = w[0];
+                *gp(reinterpret_cast<uint32_t*>(row + o)) = w[0];
             }
         } else if (o < rowbytes) {
 #pragma unroll
             for (int k = 0; k < NPX; ++k) {
                 const int ob = o + k * (int)sizeof(T);
                 if (ob < rowbytes)
-                    *reinterpret_cast<T*>(row + ob) = (T)((w[k / PPW] >> (BITS * (k % PPW))) & PM);
+                    *gp(reinterpret_cast<T*>(row + ob)) = (T)((w[k / PPW] >> (BITS * (k % PPW))) & PM);
             }
         }
     }
@@ -213,7 +225,15 @@ DEV bool span_on(const TileMask& tm, int x0, int x1, int y) {
 }
 
 // ------------------------------------------------------------ descriptors ---
-DEV const ImgDesc& desc(const Descs& a, int i) { return a.d ? a.d[i] : a.one; }
+// The descriptor of image i, by value: from the kernel argument for a single
+// image (scalar loads of the argument segment), else from the device table
+// (the index is wave-uniform, so scalar loads as well).  Returning a
+// reference would merge two address spaces into a generic pointer and turn
+// every field read into a flat load waited on one at a time.
+DEV ImgDesc desc(const Descs& a, int i) {
+    if (a.d == nullptr) return a.one;
+    return a.d[__builtin_amdgcn_readfirstlane(i)];
+}
 // Image owning global work item `idx` for a prefix field (binary search).
 #define IWPP_FIND(NAME, FIELD)                                           \
     DEV int NAME(const Descs& a, int idx) {                              \
@@ -520,7 +540,7 @@ __global__ __launch_bounds__(kWG) void k_row_pass_batch(Descs ds, uint32_t* chan
     __shared__ uint32_t sP[2][4], sQ[2][4];
     bool any_change = false;
     for (int r = blockIdx.x; r < ds.total; r += gridDim.x) {
-        const ImgDesc& d = desc(ds, find_row(ds, r));
+        const ImgDesc d = desc(ds, find_row(ds, r));
         row_stream<T, VB, 0, CLIP>(image_of(d), r - d.row_base, d.tm, sP, sQ, any_change);
     }
     if (any_change && changed) *changed = 1u;
@@ -564,48 +584,68 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
         V::unpack(w, prev);
     }
     bool any_change = false;
-    for (int i = 0; i < y1 - y0; ++i) {
-        const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
-        const int yp = y - DIR;
-        uint32_t jw[NW], iw[NW], v[NPX], c[NPX];
-        V::load(S + (size_t)y * ps, o, rowbytes, jw);
-        V::load(I + (size_t)y * pi, o, rowbytes, iw);
-        V::unpack(jw, v);
-        V::unpack(iw, c);
-        if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
+    // Rows go in groups of CU: the group's loads (source, mask and, for 8-
+    // connectivity, the strip's outside neighbours in the previous source row)
+    // are issued together before any of its rows is computed, so a wave keeps
+    // 2*CU loads in flight instead of waiting for each row.  Rows past the
+    // block are loaded from its last row (in bounds) and not computed.
+    constexpr int CU = VB == 16 ? 4 : 8;  // 16-byte lanes: fewer rows, same bytes in flight
+    const int n = y1 - y0;
+    for (int i0 = 0; i0 < n; i0 += CU) {
+        uint32_t jw[CU][NW], iw[CU][NW], le[CU], re[CU];
 #pragma unroll
-            for (int k = 0; k < NPX; ++k) c[k] = 0;
+        for (int u = 0; u < CU; ++u) {
+            const int i = min(i0 + u, n - 1);
+            const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
+            V::load(S + (size_t)y * ps, o, rowbytes, jw[u]);
+            V::load(I + (size_t)y * pi, o, rowbytes, iw[u]);
+            le[u] = re[u] = 0;
+            if constexpr (CONN == 8) {
+                const int yp = y - DIR;
+                const bool yp_in = yp >= 0 && yp < H;
+                if (lane == 0 && yp_in && xs0 > 0)
+                    le[u] = *gp(reinterpret_cast<const T*>(S + (size_t)yp * ps + (size_t)(xs0 - 1) * sizeof(T)));
+                if (lane == 63 && yp_in && xs1 < W)
+                    re[u] = *gp(reinterpret_cast<const T*>(S + (size_t)yp * ps + (size_t)xs1 * sizeof(T)));
+            }
         }
-        uint32_t m[NPX];
-        if constexpr (CONN == 8) {
-            uint32_t l_edge = 0, r_edge = 0;
-            const bool yp_in = yp >= 0 && yp < H;
-            if (lane == 0 && yp_in && xs0 > 0)
-                l_edge = *reinterpret_cast<const T*>(S + (size_t)yp * ps + (size_t)(xs0 - 1) * sizeof(T));
-            if (lane == 63 && yp_in && xs1 < W)
-                r_edge = *reinterpret_cast<const T*>(S + (size_t)yp * ps + (size_t)xs1 * sizeof(T));
-            const uint32_t left = from_left(prev[NPX - 1], l_edge);
-            const uint32_t right = from_right(prev[0], r_edge);
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+            if (i0 + u >= n) break;  // the same in every lane
+            const int i = i0 + u;
+            const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
+            uint32_t v[NPX], c[NPX], m[NPX];
+            V::unpack(jw[u], v);
+            V::unpack(iw[u], c);
+            if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
+#pragma unroll
+                for (int k = 0; k < NPX; ++k) c[k] = 0;
+            }
+            if constexpr (CONN == 8) {
+                const uint32_t left = from_left(prev[NPX - 1], le[u]);
+                const uint32_t right = from_right(prev[0], re[u]);
+#pragma unroll
+                for (int k = 0; k < NPX; ++k) {
+                    const uint32_t a = k > 0 ? prev[k - 1] : left;
+                    const uint32_t b = k < NPX - 1 ? prev[k + 1] : right;
+                    m[k] = max(prev[k], max(a, b));
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < NPX; ++k) m[k] = prev[k];
+            }
+            bool d = false;
 #pragma unroll
             for (int k = 0; k < NPX; ++k) {
-                const uint32_t a = k > 0 ? prev[k - 1] : left;
-                const uint32_t b = k < NPX - 1 ? prev[k + 1] : right;
-                m[k] = max(prev[k], max(a, b));
+                const uint32_t out = max(v[k], min(m[k], c[k]));
+                d |= out != v[k];
+                prev[k] = out;
             }
-        } else {
-#pragma unroll
-            for (int k = 0; k < NPX; ++k) m[k] = prev[k];
+            any_change |= d;
+            uint32_t ow[NW];
+            V::pack(prev, ow);
+            V::store(D + (size_t)y * pd, o, rowbytes, ow);
         }
-        bool d = false;
-#pragma unroll
-        for (int k = 0; k < NPX; ++k) {
-            const uint32_t out = max(v[k], min(m[k], c[k]));
-            d |= out != v[k];
-            prev[k] = out;
-        }
-        any_change |= d;
-        V::pack(prev, jw);
-        V::store(D + (size_t)y * pd, o, rowbytes, jw);
     }
     if (any_change && changed) *changed = 1u;
 }
@@ -616,7 +656,7 @@ template <class T, int VB, int CONN, int DIR, bool JP>
 __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ds.total) return;
-    const ImgDesc& d = desc(ds, find_cw(ds, w));
+    const ImgDesc d = desc(ds, find_cw(ds, w));
     const int local = w - d.cw_base;
     const int strip = local % d.strips, seg = local / d.strips;
     if (JP) col_block<T, VB, CONN, DIR>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
@@ -661,7 +701,7 @@ DEV void wave_lds_sync() {
 
 template <class T>
 DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
-    return *reinterpret_cast<const T*>(base + (size_t)y * pitch + (size_t)x * sizeof(T));
+    return *gp(reinterpret_cast<const T*>(base + (size_t)y * pitch + (size_t)x * sizeof(T)));
 }
 // One marker pixel through a device-coherent load of its (4-byte aligned) word.
 template <class T>
@@ -693,7 +733,7 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
         const int y = y0 - 1 + it * RPI + rl;
         const int yc = min(max(y, 0), im.H - 1);
         gj[it] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
-        gi[it] = *reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff);
+        gi[it] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff));
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -980,7 +1020,7 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
     const int lane = lane_id();
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ds.total) return;
-    const ImgDesc& dsc = desc(ds, find_sw(ds, w));
+    const ImgDesc dsc = desc(ds, find_sw(ds, w));
     const Image im = image_of(dsc);
     const int nmx = dsc.nmx;
     uint8_t* flags = flags_all + dsc.mp_base;
@@ -1004,49 +1044,71 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
     auto edge = [&](int y, int x) -> uint32_t {
         return (y >= 0 && y < im.H && x >= 0 && x < im.W) ? ldp<T>(im.J, im.pj, x, y) : 0u;
     };
-    uint32_t up[NPX], cur[NPX], dn[NPX], msk[NPX];
+    uint32_t up[NPX], cur[NPX];
     load_row(im.J, im.pj, y0 - 1, up);
     load_row(im.J, im.pj, y0, cur);
     uint32_t upL = lane == 0 ? edge(y0 - 1, xs0 - 1) : 0u, upR = lane == 63 ? edge(y0 - 1, xs1) : 0u;
     uint32_t cuL = lane == 0 ? edge(y0, xs0 - 1) : 0u, cuR = lane == 63 ? edge(y0, xs1) : 0u;
     bool un = false;
-    for (int y = y0; y < y1; ++y) {
-        load_row(im.J, im.pj, y + 1, dn);
-        load_row(im.I, im.pi, y, msk);
-        const uint32_t dnL = lane == 0 ? edge(y + 1, xs0 - 1) : 0u;
-        const uint32_t dnR = lane == 63 ? edge(y + 1, xs1) : 0u;
-        // neighbours left of pixel 0 and right of pixel NPX-1 in rows y-1, y, y+1
-        const uint32_t lc = from_left(cur[NPX - 1], cuL), rc = from_right(cur[0], cuR);
-        uint32_t lu = 0, ru = 0, ld = 0, rd = 0;
-        if constexpr (CONN == 8) {
-            lu = from_left(up[NPX - 1], upL);
-            ru = from_right(up[0], upR);
-            ld = from_left(dn[NPX - 1], dnL);
-            rd = from_right(dn[0], dnR);
-        }
+    // Groups of SU rows: the loads of the group (next marker rows, mask rows,
+    // outside neighbours) are issued before any row of it is tested.
+    constexpr int SU = VB == 16 ? 4 : 8;
+    for (int g = y0; g < y1; g += SU) {
+        uint32_t dnw[SU][NW], mkw[SU][NW], dL[SU], dR[SU];
 #pragma unroll
-        for (int k = 0; k < NPX; ++k) {
-            uint32_t m = max(up[k], dn[k]);
-            m = max(m, max(k > 0 ? cur[k - 1] : lc, k < NPX - 1 ? cur[k + 1] : rc));
-            if constexpr (CONN == 8) {
-                m = max(m, max(k > 0 ? up[k - 1] : lu, k < NPX - 1 ? up[k + 1] : ru));
-                m = max(m, max(k > 0 ? dn[k - 1] : ld, k < NPX - 1 ? dn[k + 1] : rd));
+        for (int u = 0; u < SU; ++u) {
+            const int yd = g + u + 1;                    // marker row below row g+u
+            const int ym = min(g + u, y1 - 1);           // mask row g+u (clamped)
+            const int ydc = min(yd, im.H - 1);           // in-bounds row to load
+            V::load(im.J + (size_t)ydc * im.pj, o, rowbytes, dnw[u]);
+            V::load(im.I + (size_t)ym * im.pi, o, rowbytes, mkw[u]);
+            if (yd >= im.H) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i) dnw[u][i] = 0;  // below the image: absent
             }
-            un |= min(m, msk[k]) > cur[k];
+            dL[u] = lane == 0 ? edge(yd, xs0 - 1) : 0u;
+            dR[u] = lane == 63 ? edge(yd, xs1) : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < NPX; ++k) {
-            up[k] = cur[k];
-            cur[k] = dn[k];
+        for (int u = 0; u < SU; ++u) {
+            if (g + u >= y1) break;  // the same in every lane
+            uint32_t dn[NPX], msk[NPX];
+            V::unpack(dnw[u], dn);
+            V::unpack(mkw[u], msk);
+            const uint32_t dnL = dL[u], dnR = dR[u];
+            // neighbours left of pixel 0 and right of pixel NPX-1 in rows y-1, y, y+1
+            const uint32_t lc = from_left(cur[NPX - 1], cuL), rc = from_right(cur[0], cuR);
+            uint32_t lu = 0, ru = 0, ld = 0, rd = 0;
+            if constexpr (CONN == 8) {
+                lu = from_left(up[NPX - 1], upL);
+                ru = from_right(up[0], upR);
+                ld = from_left(dn[NPX - 1], dnL);
+                rd = from_right(dn[0], dnR);
+            }
+#pragma unroll
+            for (int k = 0; k < NPX; ++k) {
+                uint32_t m = max(up[k], dn[k]);
+                m = max(m, max(k > 0 ? cur[k - 1] : lc, k < NPX - 1 ? cur[k + 1] : rc));
+                if constexpr (CONN == 8) {
+                    m = max(m, max(k > 0 ? up[k - 1] : lu, k < NPX - 1 ? up[k + 1] : ru));
+                    m = max(m, max(k > 0 ? dn[k - 1] : ld, k < NPX - 1 ? dn[k + 1] : rd));
+                }
+                un |= min(m, msk[k]) > cur[k];
+            }
+#pragma unroll
+            for (int k = 0; k < NPX; ++k) {
+                up[k] = cur[k];
+                cur[k] = dn[k];
+            }
+            upL = cuL; upR = cuR; cuL = dnL; cuR = dnR;
         }
-        upL = cuL; upR = cuR; cuL = dnL; cuR = dnR;
     }
     const uint64_t b = ballot(un);
     const int mx0 = xs0 / kMP;
     if (lane < MPS && mx0 + lane < nmx) {
         const bool f = ((b >> (lane * LPM)) & ((LPM == 64) ? ~0ull : ((1ull << LPM) - 1ull))) != 0;
-        flags[(size_t)my * nmx + mx0 + lane] = f ? 1 : 0;
-        if (f && count) atomicAdd(count, 1u);
+        *gp(flags + (size_t)my * nmx + mx0 + lane) = f ? 1 : 0;
+        if (f && count) a_add(count, 1u);
     }
 }
 
@@ -1115,7 +1177,8 @@ struct PhaseTimer {
     __device__ void flush(uint32_t* ctl) {
         if (!on || lane_id() != 0) return;
         for (int k = 0; k < 5; ++k)
-            atomicAdd(reinterpret_cast<unsigned long long*>(ctl_at(ctl, C_PH_POP + k)), acc[k]);
+            __hip_atomic_fetch_add(gp(reinterpret_cast<unsigned long long*>(ctl_at(ctl, C_PH_POP + k))),
+                                   (unsigned long long)acc[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 };
 
@@ -1206,7 +1269,7 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
         m = first_lane(m);
         mem_wait();
         pt.lap(0);
-        const ImgDesc& dsc = desc(ds, find_mp(ds, (int)m));
+        const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
         const Image im = image_of(dsc);
         const int ml = (int)m - dsc.mp_base;
         const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
@@ -1247,7 +1310,8 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
     pt.lap(0);
     pt.flush(q.ctl);
     if (lane == 0 && visits) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(ctl_at(q.ctl, C_VISITS_LO)), visits);
+        __hip_atomic_fetch_add(gp(reinterpret_cast<unsigned long long*>(ctl_at(q.ctl, C_VISITS_LO))),
+                               (unsigned long long)visits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1306,7 +1370,7 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
             m = first_lane(m);
             mem_wait();
             pt.lap(0);
-            const ImgDesc& dsc = desc(ds, find_mp(ds, (int)m));
+            const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
             const Image im = image_of(dsc);
             const int ml = (int)m - dsc.mp_base;
             const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
@@ -1361,7 +1425,8 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
     pt.lap(0);
     pt.flush(q.ctl);
     if (lane == 0 && visits) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(ctl_at(q.ctl, C_VISITS_LO)), visits);
+        __hip_atomic_fetch_add(gp(reinterpret_cast<unsigned long long*>(ctl_at(q.ctl, C_VISITS_LO))),
+                               (unsigned long long)visits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

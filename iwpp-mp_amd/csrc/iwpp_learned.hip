@@ -23,7 +23,7 @@ namespace iwpp {
 // Work split: one workgroup per (tile, slab of kSlab sampled rows); partial
 // statistics go to part[tile][slab][16] and a second kernel combines them.
 // Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
-constexpr int kSlab = 32;
+constexpr int kSlab = 8;
 constexpr int kStats = 16;
 
 DEV bool is_max_stat(int k) { return k == 4 || k == 8 || k == 15; }
@@ -207,45 +207,55 @@ constexpr int O_W1 = 0, O_B1 = O_W1 + 64 * 16, O_W2 = O_B1 + 64, O_B2 = O_W2 + 6
               O_W3 = O_B2 + 64, O_B3 = O_W3 + 2 * 64;
 static_assert(O_B3 + 2 == kMlpParams, "MLP layout");
 
-// One thread per sample; weights in LDS as float (from BF16 or FP32).
-// acts (optional, training): per sample h1[64] h2[64] pre-activations' sign
-// is recovered from h (ReLU).
+// One wave per sample (grid-stride over samples); lane j owns hidden unit j
+// of both layers and keeps its weights in registers (W1 row j, W2 row j, the
+// two W3 entries of column j).  Inputs and layer-1 activations reach every
+// lane through readlane; the outputs are wave sums.  FP32 accumulation.
+// acts (optional, training): per sample h1[64] then h2[64] (after ReLU).
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_mlp_forward(const void* params, const float* x, int n,
                                                       float* y, float* acts) {
-    __shared__ float w[kMlpParams];
-    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x)
-        w[i] = BF16 ? bf16_to_f32(reinterpret_cast<const uint16_t*>(params)[i])
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    auto W = [&](int i) -> float {
+        return BF16 ? bf16_to_f32(reinterpret_cast<const uint16_t*>(params)[i])
                     : reinterpret_cast<const float*>(params)[i];
-    __syncthreads();
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        float in[16];
+    };
+    float w1[16], w2[64];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) in[k] = x[(size_t)s * 16 + k];
-        float h1[64];
-#pragma unroll 8
-        for (int o = 0; o < 64; ++o) {
-            float a = w[O_B1 + o];
+    for (int k = 0; k < 16; ++k) w1[k] = W(O_W1 + lane * 16 + k);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) a = fmaf(w[O_W1 + o * 16 + k], in[k], a);
-            h1[o] = fmaxf(a, 0.f);
-        }
-        float y0 = w[O_B3], y1 = w[O_B3 + 1];
-        for (int o = 0; o < 64; ++o) {
-            float a = w[O_B2 + o];
+    for (int k = 0; k < 64; ++k) w2[k] = W(O_W2 + lane * 64 + k);
+    const float b1 = W(O_B1 + lane), b2 = W(O_B2 + lane);
+    const float w30 = W(O_W3 + lane), w31 = W(O_W3 + 64 + lane);
+    const float b30 = W(O_B3), b31 = W(O_B3 + 1);
+    for (int s = wave; s < n; s += nwaves) {
+        const float xv = lane < 16 ? x[(size_t)s * 16 + lane] : 0.f;
+        float a = b1;
 #pragma unroll
-            for (int k = 0; k < 64; ++k) a = fmaf(w[O_W2 + o * 64 + k], h1[k], a);
-            const float h2 = fmaxf(a, 0.f);
-            y0 = fmaf(w[O_W3 + o], h2, y0);
-            y1 = fmaf(w[O_W3 + 64 + o], h2, y1);
-            if (acts) acts[(size_t)s * 128 + 64 + o] = h2;
+        for (int k = 0; k < 16; ++k)
+            a = fmaf(w1[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), k)), a);
+        const float h1 = fmaxf(a, 0.f);
+        a = b2;
+#pragma unroll
+        for (int k = 0; k < 64; ++k)
+            a = fmaf(w2[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k)), a);
+        const float h2 = fmaxf(a, 0.f);
+        float y0 = w30 * h2, y1 = w31 * h2;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            y0 += __shfl_xor(y0, 32 >> i, 64);
+            y1 += __shfl_xor(y1, 32 >> i, 64);
         }
         if (acts) {
-#pragma unroll 8
-            for (int k = 0; k < 64; ++k) acts[(size_t)s * 128 + k] = h1[k];
+            acts[(size_t)s * 128 + lane] = h1;
+            acts[(size_t)s * 128 + 64 + lane] = h2;
         }
-        y[(size_t)s * 2] = y0;
-        y[(size_t)s * 2 + 1] = y1;
+        if (lane == 0) {
+            y[(size_t)s * 2] = y0 + b30;
+            y[(size_t)s * 2 + 1] = y1 + b31;
+        }
     }
 }
 
@@ -364,7 +374,7 @@ hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, 
 hipError_t launch_mlp_forward(const void* params, bool bf16, const float* x, int n, float* y,
                               float* acts, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    const int grid = std::min(1024, (n + 255) / 256);
+    const int grid = std::min(2048, (n + 3) / 4);  // 4 waves (samples) per workgroup
     if (bf16) hipLaunchKernelGGL(k_mlp_forward<true>, grid, 256, 0, s, params, x, n, y, acts);
     else hipLaunchKernelGGL(k_mlp_forward<false>, grid, 256, 0, s, params, x, n, y, acts);
     return hipGetLastError();

@@ -16,7 +16,8 @@ Traces.  For one training image, with the schedule's tile size:
   * the tile's mean reconstructed value.
 Labels.  For every training image the whole reconstruction is timed with
 k = 0..4 uniform sweep rounds; every tile of the image gets the fastest k,
-and priority_t = mean reconstructed value of t / max mask value, so tiles
+(the fewest rounds among those within 3% of it), and priority_t = mean
+reconstructed value of t / max mask value, so tiles
 whose flood carries the highest values are seeded first (their propagation
 overwrites lower floods, which would otherwise be propagated and then
 overtaken).  The reference has no counterpart; its schedule is fixed
@@ -187,11 +188,16 @@ def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=3, workspace=Non
     return out
 
 
+TIE = 1.03  # timings within 3% count as equal (the smaller k wins)
+
+
 def labels(tr, timings):
-    """(k*, priority) per tile: k* = the number of uniform sweep rounds with
-    the shortest measured reconstruction of the tile's image; priority = the
+    """(k*, priority) per tile: k* = the fewest uniform sweep rounds whose
+    measured reconstruction of the tile's image is within TIE of the
+    fastest; priority = the
     tile's mean reconstructed value over the image's largest mask value."""
-    k = float(np.argmin(timings))
+    t = np.asarray(timings, dtype=np.float64)
+    k = float(np.flatnonzero(t <= TIE * t.min())[0])  # fewest rounds within TIE of the best
     prio = (tr["final_mean"] / max(tr["maxval"], 1.0)).astype(np.float32)
     return np.stack([np.full(prio.shape, k, np.float32), prio], 1)
 

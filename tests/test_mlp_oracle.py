@@ -205,6 +205,8 @@ def test_labels_pick_fastest_round():
     tr = {"final_mean": np.array([10.0, 20.0, 30.0]), "maxval": 255.0}
     y = learned.labels(tr, np.array([5.0, 3.0, 2.5, 2.7, 9.0]))
     assert y[:, 0].tolist() == [2, 2, 2]
+    # within 3% of the best, the fewest rounds win
+    assert learned.labels(tr, np.array([5.0, 2.55, 2.5, 2.7, 9.0]))[0, 0] == 1
     np.testing.assert_allclose(y[:, 1], [10 / 255, 20 / 255, 30 / 255], rtol=1e-6)
 
 
