@@ -3,11 +3,15 @@
 
 A step = one full reconstruction of the workload's marker under its mask
 (marker reset from a resident copy, then iwpp_reconstruct_ex), inputs already
-in HBM.  value = image pixels / step time, in Mpixel/s.  The output of the last
-timed step is compared bit for bit with the CPU oracle.
+in HBM.  value = image pixels / step time, in Mpixel/s.  The default workload
+is the 32768^2 learned-schedule configuration: the step is timed with the
+learned schedule (value) and with the plain heuristic schedule (reported
+beside it).  The outputs of the timed steps are compared bit for bit with the
+CPU oracle.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--queue uq]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5]
+                    [--schedule both|learned|plain] [--queue auto]
 """
 import argparse
 import json
@@ -34,9 +38,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--queue", default="auto", choices=["mutex", "dq", "uq", "auto"])
-    ap.add_argument("--schedule", default="plain", choices=["plain", "learned"])
+    ap.add_argument("--schedule", default="both", choices=["both", "plain", "learned"],
+                    help="both: value is the learned schedule, the plain one is timed beside it")
     ap.add_argument("--sweeps", type=int, default=-1, help="sweep rounds (-1: schedule)")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
@@ -71,25 +76,41 @@ def main():
     ws = iwpp.Workspace(px, marker0.dtype)
     opt = iwpp.default_options(sweep_rounds=args.sweeps)
 
-    def step():
-        work.buf.copy_(marker0.buf)
-        return iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue,
-                                args.schedule, opt, ws)
+    from iwpp import learned
 
-    for i in range(args.warmup):
-        st = step()
-        log(f"warmup {i}: {st}")
-    torch.cuda.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(step())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    def run_schedule(schedule):
+        if schedule == "learned":
+            learned.install()
+        else:
+            iwpp.set_model(None)
+
+        def step():
+            work.buf.copy_(marker0.buf)
+            return iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue,
+                                    schedule, opt, ws)
+
+        for i in range(args.warmup):
+            st = step()
+            log(f"{schedule} warmup {i}: {st}")
+        torch.cuda.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            stats.append(step())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / args.steps
+        log(f"{schedule}: timed {args.steps} steps: {ms:.3f} ms/step, "
+            f"{px / (ms * 1e-3) / 1e6:.1f} Mpixel/s")
+        return ms, stats, work.numpy()
+
+    runs = {}
+    for sched in (("plain", "learned") if args.schedule == "both" else (args.schedule,)):
+        runs[sched] = run_schedule(sched)
+    main_sched = "learned" if "learned" in runs else "plain"
+    ms_step, stats, result = runs[main_sched]
     value = px / (ms_step * 1e-3) / 1e6
-    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.1f} Mpixel/s")
-    result = work.numpy()
+    iwpp.set_model(None)
 
     # --- per-kernel device times (HIP events on the stream the kernels run on)
     stream = torch.cuda.current_stream()
@@ -176,8 +197,8 @@ def main():
                "sample": f"full {case.name} image, median of {len(times)} runs "
                          f"({med * 1e3:.1f} ms each), oracle/iwpp_oracle.c -O3",
                "host_cpus": os.cpu_count()}
-        exact = bool(np.array_equal(result, want))
-        log(f"cpu oracle {med * 1e3:.1f} ms; GPU output bit-exact: {exact}")
+        exact = all(bool(np.array_equal(r[2], want)) for r in runs.values())
+        log(f"cpu oracle {med * 1e3:.1f} ms; GPU outputs bit-exact: {exact}")
 
     out = {
         "metric": METRIC,
@@ -194,7 +215,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": case.name, "pixels": px, "connectivity": case.connectivity,
                    "tile": case.tile, "queue": args.queue, "queue_used": used_queue,
-                   "schedule": args.schedule,
+                   "schedule": main_sched,
                    **case.meta},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -202,7 +223,14 @@ def main():
         "phases_ms": phases,
         "counters": counters,
         "sweep_kernels": sweep_kernels,
+        "schedules": {k: {"ms_per_step": v[0], "mpixel_s": px / (v[0] * 1e-3) / 1e6,
+                          "sweep_rounds": v[1][-1]["sweep_rounds"],
+                          "tiles_skipped": v[1][-1]["sweeps_skipped_tiles"],
+                          "schedule_used": {0: "plain", 1: "learned"}[v[1][-1]["schedule"]]}
+                      for k, v in runs.items()},
     }
+    if len(runs) == 2:
+        out["learned_vs_plain"] = runs["plain"][0] / runs["learned"][0]
     print(json.dumps(out), flush=True)
 
 
@@ -280,6 +308,17 @@ def bench_batch(args):
     batch_st = stats[-1]
     batch_same = all(np.array_equal(m.numpy(), o) for m, o in zip(mkw, loop_out))
     log(f"batched: {ms_batch:.2f} ms, {px / ms_batch / 1e3:.1f} Mpixel/s, same as loop: {batch_same}")
+    from iwpp import learned
+    learned.install()
+    stats_l = []
+    ms_learned = timed(lambda: stats_l.append(iwpp.reconstruct_batch(pairs, 8, 0, "auto",
+                                                                     "learned", None, ws)),
+                       args.steps, args.warmup)
+    learned_st = stats_l[-1]
+    learned_same = all(np.array_equal(m.numpy(), o) for m, o in zip(mkw, loop_out))
+    iwpp.set_model(None)
+    log(f"batched, learned schedule: {ms_learned:.2f} ms, {px / ms_learned / 1e3:.1f} Mpixel/s, "
+        f"same as loop: {learned_same}")
     gws = iwpp.Workspace()
     reset()
     torch.cuda.synchronize()
@@ -289,8 +328,9 @@ def bench_batch(args):
     graph_same = all(np.array_equal(m.numpy(), o) for m, o in zip(mkw, loop_out))
     graph_st = g.stats()
     log(f"graph: {ms_graph:.2f} ms, {px / ms_graph / 1e3:.1f} Mpixel/s, same as loop: {graph_same}")
-    best_ms = min(ms_batch, ms_graph)
-    form = "graph" if ms_graph <= ms_batch else "batched call"
+    forms = {"graph": ms_graph, "batched call": ms_batch, "batched call, learned": ms_learned}
+    form = min(forms, key=forms.get)
+    best_ms = forms[form]
 
     cpu, exact = None, None
     if not args.no_cpu:
@@ -307,7 +347,7 @@ def bench_batch(args):
         cpu = {"value": px_cpu / t_cpu / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
                "sample": f"first {px_cpu} px of the c4 images in order, "
                          f"{t_cpu:.1f} s, oracle/iwpp_oracle.c -O3", "host_cpus": os.cpu_count()}
-        exact = bool(ok and batch_same and graph_same)
+        exact = bool(ok and batch_same and graph_same and learned_same)
         log(f"oracle on the sample: {ok}")
     phases = {k: batch_st[k] for k in ("ms_sweep", "ms_seed", "ms_queue")}
     rounds = max(1, batch_st["sweep_rounds"])
@@ -345,7 +385,11 @@ def bench_batch(args):
         "batched_call": {"ms": ms_batch, "mpixel_s": px / ms_batch / 1e3, "counters": batch_st},
         "graph_replay": {"ms": ms_graph, "mpixel_s": px / ms_graph / 1e3,
                          "counters": graph_st},
+        "batched_learned": {"ms": ms_learned, "mpixel_s": px / ms_learned / 1e3,
+                            "counters": learned_st, "same_as_loop": learned_same},
         "speedup_vs_loop": ms_loop / best_ms,
+        "learned_batch_vs_loop": ms_loop / ms_learned,
+        "learned_batch_vs_plain_batch": ms_batch / ms_learned,
     }
     g.close()
     print(json.dumps(out), flush=True)

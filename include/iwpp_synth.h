@@ -13,8 +13,9 @@
  * Kovesi's "fast almost-Gaussian filtering" rule for the requested sigma,
  * reflect borders, 8 fractional bits kept between passes.
  *
- * dtype: 1 = uint8, 2 = uint16.  Pitches are row strides in bytes.
- * Functions return 0 on success and a negative value on bad arguments.
+ * dtype: 1 = uint8, 2 = uint16.  Pitches are row strides in bytes; uint16
+ * images need even base addresses and pitches.  Functions return 0 on
+ * success and a negative value on bad arguments.
  */
 #ifndef IWPP_SYNTH_H
 #define IWPP_SYNTH_H

@@ -101,10 +101,13 @@ struct iwpp_workspace {
     size_t feat_n = 0;
     float* pred = nullptr;      // MLP outputs per tile
     size_t pred_n = 0;
-    uint32_t* order = nullptr;  // seeding order of the megapixels
+    uint32_t* order = nullptr;  // seeding order: tile ranks then tile starts
     size_t order_n = 0;
+    void* host_plan = nullptr;  // pinned staging for the learned plan (both ways)
+    size_t host_plan_n = 0;
     uint16_t* model = nullptr;  // BF16 MLP weights on this device
     int model_version = -1;
+    hipStream_t own = nullptr;  // used when the caller passes the NULL stream
 };
 
 extern "C" void iwpp_default_options(iwpp_options* o) {
@@ -153,6 +156,8 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->order);
     (void)hipFree(ws->model);
     if (ws->host_ctl) (void)hipHostFree(ws->host_ctl);
+    if (ws->host_plan) (void)hipHostFree(ws->host_plan);
+    if (ws->own) (void)hipStreamDestroy(ws->own);
     for (hipEvent_t e : ws->ev)
         if (e) (void)hipEventDestroy(e);
     delete ws;
@@ -170,6 +175,24 @@ iwpp_workspace* default_ws() {
     if ((int)g_default_ws.size() <= dev) g_default_ws.resize(dev + 1, nullptr);
     if (!g_default_ws[dev]) g_default_ws[dev] = iwpp_workspace_create(0, 0);
     return g_default_ws[dev];
+}
+
+// The stream a call runs on.  For the NULL stream the workspace's own stream
+// (created on first use, blocking, so it is ordered after earlier work on the
+// NULL stream; the calls that use it synchronise it before returning):
+// cooperative launches (DQ/UQ) are refused on the NULL stream by some HIP
+// runtimes.
+hipError_t call_stream(iwpp_workspace* ws, void* stream, hipStream_t* out) {
+    if (stream) {
+        *out = (hipStream_t)stream;
+        return hipSuccess;
+    }
+    if (!ws->own) {
+        const hipError_t e = hipStreamCreate(&ws->own);
+        if (e != hipSuccess) return e;
+    }
+    *out = ws->own;
+    return hipSuccess;
 }
 
 // The learned schedule's model: BF16 weights, versioned so that workspaces
@@ -213,7 +236,8 @@ struct Run {
     // learned schedule (single image): per-round tile masks, priority order
     const uint8_t* round_masks = nullptr;
     int sched_tile = 0, sched_ntx = 0, sched_nt = 0;
-    const uint32_t* order = nullptr;
+    const uint32_t* order_tiles = nullptr, *order_start = nullptr;
+    int order_nt = 0, order_ntx = 0, order_mpt = 0;
 };
 
 int ensure_events(iwpp_workspace* ws) {
@@ -322,17 +346,22 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
     return IWPP_OK;
 }
 
-Descs with_total(Descs d, int total) {
+Descs with_total(Descs d, int total, int round = 0) {
     d.total = total;
+    d.round = round;
     return d;
 }
 
 // Sweep phase: `rounds` rounds of row pass + column pass down + column pass up.
 int sweep_phase(Run& r, int rounds) {
-    if (rounds <= 0) {
-        // clip only
-        for (const ImgDesc& d : r.h)
+    if (rounds <= 0) {  // clip only
+        if (r.ds.n == 1) {
+            const ImgDesc& d = r.h[0];
             HIP_TRY(launch_clip(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.s));
+        } else {  // a round index past every image's rounds: rows are only clipped
+            HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, 0x7FFFFFFF), r.dtype,
+                                          true, nullptr, r.s));
+        }
         return IWPP_OK;
     }
     if (r.round_masks) {  // blocks may be skipped: start the scratch plane at 0
@@ -351,12 +380,12 @@ int sweep_phase(Run& r, int rounds) {
             HIP_TRY(launch_row_pass(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, k == 0, d.tm,
                                     nullptr, r.s));
         } else {
-            HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows), r.dtype, k == 0,
+            HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0,
                                           nullptr, r.s));
         }
-        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw), r.dtype, r.conn, +1, true,
+        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1, true,
                                       r.wide, nullptr, r.s));
-        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw), r.dtype, r.conn, -1, false,
+        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1, false,
                                       r.wide, nullptr, r.s));
     }
     r.st.sweep_rounds += rounds;
@@ -395,7 +424,15 @@ int setup_queue(Run& r) {
     q.max_rounds = 1 << 30;
     q.timeout_ticks = timeout_ticks(r.opt.timeout_s > 0 ? r.opt.timeout_s : 60.0);
     q.trace = r.opt.trace_phases ? 1 : 0;
-    q.order = r.order;
+    q.order_tiles = r.order_tiles;
+    q.order_start = r.order_start;
+    q.order_nt = r.order_nt;
+    q.order_ntx = r.order_ntx;
+    q.order_mpt = r.order_mpt;
+    if (r.order_tiles) {
+        q.order_nmx = r.h[0].nmx;
+        q.order_nmy = r.h[0].nmy;
+    }
     r.cap = cap;
     return IWPP_OK;
 }
@@ -484,25 +521,124 @@ int check_common(int dtype, int conn, int tile, int queue, int schedule) {
 // priority.  Round r of the sweep phase touches the tiles scheduled for more
 // than r rounds; the queue is seeded tile by tile in decreasing priority.
 // Without a model the plain plan is kept.
-int learned_plan(Run& r, SchedulePlan& plan) {
-    iwpp_workspace* ws = r.ws;
+// The current model on the workspace's device; *have = false without one.
+int ensure_model(iwpp_workspace* ws, bool* have) {
     std::vector<uint16_t> model;
     int version = 0;
     {
         std::lock_guard<std::mutex> lk(g_model_mu);
-        model = g_model;
+        *have = !g_model.empty();
+        if (!*have) return IWPP_OK;
+        if (ws->model_version != g_model_version) model = g_model;
         version = g_model_version;
     }
-    if (model.empty()) return IWPP_OK;  // fallback: the plain heuristic
+    if (ws->model_version != version) {
+        if (!ws->model) HIP_TRY(hipMalloc((void**)&ws->model, kMlpParams * sizeof(uint16_t)));
+        HIP_TRY(hipMemcpy(ws->model, model.data(), kMlpParams * sizeof(uint16_t),
+                          hipMemcpyHostToDevice));
+        ws->model_version = version;
+    }
+    return IWPP_OK;
+}
+
+// Pinned host staging of at least `bytes`.
+int ensure_host_plan(iwpp_workspace* ws, size_t bytes) {
+    if (ws->host_plan_n >= bytes) return IWPP_OK;
+    if (ws->host_plan) HIP_TRY(hipHostFree(ws->host_plan));
+    ws->host_plan = nullptr;
+    ws->host_plan_n = 0;
+    HIP_TRY(hipHostMalloc(&ws->host_plan, bytes, 0));
+    ws->host_plan_n = bytes;
+    return IWPP_OK;
+}
+
+int clamp_rounds(float v) {
+    return std::isfinite(v) ? (int)std::lround(std::min(4.f, std::max(0.f, v))) : 0;
+}
+
+// Learned schedule for a batch: features of every tile of every image (one
+// set of launches), MLP, then per image R_i = round(area-weighted mean of its
+// tiles' predicted rounds); the sweep kernels skip an image from round R_i on
+// (images with R_i = 0 are only clipped).  Seeding keeps index order.
+int learned_plan_batch(Run& r, SchedulePlan& plan) {
+    iwpp_workspace* ws = r.ws;
+    bool have = false;
+    int rc = ensure_model(ws, &have);
+    if (rc || !have) return rc;
+    const int n = (int)r.h.size(), tile = plan.tile;
+    std::vector<uint32_t> base(n + 1, 0);
+    bool v16 = true;
+    for (int i = 0; i < n; ++i) {
+        const ImgDesc& d = r.h[i];
+        base[i + 1] = base[i] + (uint32_t)(((d.W + tile - 1) / tile) * ((d.H + tile - 1) / tile));
+        v16 = v16 && (uintptr_t)d.J % 16 == 0 && (uintptr_t)d.I % 16 == 0 && d.pj % 16 == 0 &&
+              d.pi % 16 == 0;
+    }
+    const int total = (int)base[n];
+    rc = ensure_host_plan(ws, (size_t)total * 2 * sizeof(float) + (size_t)(n + 1) * 4 + 64);
+    if (rc) return rc;
+    float* pred = (float*)ws->host_plan;
+    uint32_t* h_base = (uint32_t*)(pred + (size_t)total * 2);
+    std::copy(base.begin(), base.end(), h_base);
+    HIP_TRY(grow(ws->order, ws->order_n, (size_t)n + 1));
+    HIP_TRY(hipMemcpyAsync(ws->order, h_base, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, r.s));
+    HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
+                 (size_t)total * features_slabs(tile, kFeatureSub) * 16));
+    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)total * 16));
+    HIP_TRY(grow(ws->feat, ws->feat_n, (size_t)total * 16));
+    HIP_TRY(grow(ws->pred, ws->pred_n, (size_t)total * 2));
+    HIP_TRY(launch_tile_features_batch(r.ds, ws->order, total, v16, r.dtype, r.conn, tile,
+                                       kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat, r.s));
+    HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, total, ws->pred, nullptr, r.s));
+    HIP_TRY(hipMemcpyAsync(pred, ws->pred, (size_t)total * 2 * sizeof(float),
+                           hipMemcpyDeviceToHost, r.s));
+    HIP_TRY(hipStreamSynchronize(r.s));
+    int rmax = 0;
+    long long skipped = 0;
+    std::vector<int> R(n);
+    for (int i = 0; i < n; ++i) {
+        const ImgDesc& d = r.h[i];
+        const int ntx = (d.W + tile - 1) / tile;
+        double wsum = 0, wy = 0;
+        for (uint32_t g = base[i]; g < base[i + 1]; ++g) {
+            const int t = (int)(g - base[i]), tx = t % ntx, ty = t / ntx;
+            const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
+                                (double)(std::min(d.H, (ty + 1) * tile) - ty * tile);
+            const float v = pred[(size_t)g * 2];
+            wy += area * (std::isfinite(v) ? std::min(4.f, std::max(0.f, v)) : 0.f);
+            wsum += area;
+        }
+        R[i] = clamp_rounds((float)(wy / wsum));
+        rmax = std::max(rmax, R[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+        r.h[i].rounds = R[i];
+        skipped += (long long)(rmax - R[i]) * (base[i + 1] - base[i]);
+    }
+    HIP_TRY(hipMemcpyAsync(ws->descs, r.h.data(), sizeof(ImgDesc) * n, hipMemcpyHostToDevice,
+                           r.s));
+    HIP_TRY(hipStreamSynchronize(r.s));  // r.h may change before the copy has read it
+    plan.sweep_rounds = rmax;
+    r.st.schedule = IWPP_SCHED_LEARNED;
+    r.st.sweeps_skipped_tiles = (int)std::min<long long>(skipped, 0x7FFFFFFF);
+    return IWPP_OK;
+}
+
+int learned_plan(Run& r, SchedulePlan& plan) {
+    iwpp_workspace* ws = r.ws;
+    bool have = false;
+    int rc0 = ensure_model(ws, &have);
+    if (rc0 || !have) return rc0;  // no model: the plain heuristic
     const ImgDesc& d = r.h[0];
     const int tile = plan.tile;
     const int ntx = (d.W + tile - 1) / tile, nty = (d.H + tile - 1) / tile, nt = ntx * nty;
-    if (ws->model_version != version) {
-        if (!ws->model) HIP_TRY(hipMalloc((void**)&ws->model, kMlpParams * sizeof(uint16_t)));
-        HIP_TRY(hipMemcpyAsync(ws->model, model.data(), kMlpParams * sizeof(uint16_t),
-                               hipMemcpyHostToDevice, r.s));
-        ws->model_version = version;
-    }
+    // pinned staging: predictions in, tile order + starts + masks out
+    rc0 = ensure_host_plan(ws, (size_t)nt * 2 * sizeof(float) + (size_t)(2 * nt + 1) * 4 +
+                                   (size_t)4 * nt + 64);
+    if (rc0) return rc0;
+    float* pred = (float*)ws->host_plan;
+    uint32_t* h_order = (uint32_t*)(pred + (size_t)nt * 2);  // tiles[nt], starts[nt+1]
+    uint8_t* h_masks = (uint8_t*)(h_order + 2 * nt + 1);
     HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
                  (size_t)nt * features_slabs(tile, kFeatureSub) * 16));
     HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)nt * 16));
@@ -511,16 +647,12 @@ int learned_plan(Run& r, SchedulePlan& plan) {
     HIP_TRY(launch_tile_features(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.conn, tile,
                                  kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat, r.s));
     HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, nt, ws->pred, nullptr, r.s));
-    std::vector<float> pred((size_t)nt * 2);
-    HIP_TRY(hipMemcpyAsync(pred.data(), ws->pred, pred.size() * sizeof(float),
+    HIP_TRY(hipMemcpyAsync(pred, ws->pred, (size_t)nt * 2 * sizeof(float),
                            hipMemcpyDeviceToHost, r.s));
     HIP_TRY(hipStreamSynchronize(r.s));
     // The image runs R = round(area-weighted mean of the predicted rounds)
     // rounds; a tile whose own prediction is at least two rounds lower skips
     // the rounds from its prediction on.
-    auto clamp_rounds = [](float v) {
-        return std::isfinite(v) ? (int)std::lround(std::min(4.f, std::max(0.f, v))) : 0;
-    };
     double wsum = 0, wy = 0;
     for (int t = 0; t < nt; ++t) {
         const int tx = t % ntx, ty = t / ntx;
@@ -542,40 +674,43 @@ int learned_plan(Run& r, SchedulePlan& plan) {
     for (int t = 0; t < nt; ++t) skipped += rounds - k[t];
     r.st.sweeps_skipped_tiles = skipped;
     if (nt == 1) return IWPP_OK;  // one tile: uniform rounds, nothing to order
-    std::vector<uint8_t> masks;   // per-round tile masks, only when k varies
-    if (skipped > 0) {
-        masks.resize((size_t)rounds * nt);
-        for (int rr = 0; rr < rounds; ++rr)
-            for (int t = 0; t < nt; ++t) masks[(size_t)rr * nt + t] = k[t] > rr ? 1 : 0;
-    }
-    std::vector<int> tiles(nt);  // seeding order: tiles by decreasing priority
+    // seeding order: tiles by decreasing priority (the kernel expands a tile
+    // into its megapixels)
+    std::vector<int> tiles(nt);
     for (int t = 0; t < nt; ++t) tiles[t] = t;
     std::stable_sort(tiles.begin(), tiles.end(), [&](int a, int b) {
         return pred[(size_t)a * 2 + 1] > pred[(size_t)b * 2 + 1];
     });
-    std::vector<uint32_t> order;
-    order.reserve((size_t)d.nmx * d.nmy);
     const int mpt = tile / kMP;  // megapixels per tile side
-    for (int t : tiles) {
-        const int tx = t % ntx, ty = t / ntx;
-        for (int my = ty * mpt; my < std::min(d.nmy, (ty + 1) * mpt); ++my)
-            for (int mx = tx * mpt; mx < std::min(d.nmx, (tx + 1) * mpt); ++mx)
-                order.push_back((uint32_t)(my * d.nmx + mx));
+    uint32_t pos = 0;
+    for (int rk = 0; rk < nt; ++rk) {
+        const int t = tiles[rk], tx = t % ntx, ty = t / ntx;
+        h_order[rk] = (uint32_t)t;
+        h_order[nt + rk] = pos;
+        pos += (uint32_t)(std::min(mpt, d.nmx - tx * mpt) * std::min(mpt, d.nmy - ty * mpt));
     }
-    HIP_TRY(grow(ws->order, ws->order_n, order.size()));
-    HIP_TRY(hipMemcpyAsync(ws->order, order.data(), order.size() * 4, hipMemcpyHostToDevice,
+    h_order[2 * nt] = pos;
+    HIP_TRY(grow(ws->order, ws->order_n, (size_t)2 * nt + 1));
+    HIP_TRY(hipMemcpyAsync(ws->order, h_order, ((size_t)2 * nt + 1) * 4, hipMemcpyHostToDevice,
                            r.s));
-    if (!masks.empty()) {
-        HIP_TRY(grow(ws->tiles, ws->tiles_n, masks.size()));
-        HIP_TRY(hipMemcpyAsync(ws->tiles, masks.data(), masks.size(), hipMemcpyHostToDevice,
-                               r.s));
+    r.order_tiles = ws->order;
+    r.order_start = ws->order + nt;
+    r.order_nt = nt;
+    r.order_ntx = ntx;
+    r.order_mpt = mpt;
+    if (skipped > 0) {  // per-round tile masks, only when some tile stops early
+        const size_t nm = (size_t)rounds * nt;
+        uint8_t* mk = nm <= (size_t)4 * nt ? h_masks : nullptr;
+        if (!mk) return fail(IWPP_ERR_ARG, "internal: mask staging too small");
+        for (int rr = 0; rr < rounds; ++rr)
+            for (int t = 0; t < nt; ++t) mk[(size_t)rr * nt + t] = k[t] > rr ? 1 : 0;
+        HIP_TRY(grow(ws->tiles, ws->tiles_n, nm));
+        HIP_TRY(hipMemcpyAsync(ws->tiles, mk, nm, hipMemcpyHostToDevice, r.s));
         r.round_masks = ws->tiles;
         r.sched_tile = tile;
         r.sched_ntx = ntx;
         r.sched_nt = nt;
     }
-    HIP_TRY(hipStreamSynchronize(r.s));  // the host vectors go out of scope
-    r.order = ws->order;
     return IWPP_OK;
 }
 
@@ -611,8 +746,8 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
         return fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
     rc = setup_images(r, imgs, n);
     if (rc) return rc;
-    if (schedule == IWPP_SCHED_LEARNED && n == 1 && !r.graph) {
-        rc = learned_plan(r, plan);
+    if (schedule == IWPP_SCHED_LEARNED && !r.graph) {
+        rc = n == 1 ? learned_plan(r, plan) : learned_plan_batch(r, plan);
         if (rc) return rc;
     }
     if (r.graph) {
@@ -655,7 +790,7 @@ extern "C" int iwpp_reconstruct_ex(void* marker, const void* mask, size_t pm, si
     r.dtype = dtype;
     r.conn = conn;
     r.ws = ws;
-    r.s = (hipStream_t)stream;
+    HIP_TRY(call_stream(ws, stream, &r.s));
     r.timed = stats != nullptr;
     const iwpp_image img{marker, mask, pm, pk, w, h};
     rc = reconstruct_run(r, &img, 1, tile, queue, schedule, opt);
@@ -690,7 +825,7 @@ extern "C" int iwpp_reconstruct_batch(const iwpp_image* imgs, int n, int dtype, 
     r.dtype = dtype;
     r.conn = conn;
     r.ws = ws;
-    r.s = (hipStream_t)stream;
+    HIP_TRY(call_stream(ws, stream, &r.s));
     r.timed = stats != nullptr;
     rc = reconstruct_run(r, imgs, n, tile, queue, schedule, opt);
     if (rc) return rc;
@@ -842,7 +977,8 @@ extern "C" int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int conn,
     }
     if (!ws) ws = default_ws();
     if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = nullptr;
+    HIP_TRY(call_stream(ws, stream, &s));
     // snapshot of the input markers (packed rows), restored before every trial
     uint8_t* snap = nullptr;
     HIP_TRY(hipMalloc((void**)&snap, std::max<size_t>(snap_bytes, 1)));

@@ -71,6 +71,8 @@ struct ImgDesc {
     int cw_base;         // first column-pass wave
     int sw_base;         // first seeding wave
     TileMask tm;         // walls for the sweep passes
+    int rounds = 0x7FFFFFFF;  // sweep rounds this image takes (a batch under the
+                              // learned schedule); later rounds skip it
 };
 
 // Images of a launch: a device array, or (d == nullptr) the single image `one`
@@ -81,6 +83,7 @@ struct Descs {
     const uint32_t* img_of_mp;
     int n;
     int total;           // work items of the launch (kernel specific)
+    int round = 0;       // sweep round of the launch (0-based)
     ImgDesc one;
 };
 
@@ -96,7 +99,12 @@ struct QueueArgs {
     int max_rounds;     // bsp round limit
     unsigned long long timeout_ticks;  // wall-clock ticks a wait may last
     int trace;          // accumulate the phase timers
-    const uint32_t* order;  // seeding order of the megapixels (nullptr = index order)
+    // Seeding order by tiles (nullptr = megapixel index order): rank r takes
+    // tile order_tiles[r]; its megapixels occupy queue positions
+    // order_start[r] .. order_start[r+1]-1, row by row.  One image only.
+    const uint32_t* order_tiles;
+    const uint32_t* order_start;
+    int order_nt, order_ntx, order_mpt, order_nmx, order_nmy;
 };
 
 enum QueueKind : int { Q_MUTEX = 0, Q_DQ = 1, Q_UQ = 2 };

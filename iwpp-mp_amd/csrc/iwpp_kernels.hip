@@ -535,12 +535,44 @@ __global__ __launch_bounds__(kWG) void k_row_pass_long(Image im, TileMask tm, ui
 }
 
 // The same over the rows of several images (global row index -> image).
+// Rows of images whose sweep rounds are done are skipped; in the first round
+// such rows are only clipped (marker = min(marker, mask)).
+template <class T, int VB>
+DEV void row_clip(const Image& im, int y, bool& any_change) {
+    using V = VecIO<T, VB>;
+    const int rowbytes = im.W * (int)sizeof(T);
+    uint8_t* jr = im.J + (size_t)y * im.pj;
+    const uint8_t* ir = im.I + (size_t)y * im.pi;
+    for (int o = threadIdx.x * VB; o < rowbytes; o += kWG * VB) {
+        uint32_t jw[V::NW], iw[V::NW], a[V::NPX], b[V::NPX];
+        V::load(jr, o, rowbytes, jw);
+        V::load(ir, o, rowbytes, iw);
+        V::unpack(jw, a);
+        V::unpack(iw, b);
+        bool ch = false;
+#pragma unroll
+        for (int k = 0; k < V::NPX; ++k) {
+            ch |= a[k] > b[k];
+            a[k] = min(a[k], b[k]);
+        }
+        if (ch) {
+            V::pack(a, jw);
+            V::store(jr, o, rowbytes, jw);
+            any_change = true;
+        }
+    }
+}
+
 template <class T, int VB, bool CLIP>
 __global__ __launch_bounds__(kWG) void k_row_pass_batch(Descs ds, uint32_t* changed) {
     __shared__ uint32_t sP[2][4], sQ[2][4];
     bool any_change = false;
     for (int r = blockIdx.x; r < ds.total; r += gridDim.x) {
         const ImgDesc d = desc(ds, find_row(ds, r));
+        if (ds.round >= d.rounds) {  // the same for the whole workgroup
+            if (CLIP) row_clip<T, VB>(image_of(d), r - d.row_base, any_change);
+            continue;
+        }
         row_stream<T, VB, 0, CLIP>(image_of(d), r - d.row_base, d.tm, sP, sQ, any_change);
     }
     if (any_change && changed) *changed = 1u;
@@ -657,6 +689,7 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ds.total) return;
     const ImgDesc d = desc(ds, find_cw(ds, w));
+    if (ds.round >= d.rounds) return;  // this image's sweep rounds are done
     const int local = w - d.cw_base;
     const int strip = local % d.strips, seg = local / d.strips;
     if (JP) col_block<T, VB, CONN, DIR>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
@@ -1134,7 +1167,20 @@ __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nm
     for (int base = (blockIdx.x * kWG + threadIdx.x) & ~63; base < nmp;
          base += gridDim.x * kWG) {
         const int pos_in = base + lane;
-        const int m = pos_in < nmp ? (q.order ? (int)q.order[pos_in] : pos_in) : nmp;
+        int m = pos_in < nmp ? pos_in : nmp;
+        if (q.order_tiles && pos_in < nmp) {  // tile-ordered seeding
+            int lo = 0, hi = q.order_nt - 1;  // last rank with start <= pos
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int)q.order_start[mid] <= pos_in) lo = mid;
+                else hi = mid - 1;
+            }
+            const int t = (int)q.order_tiles[lo];
+            const int local = pos_in - (int)q.order_start[lo];
+            const int tx = t % q.order_ntx, ty = t / q.order_ntx;
+            const int wmp = min(q.order_mpt, q.order_nmx - tx * q.order_mpt);
+            m = (ty * q.order_mpt + local / wmp) * q.order_nmx + tx * q.order_mpt + local % wmp;
+        }
         const bool f = m < nmp && flags[m] != 0;
         const uint64_t b = ballot(f);
         const uint32_t n = (uint32_t)__popcll(b);
@@ -1635,8 +1681,11 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
     else if (kind == Q_DQ) fn = (const void*)k_queue_bsp<T, CONN, Q_DQ>;
     else fn = (const void*)k_queue_bsp<T, CONN, Q_UQ>;
     int per_cu = 0;
-    const size_t lds_bytes = 4 * WAVE_LDS;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWG, lds_bytes);
+    const size_t lds_bytes = 4 * WAVE_LDS;  // 70 KiB: beyond the 64 KiB default, so say so
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWG, lds_bytes);
     if (e != hipSuccess) return e;
     if (per_cu <= 0) return hipErrorInvalidConfiguration;
     if (wpc > 0) per_cu = std::min(per_cu, wpc);
@@ -1649,7 +1698,19 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
         return hipGetLastError();
     }
     void* args[] = {&dc, &qc};
-    return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, lds_bytes, s);
+    e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, lds_bytes, s);
+    if (e == hipSuccess) return e;
+    // The runtime refuses cooperative launches in some states (for instance
+    // while another process holds the device).  The grid is sized to fit the
+    // device at once, and every device-side wait of the kernel is bounded in
+    // time, so an ordinary launch either runs co-resident as intended or ends
+    // with the timeout flag set -- it cannot hang.
+    (void)hipGetLastError();
+    if (kind == Q_DQ)
+        hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_DQ>), grid, kWG, lds_bytes, s, dc, qc);
+    else
+        hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_UQ>), grid, kWG, lds_bytes, s, dc, qc);
+    return hipGetLastError();
 }
 
 hipError_t launch_queue_phase(const Descs& ds, int dtype, int conn, int kind,

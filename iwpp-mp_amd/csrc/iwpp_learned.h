@@ -28,6 +28,11 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 int features_slabs(int tile, int sub);
 hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, int sub,
                                 float* part, float* raw, float* feat, hipStream_t s);
+// The same over several images (ds: device descriptor table); image i owns
+// the ntiles-wide numbering's tiles tile_base[i] .. tile_base[i+1]-1 (device).
+hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base, int ntiles,
+                                      bool v16, int dtype, int conn, int tile, int sub,
+                                      float* part, float* raw, float* feat, hipStream_t s);
 // y: n x 2; acts (optional): n x 128 hidden activations for training.
 hipError_t launch_mlp_forward(const void* params, bool bf16, const float* x, int n, float* y,
                               float* acts, hipStream_t s);

@@ -37,13 +37,32 @@ DEV float wave_reduce(float v, bool mx) {
     return v;
 }
 
+// Tiles of several images are numbered consecutively: image i owns global
+// tiles tile_base[i] .. tile_base[i+1]-1 (tile_base == nullptr: one image).
+DEV int image_of_tile(const uint32_t* tile_base, int n, int gt) {
+    if (!tile_base) return 0;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)tile_base[mid] <= gt) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <class T, int VB>
-__global__ __launch_bounds__(256) void k_tile_features_part(Image im, int tile, int ntx, int sub,
-                                                             int nslab, float* part) {
+__global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint32_t* tile_base,
+                                                             int tile, int sub, int nslab,
+                                                             float* part) {
     constexpr int NPX = VB / (int)sizeof(T), PPW = 4 / (int)sizeof(T);
     constexpr int BITS = 8 * (int)sizeof(T);
     constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
-    const int t = blockIdx.x / nslab, sl = blockIdx.x % nslab;
+    const int gt = blockIdx.x / nslab, sl = blockIdx.x % nslab;
+    const int img = image_of_tile(tile_base, ds.n, gt);
+    const ImgDesc dd = ds.d ? ds.d[img] : ds.one;
+    const Image im{dd.J, dd.I, dd.pj, dd.pi, dd.W, dd.H};
+    const int ntx = (im.W + tile - 1) / tile;
+    const int t = gt - (tile_base ? (int)tile_base[img] : 0);
     const int tx = t % ntx, ty = t / ntx;
     const int x0 = tx * tile, y0 = ty * tile;
     const int x1 = min(im.W, x0 + tile), y1 = min(im.H, y0 + tile);
@@ -137,7 +156,7 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Image im, int tile, 
         const bool mx = is_max_stat(k);
         float v = red[k][0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = mx ? fmaxf(v, red[k][w]) : v + red[k][w];
-        part[((size_t)t * nslab + sl) * kStats + k] = v;
+        part[((size_t)gt * nslab + sl) * kStats + k] = v;
     }
 }
 
@@ -182,12 +201,16 @@ DEV void normalise(const float* r, float M, float tile_px, int tile, int dtype, 
     f[15] = r[14] / fmaxf(tile_px, 1.f);
 }
 
-__global__ __launch_bounds__(256) void k_features_normalise(const float* raw, int ntiles,
-                                                             int tile, int sub, int dtype,
-                                                             int conn, float* feat) {
+// One workgroup per image: M = the image's largest mask value, then its tiles.
+__global__ __launch_bounds__(256) void k_features_normalise(const float* raw,
+                                                             const uint32_t* tile_base,
+                                                             int ntiles, int tile, int sub,
+                                                             int dtype, int conn, float* feat) {
     __shared__ float smax[256];
+    const int t0 = tile_base ? (int)tile_base[blockIdx.x] : 0;
+    const int t1 = tile_base ? (int)tile_base[blockIdx.x + 1] : ntiles;
     float m = 0.f;
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x) m = fmaxf(m, raw[(size_t)t * 16 + 15]);
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) m = fmaxf(m, raw[(size_t)t * 16 + 15]);
     smax[threadIdx.x] = m;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -196,7 +219,7 @@ __global__ __launch_bounds__(256) void k_features_normalise(const float* raw, in
     }
     const float M = smax[0];
     const float tile_px = (float)tile * (float)((tile + sub - 1) / sub);
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
         normalise(raw + (size_t)t * 16, M, tile_px, tile, dtype, conn, feat + (size_t)t * 16);
 }
 
@@ -350,25 +373,41 @@ __global__ void k_to_bf16(const float* w, uint16_t* wbf, int n) {
 // ------------------------------------------------------------ launchers ----
 int features_slabs(int tile, int sub) { return ((tile + sub - 1) / sub + kSlab - 1) / kSlab; }
 
+hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base, int ntiles,
+                                      bool v16, int dtype, int conn, int tile, int sub,
+                                      float* part, float* raw, float* feat, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    const int nslab = features_slabs(tile, sub);
+    const int grid = ntiles * nslab;
+    if (dtype == 1) {
+        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint8_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+        else hipLaunchKernelGGL((k_tile_features_part<uint8_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+    } else {
+        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint16_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+        else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+    }
+    hipLaunchKernelGGL(k_tile_features_combine, (ntiles * kStats + 255) / 256, 256, 0, s, part,
+                       ntiles, nslab, raw);
+    hipLaunchKernelGGL(k_features_normalise, tile_base ? ds.n : 1, 256, 0, s, raw, tile_base,
+                       ntiles, tile, sub, dtype, conn, feat);
+    return hipGetLastError();
+}
+
 hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, int sub,
                                 float* part, float* raw, float* feat, hipStream_t s) {
-    const int ntx = (im.W + tile - 1) / tile, nty = (im.H + tile - 1) / tile;
-    const int nt = ntx * nty, nslab = features_slabs(tile, sub);
+    const int nt = ((im.W + tile - 1) / tile) * ((im.H + tile - 1) / tile);
+    Descs ds{};
+    ds.n = 1;
+    ds.one.J = im.J;
+    ds.one.I = im.I;
+    ds.one.pj = im.pj;
+    ds.one.pi = im.pi;
+    ds.one.W = im.W;
+    ds.one.H = im.H;
     const bool v16 = (uintptr_t)im.J % 16 == 0 && (uintptr_t)im.I % 16 == 0 && im.pj % 16 == 0 &&
                      im.pi % 16 == 0;
-    const int grid = nt * nslab;
-    if (dtype == 1) {
-        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint8_t, 16>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
-        else hipLaunchKernelGGL((k_tile_features_part<uint8_t, 4>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
-    } else {
-        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint16_t, 16>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
-        else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, im, tile, ntx, sub, nslab, part);
-    }
-    hipLaunchKernelGGL(k_tile_features_combine, (nt * kStats + 255) / 256, 256, 0, s, part, nt,
-                       nslab, raw);
-    hipLaunchKernelGGL(k_features_normalise, std::max(1, (nt + 255) / 256), 256, 0, s, raw, nt,
-                       tile, sub, dtype, conn, feat);
-    return hipGetLastError();
+    return launch_tile_features_batch(ds, nullptr, nt, v16, dtype, conn, tile, sub, part, raw,
+                                      feat, s);
 }
 
 hipError_t launch_mlp_forward(const void* params, bool bf16, const float* x, int n, float* y,

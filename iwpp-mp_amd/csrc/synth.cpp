@@ -243,8 +243,10 @@ void seeds_t(void* marker, const void* mask, size_t pj, size_t pi, int W, int H,
     }
 }
 
+// uint16 images need 2-byte aligned rows (pixels are accessed as uint16_t).
 bool bad_image(const void* a, size_t pitch, int W, int H, int dtype) {
-    return !a || W <= 0 || H <= 0 || (dtype != 1 && dtype != 2) || pitch < (size_t)W * dtype;
+    return !a || W <= 0 || H <= 0 || (dtype != 1 && dtype != 2) || pitch < (size_t)W * dtype ||
+           pitch % dtype || (uintptr_t)a % dtype;
 }
 
 }  // namespace

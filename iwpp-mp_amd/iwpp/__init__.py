@@ -101,6 +101,14 @@ SYNTH_SIGNATURES = {
     "iwpp_synth_random_case": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, ctypes.c_uint64]),
 }
 
+# include/iwpp_io.h (PGM files), exported by the same host library.
+IO_SIGNATURES = {
+    "iwpp_pgm_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_VP), ctypes.POINTER(_I),
+                           ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "iwpp_pgm_write": (_I, [ctypes.c_char_p, _VP, _SZ, _I, _I, _I]),
+    "iwpp_pgm_free": (None, [_VP]),
+}
+
 _LIB = None
 _SYNTH = None
 
@@ -137,7 +145,7 @@ def lib():
 def synth_lib():
     global _SYNTH
     if _SYNTH is None:
-        _SYNTH = _bind(SYNTH_PATH, SYNTH_SIGNATURES)
+        _SYNTH = _bind(SYNTH_PATH, {**SYNTH_SIGNATURES, **IO_SIGNATURES})
     return _SYNTH
 
 
@@ -507,3 +515,39 @@ def synth_random_case(height, width, seed):
     if rc:
         raise ValueError("iwpp_synth_random_case failed")
     return j, i
+
+
+# ------------------------------------------------------------ PGM files -----
+class PGMError(IOError):
+    pass
+
+
+def pgm_read(path):
+    """numpy array (uint8 when maxval <= 255, else uint16) and maxval."""
+    L = synth_lib()
+    data, w, h, mv = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = L.iwpp_pgm_read(os.fsencode(path), ctypes.byref(data), ctypes.byref(w),
+                         ctypes.byref(h), ctypes.byref(mv))
+    if rc != 0:
+        raise PGMError(rc, f"iwpp_pgm_read({path}) failed")
+    try:
+        dt = np.uint8 if mv.value <= 255 else np.uint16
+        n = w.value * h.value * np.dtype(dt).itemsize
+        buf = (ctypes.c_uint8 * n).from_address(data.value)
+        out = np.frombuffer(buf, dtype=dt).reshape(h.value, w.value).copy()
+    finally:
+        L.iwpp_pgm_free(data)
+    return out, mv.value
+
+
+def pgm_write(path, a, maxval=None):
+    a = np.ascontiguousarray(a)
+    if a.dtype not in (np.uint8, np.uint16) or a.ndim != 2:
+        raise TypeError("2-D uint8 or uint16 array")
+    mv = maxval if maxval is not None else (255 if a.dtype == np.uint8 else 65535)
+    if (a.dtype == np.uint8) != (mv <= 255):
+        raise ValueError("maxval does not match the array type")
+    rc = synth_lib().iwpp_pgm_write(os.fsencode(path), a.ctypes.data, a.strides[0], a.shape[1],
+                                    a.shape[0], mv)
+    if rc != 0:
+        raise PGMError(rc, f"iwpp_pgm_write({path}) failed")

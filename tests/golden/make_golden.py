@@ -20,7 +20,14 @@ from PIL import Image
 from scipy import ndimage
 
 REF = "/root/reference/IWPP MP/Imagens_de_Teste"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "jpeg_crops.npz")
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "jpeg_crops.npz")
+# A 2048 x 2048 window of the decoded pair (the full 4096^2 pair compresses to
+# 8.5 MB; this window to about 2 MB) and the SHA-256 of its reconstruction by
+# the oracle, for the regression over every queue variant and tile size (the
+# counterpart of IWPP MP/test_script.sh, without the external verifier).
+PAIR = os.path.join(HERE, "pair50.npz")
+PAIR_WINDOW = (1024, 1024, 2048, 2048)  # row, col, height, width
 
 # (row, col, height, width) crops, one of them ragged.
 CROPS = [(0, 0, 160, 160), (1024, 2048, 160, 160), (3936, 3936, 160, 160),
@@ -55,6 +62,18 @@ def main():
             out[f"crop{k}_out{conn}"] = brute_force(jm, im, conn)
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, {k: v.shape for k, v in out.items() if hasattr(v, "shape")})
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    import oracle
+    r, c, h, w = PAIR_WINDOW
+    pm = np.ascontiguousarray(marker[r:r + h, c:c + w])
+    pk = np.ascontiguousarray(mask[r:r + h, c:c + w])
+    pair = {"marker": pm, "mask": pk, "window": np.array(PAIR_WINDOW)}
+    for conn in (4, 8):
+        res = oracle.reconstruct(pm, pk, conn)
+        pair[f"out{conn}_sha256"] = np.array(hashlib.sha256(res.tobytes()).hexdigest())
+    np.savez_compressed(PAIR, **pair)
+    print("wrote", PAIR, {k: str(v) if v.ndim == 0 else v.shape for k, v in pair.items()})
 
 
 if __name__ == "__main__":

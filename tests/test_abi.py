@@ -33,6 +33,14 @@ def test_synth_exports_every_declared_symbol():
     assert names == set(iwpp.SYNTH_SIGNATURES)
 
 
+def test_io_exports_every_declared_symbol():
+    names = declared("iwpp_io.h")
+    L = ctypes.CDLL(iwpp.SYNTH_PATH)
+    for n in names:
+        assert hasattr(L, n), n
+    assert names == set(iwpp.IO_SIGNATURES)
+
+
 def test_library_carries_gfx950_code():
     data = open(iwpp.LIB_PATH, "rb").read()
     assert b"gfx950" in data

@@ -224,3 +224,25 @@ def test_committed_model_exact_on_scaled_workloads(torch_cuda, model_guard):
                               schedule="learned")
         assert st["schedule"] == iwpp.SCHEDULES["learned"]
         assert np.array_equal(J.numpy(), want)
+
+
+@pytest.mark.parametrize("queue", ["mutex", "dq", "uq"])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_learned_batch_exact(torch_cuda, model_guard, queue, dtype):
+    # per-image round counts (0..4 across images under the random models,
+    # with 0 meaning clip only) must leave every image exact
+    rng = np.random.default_rng(zlib.crc32(f"batch/{queue}/{dtype.__name__}".encode()))
+    shapes = [(1, 1), (1, 300), (300, 1), (64, 64), (65, 129), (200, 333), (640, 257),
+              (77, 1500), (1029, 70), (512, 512)]
+    for mi, (name, kb) in enumerate(MODELS):
+        iwpp.set_model(_model_with_bias(mi + 11, kb))
+        cases = [rand_case(rng, h, w, dtype, "seeds" if k % 2 else "hdome")
+                 for k, (h, w) in enumerate(shapes)]
+        for conn in (4, 8):
+            say("learned batch", name, queue, dtype.__name__, conn)
+            pairs = [(iwpp.DeviceImage.from_numpy(J), iwpp.DeviceImage.from_numpy(I))
+                     for J, I in cases]
+            st = iwpp.reconstruct_batch(pairs, conn, 128, queue, "learned")
+            assert st["schedule"] == iwpp.SCHEDULES["learned"]
+            for (J, I), (dj, _) in zip(cases, pairs):
+                assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn)), (name, J.shape)

@@ -217,3 +217,12 @@ def test_planted_mistake_is_caught():
     wrong = oracle.reconstruct(marker, mask, 4)
     assert not np.array_equal(wrong, brute_force(marker, mask, 8))
     assert np.array_equal(oracle.reconstruct(marker, mask, 8), brute_force(marker, mask, 8))
+
+
+def test_pair50_window_hashes():
+    # the committed hashes of the regression window are the oracle's output
+    import hashlib
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pair50.npz"))
+    for conn in (4, 8):
+        out = oracle.reconstruct(d["marker"], d["mask"], conn)
+        assert hashlib.sha256(out.tobytes()).hexdigest() == str(d[f"out{conn}_sha256"])
