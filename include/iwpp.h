@@ -190,7 +190,7 @@ void iwpp_autotune_clear(void);
 int iwpp_sched_set_model(const float* weights, int n);
 
 /* Tile features of one image (device pointers), as a reconstruction with
- * IWPP_SCHED_LEARNED computes them (there with sub = 4): raw[t][16]
+ * IWPP_SCHED_LEARNED computes them (there with sub = 8): raw[t][16]
  * statistics over every `sub`-th row and feat[t][16] the normalised features
  * the model reads, t over ceil(h/tile) x ceil(w/tile) tiles in row-major
  * order.  Scratch comes from ws (NULL: the device's default workspace). */

@@ -229,6 +229,7 @@ struct Run {
     std::vector<ImgDesc> h;  // host copy of the descriptors
     Descs ds{};              // what the kernels get (totals set per launch)
     bool wide = false;       // 16-byte lanes in the column / seeding passes
+    bool row16 = false;      // 16-byte lanes in the batched row pass
     int single_row_pass = 0; // one image: use the register row kernels
     int total_rows = 0, total_cw = 0, total_sw = 0, total_mp = 0;
     QueueArgs q{};
@@ -275,6 +276,8 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
     HIP_TRY(grow(ws->plane, ws->plane_n, plane_bytes + 256));
     uint8_t* plane0 = (uint8_t*)round_up((uintptr_t)ws->plane, 256);
     r.wide = true;
+    bool aligned = true;  // every plane 16-byte aligned (base and pitch)
+    double useful = 0, lanes16 = 0, chunks16 = 0;
     long strip_rows = 0;
     for (int i = 0; i < n; ++i) {
         ImgDesc& d = r.h[i];
@@ -291,7 +294,18 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         d.nmy = (d.H + kMP - 1) / kMP;
         d.tm = TileMask{nullptr, kMP, kMP, 1};
         r.wide = r.wide && col_wide(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, r.dtype);
+        aligned = aligned && (uintptr_t)d.J % 16 == 0 && (uintptr_t)d.I % 16 == 0 &&
+                  (uintptr_t)d.P % 16 == 0 && d.pj % 16 == 0 && d.pi % 16 == 0 && d.pp % 16 == 0;
+        const double rb = (double)d.W * r.dtype;
+        useful += rb * d.H;
+        lanes16 += std::ceil(rb / 1024.0) * 1024.0 * d.H;  // 64 lanes x 16 B per column strip
+        chunks16 += std::ceil(rb / 4096.0) * 4096.0 * d.H;  // 256 lanes x 16 B per row chunk
     }
+    // A batch of narrower images still has plenty of column strips: use the
+    // 16-byte lanes whenever they are at least 3/4 busy (and likewise for the
+    // row chunks of the batched row pass).
+    r.row16 = n > 1 && aligned && useful >= 0.75 * chunks16;
+    if (n > 1) r.wide = aligned && useful >= 0.75 * lanes16;
     for (int i = 0; i < n; ++i) {
         r.h[i].strips = col_strips(r.h[i].W, r.dtype, r.wide);
         strip_rows += (long)r.h[i].strips * r.h[i].H;
@@ -327,22 +341,36 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         r.ds.one = r.h[0];
         return IWPP_OK;
     }
-    // device tables: descriptors and megapixel -> image
-    std::vector<uint32_t> img_of_mp((size_t)mp);
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < r.h[i].nmx * r.h[i].nmy; ++k) img_of_mp[r.h[i].mp_base + k] = (uint32_t)i;
+    // device tables: descriptors, and work item -> image for megapixels,
+    // rows, column-block waves and seeding waves (one allocation)
+    const size_t ntab = (size_t)mp + rows + cw + sw;
+    std::vector<uint32_t> tab(ntab);
+    uint32_t* t_mp = tab.data();
+    uint32_t* t_row = t_mp + mp;
+    uint32_t* t_cw = t_row + rows;
+    uint32_t* t_sw = t_cw + cw;
+    for (int i = 0; i < n; ++i) {
+        const ImgDesc& d = r.h[i];
+        std::fill(t_mp + d.mp_base, t_mp + d.mp_base + (size_t)d.nmx * d.nmy, (uint32_t)i);
+        std::fill(t_row + d.row_base, t_row + d.row_base + d.H, (uint32_t)i);
+        std::fill(t_cw + d.cw_base, t_cw + d.cw_base + (size_t)d.strips * ((d.H + d.R - 1) / d.R),
+                  (uint32_t)i);
+        std::fill(t_sw + d.sw_base, t_sw + d.sw_base + (size_t)d.strips * d.nmy, (uint32_t)i);
+    }
     if (!r.capturing) {  // a capture reuses the tables its dry run uploaded
         HIP_TRY(grow(ws->descs, ws->descs_n, (size_t)n));
-        HIP_TRY(grow(ws->img_of_mp, ws->img_of_mp_n, (size_t)mp));
+        HIP_TRY(grow(ws->img_of_mp, ws->img_of_mp_n, ntab));
         HIP_TRY(hipMemcpyAsync(ws->descs, r.h.data(), sizeof(ImgDesc) * n,
                                hipMemcpyHostToDevice, r.s));
-        HIP_TRY(hipMemcpyAsync(ws->img_of_mp, img_of_mp.data(), 4 * (size_t)mp,
-                               hipMemcpyHostToDevice, r.s));
+        HIP_TRY(hipMemcpyAsync(ws->img_of_mp, tab.data(), 4 * ntab, hipMemcpyHostToDevice, r.s));
         // the host vectors die with this call: make the copies complete first
         HIP_TRY(hipStreamSynchronize(r.s));
     }
     r.ds.d = ws->descs;
     r.ds.img_of_mp = ws->img_of_mp;
+    r.ds.img_of_row = ws->img_of_mp + mp;
+    r.ds.img_of_cw = ws->img_of_mp + mp + rows;
+    r.ds.img_of_sw = ws->img_of_mp + mp + rows + cw;
     return IWPP_OK;
 }
 
@@ -360,7 +388,7 @@ int sweep_phase(Run& r, int rounds) {
             HIP_TRY(launch_clip(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.s));
         } else {  // a round index past every image's rounds: rows are only clipped
             HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, 0x7FFFFFFF), r.dtype,
-                                          true, nullptr, r.s));
+                                          true, r.row16, nullptr, r.s));
         }
         return IWPP_OK;
     }
@@ -380,7 +408,7 @@ int sweep_phase(Run& r, int rounds) {
             HIP_TRY(launch_row_pass(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, k == 0, d.tm,
                                     nullptr, r.s));
         } else {
-            HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0,
+            HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0, r.row16,
                                           nullptr, r.s));
         }
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1, true,

@@ -81,6 +81,11 @@ struct ImgDesc {
 struct Descs {
     const ImgDesc* d;
     const uint32_t* img_of_mp;
+    // work item -> image tables (several images only): global row, column
+    // block wave, seeding wave
+    const uint32_t* img_of_row;
+    const uint32_t* img_of_cw;
+    const uint32_t* img_of_sw;
     int n;
     int total;           // work items of the launch (kernel specific)
     int round = 0;       // sweep round of the launch (0-based)
@@ -114,8 +119,8 @@ hipError_t launch_clip(const Image& im, int dtype, hipStream_t s);
 hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask& tm,
                            uint32_t* changed, hipStream_t s);
 // Streaming row pass over the rows of every image (ds.total = all rows).
-hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, uint32_t* changed,
-                                 hipStream_t s);
+hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wide,
+                                 uint32_t* changed, hipStream_t s);
 // Column pass over every image's column blocks (ds.total = all waves).
 // j_to_p: read J write P (else read P write J); dir +1 down, -1 up; wide =
 // 16-byte lanes (every image 16-byte aligned), else 4-byte lanes.

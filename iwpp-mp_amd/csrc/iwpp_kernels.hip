@@ -246,10 +246,15 @@ DEV ImgDesc desc(const Descs& a, int i) {
         }                                                                \
         return lo;                                                       \
     }
-IWPP_FIND(find_row, row_base)
-IWPP_FIND(find_cw, cw_base)
-IWPP_FIND(find_sw, sw_base)
+IWPP_FIND(search_row, row_base)
+IWPP_FIND(search_cw, cw_base)
+IWPP_FIND(search_sw, sw_base)
 #undef IWPP_FIND
+// One load from the work-item table when there is one (a dependent chain of
+// loads per work item otherwise: measured as the main cost of batched passes).
+DEV int find_row(const Descs& a, int r) { return a.img_of_row ? (int)a.img_of_row[r] : search_row(a, r); }
+DEV int find_cw(const Descs& a, int w) { return a.img_of_cw ? (int)a.img_of_cw[w] : search_cw(a, w); }
+DEV int find_sw(const Descs& a, int w) { return a.img_of_sw ? (int)a.img_of_sw[w] : search_sw(a, w); }
 DEV int find_mp(const Descs& a, int g) { return a.img_of_mp ? (int)a.img_of_mp[g] : 0; }
 DEV Image image_of(const ImgDesc& d) { return Image{d.J, d.I, d.pj, d.pi, d.W, d.H}; }
 
@@ -1587,16 +1592,22 @@ int col_auto_rows(int W, int H, int dtype, bool wide) {
     return (int)std::min<long>(512, std::max<long>(32, r));
 }
 
-hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, uint32_t* changed,
-                                 hipStream_t s) {
-    if (ds.total <= 0) return hipSuccess;
+template <class T, int VB>
+static void row_pass_batch_t(const Descs& ds, bool clip, uint32_t* changed, hipStream_t s) {
     const int grid = std::min(ds.total, 65535);
+    if (clip) hipLaunchKernelGGL((k_row_pass_batch<T, VB, true>), grid, kWG, 0, s, ds, changed);
+    else hipLaunchKernelGGL((k_row_pass_batch<T, VB, false>), grid, kWG, 0, s, ds, changed);
+}
+
+hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wide,
+                                 uint32_t* changed, hipStream_t s) {
+    if (ds.total <= 0) return hipSuccess;
     if (dtype == 1) {
-        if (clip) hipLaunchKernelGGL((k_row_pass_batch<uint8_t, 4, true>), grid, kWG, 0, s, ds, changed);
-        else hipLaunchKernelGGL((k_row_pass_batch<uint8_t, 4, false>), grid, kWG, 0, s, ds, changed);
+        if (wide) row_pass_batch_t<uint8_t, 16>(ds, clip, changed, s);
+        else row_pass_batch_t<uint8_t, 4>(ds, clip, changed, s);
     } else {
-        if (clip) hipLaunchKernelGGL((k_row_pass_batch<uint16_t, 4, true>), grid, kWG, 0, s, ds, changed);
-        else hipLaunchKernelGGL((k_row_pass_batch<uint16_t, 4, false>), grid, kWG, 0, s, ds, changed);
+        if (wide) row_pass_batch_t<uint16_t, 16>(ds, clip, changed, s);
+        else row_pass_batch_t<uint16_t, 4>(ds, clip, changed, s);
     }
     return hipGetLastError();
 }

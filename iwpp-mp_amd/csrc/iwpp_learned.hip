@@ -21,7 +21,8 @@ namespace iwpp {
 //  of max(0, min(J(q), I(p)) - J(p)) with q the pixel across the border,
 //  14 pixels sampled, 15 max I.
 // Work split: one workgroup per (tile, slab of kSlab sampled rows); partial
-// statistics go to part[tile][slab][16] and a second kernel combines them.
+// statistics go to part[tile][slab][16]; a second kernel combines and
+// normalises them per image.
 // Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
 constexpr int kSlab = 8;
 constexpr int kStats = 16;
@@ -160,21 +161,6 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
     }
 }
 
-// raw[t][k] from the slab partials (one thread per tile and statistic).
-__global__ __launch_bounds__(256) void k_tile_features_combine(const float* part, int ntiles,
-                                                                int nslab, float* raw) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntiles * kStats) return;
-    const int t = i / kStats, k = i % kStats;
-    const bool mx = is_max_stat(k);
-    float v = part[((size_t)t * nslab) * kStats + k];
-    for (int s = 1; s < nslab; ++s) {
-        const float o = part[((size_t)t * nslab + s) * kStats + k];
-        v = mx ? fmaxf(v, o) : v + o;
-    }
-    raw[i] = v;
-}
-
 // Normalised features from raw statistics.  M = max mask value over all tiles
 // (0 -> 1); tile_px = sampled pixels of a full tile.  Features 1 and 14 carry
 // the connectivity (8 -> 1, 4 -> 0) and the pixel type (uint16 -> 1, uint8 -> 0).
@@ -201,18 +187,29 @@ DEV void normalise(const float* r, float M, float tile_px, int tile, int dtype, 
     f[15] = r[14] / fmaxf(tile_px, 1.f);
 }
 
-// One workgroup per image: M = the image's largest mask value, then its tiles.
-__global__ __launch_bounds__(256) void k_features_normalise(const float* raw,
-                                                             const uint32_t* tile_base,
-                                                             int ntiles, int tile, int sub,
-                                                             int dtype, int conn, float* feat) {
+// One workgroup per image: its tiles' raw statistics from the slab partials,
+// M = the image's largest mask value, then its normalised features.
+__global__ __launch_bounds__(256) void k_features_finish(const float* part, int nslab,
+                                                          const uint32_t* tile_base, int ntiles,
+                                                          int tile, int sub, int dtype, int conn,
+                                                          float* raw, float* feat) {
     __shared__ float smax[256];
     const int t0 = tile_base ? (int)tile_base[blockIdx.x] : 0;
     const int t1 = tile_base ? (int)tile_base[blockIdx.x + 1] : ntiles;
     float m = 0.f;
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) m = fmaxf(m, raw[(size_t)t * 16 + 15]);
+    for (int i = t0 * kStats + (int)threadIdx.x; i < t1 * kStats; i += blockDim.x) {
+        const int t = i / kStats, k = i % kStats;
+        const bool mx = is_max_stat(k);
+        float v = part[((size_t)t * nslab) * kStats + k];
+        for (int sl = 1; sl < nslab; ++sl) {
+            const float o = part[((size_t)t * nslab + sl) * kStats + k];
+            v = mx ? fmaxf(v, o) : v + o;
+        }
+        raw[i] = v;
+        if (k == 15) m = fmaxf(m, v);
+    }
     smax[threadIdx.x] = m;
-    __syncthreads();
+    __syncthreads();  // also orders this workgroup's raw[] writes before the reads below
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
         __syncthreads();
@@ -386,10 +383,8 @@ hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base
         if (v16) hipLaunchKernelGGL((k_tile_features_part<uint16_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
         else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
     }
-    hipLaunchKernelGGL(k_tile_features_combine, (ntiles * kStats + 255) / 256, 256, 0, s, part,
-                       ntiles, nslab, raw);
-    hipLaunchKernelGGL(k_features_normalise, tile_base ? ds.n : 1, 256, 0, s, raw, tile_base,
-                       ntiles, tile, sub, dtype, conn, feat);
+    hipLaunchKernelGGL(k_features_finish, tile_base ? ds.n : 1, 256, 0, s, part, nslab,
+                       tile_base, ntiles, tile, sub, dtype, conn, raw, feat);
     return hipGetLastError();
 }
 

@@ -44,7 +44,7 @@ MODEL_PATH = os.path.join(MODEL_DIR, "sched_mlp.f32")
 META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
 
 MAX_ROUNDS = 4
-FEATURE_SUB = 4  # row subsampling of the features (the library uses the same)
+FEATURE_SUB = 8  # row subsampling of the features (the library uses the same)
 
 
 def schedule_tile(width, height):
