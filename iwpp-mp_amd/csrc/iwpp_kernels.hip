@@ -150,6 +150,48 @@ DEV void scan_lanes_bwd(uint32_t& P, uint32_t& Q, int lane) {
     }
 }
 
+// Carry into wave wv of a workgroup from the NWV per-wave summaries of one
+// chunk in LDS (sp, sq), in scan direction D: xin enters wave wv, xall
+// leaves the whole chunk; carry enters the chunk.  Up to 4 waves fold the
+// summaries one after another; more waves scan them across the lanes of one
+// 16-lane DPP row (NWV <= 16).
+template <int NWV, int D>
+DEV void wave_carry(const uint32_t* sp, const uint32_t* sq, int wv, int lane, uint32_t carry,
+                    uint32_t& xin, uint32_t& xall) {
+    static_assert(NWV <= 16, "one DPP row");
+    if constexpr (NWV <= 4) {
+        xin = carry;
+        xall = carry;
+#pragma unroll
+        for (int uu = 0; uu < NWV; ++uu) {
+            const int u = D > 0 ? uu : NWV - 1 - uu;
+            const uint32_t tp = sp[u], tq = sq[u];
+            if (D > 0 ? (u < wv) : (u > wv)) xin = max(tp, min(xin, tq));
+            xall = max(tp, min(xall, tq));
+        }
+    } else {
+        uint32_t P = 0, Q = 0xFFFFFFFFu;
+        if (lane < NWV) {  // lane l: the l-th wave in scan order
+            const int src = D > 0 ? lane : NWV - 1 - lane;
+            P = sp[src];
+            Q = sq[src];
+        }
+        scan_step<DPP_ROW_SHR1>(P, Q);
+        scan_step<DPP_ROW_SHR2>(P, Q);
+        scan_step<DPP_ROW_SHR4>(P, Q);
+        scan_step<DPP_ROW_SHR8>(P, Q);
+        const int w = __builtin_amdgcn_readfirstlane(wv);
+        const int pos = D > 0 ? w : NWV - 1 - w;  // this wave's place in scan order
+        uint32_t pp = 0, pq = 0xFFFFFFFFu;
+        if (pos > 0) {
+            pp = rdlane(P, pos - 1);
+            pq = rdlane(Q, pos - 1);
+        }
+        xin = max(pp, min(carry, pq));
+        xall = max(rdlane(P, NWV - 1), min(carry, rdlane(Q, NWV - 1)));
+    }
+}
+
 // ------------------------------------------------------ vector row access ---
 template <class T, int VB>
 struct VecIO {
@@ -207,6 +249,135 @@ struct VecIO {
         for (int i = 0; i < NW; ++i) w[i] = 0;
 #pragma unroll
         for (int k = 0; k < NPX; ++k) w[k / PPW] |= px[k] << (BITS * (k % PPW));
+    }
+};
+
+// ------------------------------------------------- lane pixels for scans ---
+// A lane's marker (v) and mask (c) pixels for the row scans.  With 16-byte
+// lanes they are held as 16-bit pairs, pixel k in the low half and pixel
+// k + NP in the high half, so that one packed min/max (v_pk_min_u16 /
+// v_pk_max_u16) advances both halves of the lane's segment; the halves are
+// joined through their (P, Q) summaries.  Narrow lanes hold one pixel per
+// register.  The scan step is out = max(v, min(x, c)); a segment's summary
+// (P, Q) maps an incoming x to max(P, min(x, Q)).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+DEV uint32_t pk_max(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+DEV uint32_t pk_min(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+// v_perm_b32: byte i of the result is byte sel[i] of {hi:lo} (0x0C gives 0).
+DEV uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amdgcn_perm(hi, lo, sel); }
+
+template <class T, int VB>
+struct LanePx {
+    using V = VecIO<T, VB>;
+    static constexpr bool PAIRS = VB == 16;
+    static constexpr int NW = V::NW;
+    static constexpr int NP = PAIRS ? V::NPX / 2 : V::NPX;  // registers per plane
+    uint32_t v[NP], c[NP];
+    uint32_t hp = 0, hq = 0;  // summary of the half scanned first (pairs)
+
+    static __device__ __forceinline__ void split(const uint32_t (&w)[NW], uint32_t (&p)[NP]) {
+        if constexpr (!PAIRS) {
+            V::unpack(w, p);
+        } else if constexpr (sizeof(T) == 1) {  // p[k] = byte k | byte k+8 << 16
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                p[k] = perm(w[2 + k / 4], w[k / 4],
+                            (uint32_t)(k % 4) | 0x0C00u | ((uint32_t)(4 + k % 4) << 16) | 0x0C000000u);
+        } else {  // p[k] = pixel k | pixel k+4 << 16
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t h = 2 * (k % 2);
+                p[k] = perm(w[2 + k / 2], w[k / 2], h | (h + 1) << 8 | (h + 4) << 16 | (h + 5) << 24);
+            }
+        }
+    }
+    static __device__ __forceinline__ void join(const uint32_t (&p)[NP], uint32_t (&w)[NW]) {
+        if constexpr (!PAIRS) {
+            V::pack(p, w);
+        } else if constexpr (sizeof(T) == 1) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const uint32_t t01 = perm(p[4 * i + 1], p[4 * i], 0x06020400u);
+                const uint32_t t23 = perm(p[4 * i + 3], p[4 * i + 2], 0x06020400u);
+                w[i] = perm(t23, t01, 0x05040100u);
+                w[2 + i] = perm(t23, t01, 0x07060302u);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                w[i] = perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+                w[2 + i] = perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+            }
+        }
+    }
+    static __device__ __forceinline__ uint32_t vmax(uint32_t a, uint32_t b) { return PAIRS ? pk_max(a, b) : max(a, b); }
+    static __device__ __forceinline__ uint32_t vmin(uint32_t a, uint32_t b) { return PAIRS ? pk_min(a, b) : min(a, b); }
+
+    __device__ __forceinline__ void load(const uint32_t (&jw)[NW], const uint32_t (&iw)[NW]) {
+        split(jw, v);
+        split(iw, c);
+    }
+    __device__ __forceinline__ void store(uint32_t (&jw)[NW]) const { join(v, jw); }
+    __device__ __forceinline__ bool clip() {  // whether any pixel changed
+        bool d = false;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const uint32_t o = vmin(v[k], c[k]);
+            d |= o != v[k];
+            v[k] = o;
+        }
+        return d;
+    }
+    __device__ __forceinline__ void wall() {  // no rise, no conduction
+#pragma unroll
+        for (int k = 0; k < NP; ++k) c[k] = 0;
+    }
+    // Summary (P, Q) of the lane's pixels in scan direction D.
+    template <int D>
+    __device__ __forceinline__ void reduce(uint32_t& P, uint32_t& Q) {
+        uint32_t p = 0, q = 0xFFFFFFFFu;
+#pragma unroll
+        for (int kk = 0; kk < NP; ++kk) {
+            const int k = D > 0 ? kk : NP - 1 - kk;
+            p = vmax(v[k], vmin(p, c[k]));
+            q = vmin(q, c[k]);
+        }
+        if constexpr (!PAIRS) {
+            P = p;
+            Q = q;
+        } else {
+            const uint32_t pa = p & 0xFFFFu, pb = p >> 16, qa = q & 0xFFFFu, qb = q >> 16;
+            // forward: low half then high half; backward: high then low
+            hp = D > 0 ? pa : pb;
+            hq = D > 0 ? qa : qb;
+            P = D > 0 ? max(pb, min(pa, qb)) : max(pa, min(pb, qa));
+            Q = min(qa, qb);
+        }
+    }
+    // Scan the lane's pixels in direction D from x, the value next to its
+    // first pixel in that direction (after reduce<D>).  Returns whether any
+    // pixel changed.
+    template <int D>
+    __device__ __forceinline__ bool apply(uint32_t x) {
+        if constexpr (PAIRS) {
+            const uint32_t x2 = max(hp, min(x, hq));  // entering the second half
+            x = D > 0 ? (x | x2 << 16) : (x2 | x << 16);
+        }
+        bool d = false;
+#pragma unroll
+        for (int kk = 0; kk < NP; ++kk) {
+            const int k = D > 0 ? kk : NP - 1 - kk;
+            x = vmax(v[k], vmin(x, c[k]));
+            d |= x != v[k];
+            v[k] = x;
+        }
+        return d;
     }
 };
 
@@ -295,7 +466,8 @@ __global__ __launch_bounds__(kWG) void k_clip(Image im) {
 template <class T, int VB, int NCH, int NWV, bool CLIP>
 __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, uint32_t* changed) {
     using V = VecIO<T, VB>;
-    constexpr int NW = V::NW, NPX = V::NPX, NT = NWV * 64;
+    using L = LanePx<T, VB>;
+    constexpr int NW = V::NW, NT = NWV * 64;
     __shared__ uint32_t sP[2][NCH][NWV], sQ[2][NCH][NWV];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rowbytes = im.W * (int)sizeof(T);
@@ -304,11 +476,10 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
         uint8_t* jr = im.J + (size_t)y * im.pj;
         const uint8_t* ir = im.I + (size_t)y * im.pi;
         uint32_t jv[NCH][NW], iv[NCH][NW];
-        bool dch[NCH];
+        bool wall[NCH], dch[NCH];
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
             const int o = (ch * NT + tid) * VB;
-            dch[ch] = false;
             // A chunk whose tiles are all off is neither read nor written:
             // zero marker and mask make it a wall (carry 0, a lower bound).
             // The clipping round reads every chunk.
@@ -317,40 +488,25 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
                 (cx0 >= im.W || !span_on(tm, cx0, min(im.W, cx0 + NT * VB / (int)sizeof(T)), y))) {
 #pragma unroll
                 for (int i = 0; i < NW; ++i) jv[ch][i] = iv[ch][i] = 0;
+                wall[ch] = true;
+                dch[ch] = false;
                 continue;
             }
             V::load(jr, o, rowbytes, jv[ch]);
             V::load(ir, o, rowbytes, iv[ch]);
-            if (CLIP) {
-                uint32_t a[NPX], b[NPX];
-                V::unpack(jv[ch], a);
-                V::unpack(iv[ch], b);
-#pragma unroll
-                for (int k = 0; k < NPX; ++k) {
-                    dch[ch] |= a[k] > b[k];
-                    a[k] = min(a[k], b[k]);
-                }
-                V::pack(a, jv[ch]);
-            }
-            if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) {
-#pragma unroll
-                for (int i = 0; i < NW; ++i) iv[ch][i] = 0;  // wall: no rise, no conduction
-            }
+            wall[ch] = o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y);
+            dch[ch] = false;
         }
         // forward (left to right)
         uint32_t carry = 0;
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
-            uint32_t v[NPX], c[NPX];
-            V::unpack(jv[ch], v);
-            V::unpack(iv[ch], c);
-            uint32_t P = 0, Q = 0xFFFFFFFFu;
-#pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                P = max(v[k], min(P, c[k]));
-                Q = min(Q, c[k]);
-            }
-            uint32_t Pi = P, Qi = Q;
+            L px;
+            px.load(jv[ch], iv[ch]);
+            if (CLIP) dch[ch] |= px.clip();
+            if (wall[ch]) px.wall();
+            uint32_t Pi, Qi;
+            px.template reduce<1>(Pi, Qi);
             scan_lanes_fwd(Pi, Qi);
             const uint32_t Pe = from_left(Pi, 0u), Qe = from_left(Qi, 0xFFFFFFFFu);
             if (lane == 63) {
@@ -358,40 +514,21 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
                 sQ[0][ch][wv] = Qi;
             }
             __syncthreads();
-            uint32_t xin = carry, xall = carry;
-#pragma unroll
-            for (int u = 0; u < NWV; ++u) {
-                const uint32_t tp = sP[0][ch][u], tq = sQ[0][ch][u];
-                if (u < wv) xin = max(tp, min(xin, tq));
-                xall = max(tp, min(xall, tq));
-            }
+            uint32_t xin, xall;
+            wave_carry<NWV, 1>(sP[0][ch], sQ[0][ch], wv, lane, carry, xin, xall);
             carry = xall;
-            uint32_t x = max(Pe, min(xin, Qe));  // value left of this lane's first pixel
-            bool d = false;
-#pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                const uint32_t out = max(v[k], min(x, c[k]));
-                d |= out != v[k];
-                v[k] = out;
-                x = out;
-            }
-            dch[ch] |= d;
-            V::pack(v, jv[ch]);
+            dch[ch] |= px.template apply<1>(max(Pe, min(xin, Qe)));  // from the value left of this lane
+            px.store(jv[ch]);
         }
         // backward (right to left)
         carry = 0;
 #pragma unroll
         for (int ch = NCH - 1; ch >= 0; --ch) {
-            uint32_t v[NPX], c[NPX];
-            V::unpack(jv[ch], v);
-            V::unpack(iv[ch], c);
-            uint32_t P = 0, Q = 0xFFFFFFFFu;
-#pragma unroll
-            for (int k = NPX - 1; k >= 0; --k) {
-                P = max(v[k], min(P, c[k]));
-                Q = min(Q, c[k]);
-            }
-            uint32_t Pi = P, Qi = Q;
+            L px;
+            px.load(jv[ch], iv[ch]);
+            if (wall[ch]) px.wall();
+            uint32_t Pi, Qi;
+            px.template reduce<-1>(Pi, Qi);
             scan_lanes_bwd(Pi, Qi, lane);
             const uint32_t Pe = from_right(Pi, 0u), Qe = from_right(Qi, 0xFFFFFFFFu);
             if (lane == 0) {
@@ -399,25 +536,11 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
                 sQ[1][ch][wv] = Qi;
             }
             __syncthreads();
-            uint32_t xin = carry, xall = carry;
-#pragma unroll
-            for (int u = NWV - 1; u >= 0; --u) {
-                const uint32_t tp = sP[1][ch][u], tq = sQ[1][ch][u];
-                if (u > wv) xin = max(tp, min(xin, tq));
-                xall = max(tp, min(xall, tq));
-            }
+            uint32_t xin, xall;
+            wave_carry<NWV, -1>(sP[1][ch], sQ[1][ch], wv, lane, carry, xin, xall);
             carry = xall;
-            uint32_t x = max(Pe, min(xin, Qe));  // value right of this lane's last pixel
-            bool d = false;
-#pragma unroll
-            for (int k = NPX - 1; k >= 0; --k) {
-                const uint32_t out = max(v[k], min(x, c[k]));
-                d |= out != v[k];
-                v[k] = out;
-                x = out;
-            }
-            dch[ch] |= d;
-            V::pack(v, jv[ch]);
+            dch[ch] |= px.template apply<-1>(max(Pe, min(xin, Qe)));  // from the value right of this lane
+            px.store(jv[ch]);
         }
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
@@ -434,20 +557,20 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
 // backward sweep re-reading the row this workgroup just wrote (each lane
 // reads back its own vectors, which the caches still hold), so the memory
 // traffic stays at one read of marker and mask and one write of the marker.
-template <class T, int VB, int DIR, bool CLIP>
-DEV void row_stream(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2][4],
-                    uint32_t (&sQ)[2][4], bool& any_change) {
+// One direction (DIR = +1 left to right, -1 right to left) over one row; the
+// direction is a template parameter so that every per-pixel index is a
+// constant and the pixels stay in fixed registers.
+template <class T, int VB, int dir, bool clip>
+DEV void row_stream_dir(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2][4],
+                        uint32_t (&sQ)[2][4], bool& any_change) {
     using V = VecIO<T, VB>;
-    constexpr int NW = V::NW, NPX = V::NPX;
+    constexpr int NW = V::NW;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rowbytes = im.W * (int)sizeof(T);
     const int nch = (rowbytes + kWG * VB - 1) / (kWG * VB);
     uint8_t* jr = im.J + (size_t)y * im.pj;
     const uint8_t* ir = im.I + (size_t)y * im.pi;
-#pragma unroll 1
-    for (int pass = 0; pass < (DIR == 0 ? 2 : 1); ++pass) {
-        const int dir = DIR != 0 ? DIR : (pass == 0 ? 1 : -1);
-        const bool clip = CLIP && pass == 0;
+    {
         uint32_t carry = 0;
         for (int n = 0; n < nch; ++n) {
             const int ch = dir > 0 ? n : nch - 1 - n;
@@ -461,27 +584,15 @@ DEV void row_stream(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2
                     continue;
                 }
             }
-            uint32_t jw[NW], iw[NW], v[NPX], c[NPX], ov[NPX];
+            uint32_t jw[NW], iw[NW];
             V::load(jr, o, rowbytes, jw);
             V::load(ir, o, rowbytes, iw);
-            V::unpack(jw, v);
-            V::unpack(iw, c);
-#pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                ov[k] = v[k];
-                if (clip) v[k] = min(v[k], c[k]);
-            }
-            if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) {
-#pragma unroll
-                for (int k = 0; k < NPX; ++k) c[k] = 0;
-            }
-            uint32_t P = 0, Q = 0xFFFFFFFFu;
-#pragma unroll
-            for (int kk = 0; kk < NPX; ++kk) {
-                const int k = dir > 0 ? kk : NPX - 1 - kk;
-                P = max(v[k], min(P, c[k]));
-                Q = min(Q, c[k]);
-            }
+            LanePx<T, VB> L;
+            L.load(jw, iw);
+            bool d = clip && L.clip();
+            if (o < rowbytes && !tile_on(tm, o / (int)sizeof(T), y)) L.wall();
+            uint32_t P, Q;
+            L.template reduce<dir>(P, Q);
             uint32_t Pi = P, Qi = Q, Pe, Qe;
             if (dir > 0) {
                 scan_lanes_fwd(Pi, Qi);
@@ -504,24 +615,25 @@ DEV void row_stream(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2
                 xall = max(tp, min(xall, tq));
             }
             carry = xall;
-            uint32_t x = max(Pe, min(xin, Qe));
-            bool d = false;
-#pragma unroll
-            for (int kk = 0; kk < NPX; ++kk) {
-                const int k = dir > 0 ? kk : NPX - 1 - kk;
-                const uint32_t out = max(v[k], min(x, c[k]));
-                d |= out != ov[k];
-                v[k] = out;
-                x = out;
-            }
+            d |= L.template apply<dir>(max(Pe, min(xin, Qe)));
             if (d) {
-                V::pack(v, jw);
+                L.store(jw);
                 V::store(jr, o, rowbytes, jw);
             }
             any_change |= d;
             __syncthreads();  // the LDS slot pair is reused two chunks later
         }
     }
+}
+
+// Streaming sweep of one row: DIR = +1 / -1 one direction, DIR = 0 forward
+// then backward.  CLIP applies to the first direction swept.
+template <class T, int VB, int DIR, bool CLIP>
+DEV void row_stream(const Image& im, int y, const TileMask& tm, uint32_t (&sP)[2][4],
+                    uint32_t (&sQ)[2][4], bool& any_change) {
+    if constexpr (DIR >= 0) row_stream_dir<T, VB, 1, CLIP>(im, y, tm, sP, sQ, any_change);
+    if constexpr (DIR == 0) row_stream_dir<T, VB, -1, false>(im, y, tm, sP, sQ, any_change);
+    if constexpr (DIR < 0) row_stream_dir<T, VB, -1, CLIP>(im, y, tm, sP, sQ, any_change);
 }
 
 // Streaming row pass for wide rows: one chunk (256 lanes x VB bytes) in
@@ -1529,11 +1641,13 @@ hipError_t launch_clip(const Image& im, int dtype, hipStream_t s) {
 template <class T, int VB, bool CLIP>
 static void row_pass_t(const Image& im, const TileMask& tm, uint32_t* changed, hipStream_t s) {
     // u16 rows up to 32 KiB are held in registers (256 or 1024 lanes, 1-2
-    // chunks per lane); u8 rows beyond 4 KiB (16 pixels per lane blow up the
-    // register count) and anything wider are streamed chunk by chunk, forward
-    // and backward in one launch.  Measured on MI355X: 16384-wide u16 rows
-    // 3.4 TB/s in registers vs 2.0 TB/s streamed; 32768-wide u8 rows 2.8 TB/s
-    // streamed vs 1.1 TB/s in registers.
+    // chunks per lane); u8 rows beyond 4 KiB and anything wider are streamed
+    // chunk by chunk, forward and backward in one launch.  Measured on
+    // MI355X (packed-pair scans): 16384-wide u16 rows 3.9 TB/s in registers
+    // (1024 lanes x 2 chunks) and 3.9 TB/s streamed; 32768-wide u8 rows
+    // 4.5 TB/s streamed vs 3.0 TB/s in registers (1024 x 2), 2.1 TB/s
+    // (512 x 4) and 1.3 TB/s (256 x 8); 4096-wide u8 rows 2.6 TB/s in
+    // registers vs 2.4 TB/s streamed.
     const int rowbytes = im.W * (int)sizeof(T);
     const int grid = std::min(im.H, 65535);
     if (rowbytes <= 256 * VB)
