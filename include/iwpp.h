@@ -75,6 +75,9 @@ typedef struct {
     float ms_sweep;          /* device time of the sweep phase (HIP events) */
     float ms_seed;           /* device time of the seeding passes */
     float ms_queue;          /* device time of the queue-phase kernels */
+    float ms_plan;           /* host time of the learned schedule's planning: tile
+                                features, network, read-back (includes waiting for
+                                earlier work on the stream) */
     /* With iwpp_options.trace_phases: wave-microseconds spent in the queue
      * phase, summed over all waves (the reference's clock64 split,
      * IWPP MP/src/MorphologicalReconstruction.cu:199-231,534-540). */

@@ -9,6 +9,7 @@
 // rises and never passes the reconstruction, so any partial marker is a valid
 // restart point.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -230,6 +231,8 @@ struct Run {
     Descs ds{};              // what the kernels get (totals set per launch)
     bool wide = false;       // 16-byte lanes in the column / seeding passes
     bool row16 = false;      // 16-byte lanes in the batched row pass
+    int row_wave = 0;        // batched row pass, one wave per row: the widest row in
+                             // bytes (every plane 16-byte aligned), else 0
     int single_row_pass = 0; // one image: use the register row kernels
     int total_rows = 0, total_cw = 0, total_sw = 0, total_mp = 0;
     QueueArgs q{};
@@ -305,6 +308,9 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
     // 16-byte lanes whenever they are at least 3/4 busy (and likewise for the
     // row chunks of the batched row pass).
     r.row16 = n > 1 && aligned && useful >= 0.75 * chunks16;
+    int widest = 0;
+    for (int i = 0; i < n; ++i) widest = std::max(widest, r.h[i].W * r.dtype);
+    r.row_wave = aligned ? widest : 0;
     if (n > 1) r.wide = aligned && useful >= 0.75 * lanes16;
     for (int i = 0; i < n; ++i) {
         r.h[i].strips = col_strips(r.h[i].W, r.dtype, r.wide);
@@ -388,7 +394,7 @@ int sweep_phase(Run& r, int rounds) {
             HIP_TRY(launch_clip(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.s));
         } else {  // a round index past every image's rounds: rows are only clipped
             HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, 0x7FFFFFFF), r.dtype,
-                                          true, r.row16, nullptr, r.s));
+                                          true, r.row16, r.row_wave, nullptr, r.s));
         }
         return IWPP_OK;
     }
@@ -409,7 +415,7 @@ int sweep_phase(Run& r, int rounds) {
                                     nullptr, r.s));
         } else {
             HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0, r.row16,
-                                          nullptr, r.s));
+                                          r.row_wave, nullptr, r.s));
         }
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1, true,
                                       r.wide, nullptr, r.s));
@@ -775,8 +781,11 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
     rc = setup_images(r, imgs, n);
     if (rc) return rc;
     if (schedule == IWPP_SCHED_LEARNED && !r.graph) {
+        const auto t0 = std::chrono::steady_clock::now();
         rc = n == 1 ? learned_plan(r, plan) : learned_plan_batch(r, plan);
         if (rc) return rc;
+        r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - t0).count();
     }
     if (r.graph) {
         rc = setup_queue(r);

@@ -118,9 +118,12 @@ enum QueueKind : int { Q_MUTEX = 0, Q_DQ = 1, Q_UQ = 2 };
 hipError_t launch_clip(const Image& im, int dtype, hipStream_t s);
 hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask& tm,
                            uint32_t* changed, hipStream_t s);
-// Streaming row pass over the rows of every image (ds.total = all rows).
+// Row pass over the rows of every image (ds.total = all rows): one wave per
+// row when wave_rowbytes > 0 (every image 16-byte aligned, rows of at most
+// wave_rowbytes <= 8 KiB), else streamed by one workgroup per row with 16-byte
+// (wide) or 4-byte lanes.
 hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wide,
-                                 uint32_t* changed, hipStream_t s);
+                                 int wave_rowbytes, uint32_t* changed, hipStream_t s);
 // Column pass over every image's column blocks (ds.total = all waves).
 // j_to_p: read J write P (else read P write J); dir +1 down, -1 up; wide =
 // 16-byte lanes (every image 16-byte aligned), else 4-byte lanes.

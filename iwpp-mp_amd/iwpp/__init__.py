@@ -50,6 +50,7 @@ class Stats(ctypes.Structure):
                 ("sweeps_skipped_tiles", ctypes.c_int), ("schedule", ctypes.c_int),
                 ("ms_sweep", ctypes.c_float),
                 ("ms_seed", ctypes.c_float), ("ms_queue", ctypes.c_float),
+                ("ms_plan", ctypes.c_float),
                 ("us_wait", ctypes.c_double), ("us_fetch", ctypes.c_double),
                 ("us_converge", ctypes.c_double), ("us_store", ctypes.c_double),
                 ("us_notify", ctypes.c_double)]
