@@ -89,7 +89,8 @@ struct iwpp_workspace {
     uint32_t* host_ctl = nullptr;  // pinned copy of the control block
     uint8_t* tiles = nullptr;   // per-tile active flags (schedule)
     size_t tiles_n = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // phase timing
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // phase timing,
+                                                                        // [4]: learned plan ready
     ImgDesc* descs = nullptr;   // batch descriptors (device)
     size_t descs_n = 0;
     uint32_t* img_of_mp = nullptr;  // global megapixel -> image (device)
@@ -386,9 +387,10 @@ Descs with_total(Descs d, int total, int round = 0) {
     return d;
 }
 
-// Sweep phase: `rounds` rounds of row pass + column pass down + column pass up.
-int sweep_phase(Run& r, int rounds) {
-    if (rounds <= 0) {  // clip only
+// Sweep phase: rounds k0..k1-1 of row pass + column pass down + column pass
+// up (round 0 clips the marker to the mask in its row pass).
+int sweep_phase(Run& r, int k0, int k1) {
+    if (k1 <= 0) {  // clip only
         if (r.ds.n == 1) {
             const ImgDesc& d = r.h[0];
             HIP_TRY(launch_clip(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.s));
@@ -398,11 +400,11 @@ int sweep_phase(Run& r, int rounds) {
         }
         return IWPP_OK;
     }
-    if (r.round_masks) {  // blocks may be skipped: start the scratch plane at 0
-        const ImgDesc& d = r.h[0];
+    if (r.round_masks && k0 == 0) {  // blocks may be skipped: start the scratch plane at 0
+        const ImgDesc& d = r.h[0];  // (after a full round it holds lower bounds already)
         HIP_TRY(launch_fill_u32((uint32_t*)d.P, 0u, d.pp * d.H / 4, r.s));
     }
-    for (int k = 0; k < rounds; ++k) {
+    for (int k = k0; k < k1; ++k) {
         if (r.round_masks) {
             const TileMask tm{r.round_masks + (size_t)k * r.sched_nt, r.sched_tile, r.sched_tile,
                               r.sched_ntx};
@@ -422,7 +424,7 @@ int sweep_phase(Run& r, int rounds) {
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1, false,
                                       r.wide, nullptr, r.s));
     }
-    r.st.sweep_rounds += rounds;
+    r.st.sweep_rounds += std::max(0, k1 - k0);
     return IWPP_OK;
 }
 
@@ -590,126 +592,160 @@ int clamp_rounds(float v) {
     return std::isfinite(v) ? (int)std::lround(std::min(4.f, std::max(0.f, v))) : 0;
 }
 
-// Learned schedule for a batch: features of every tile of every image (one
-// set of launches), MLP, then per image R_i = round(area-weighted mean of its
-// tiles' predicted rounds); the sweep kernels skip an image from round R_i on
-// (images with R_i = 0 are only clipped).  Seeding keeps index order.
-int learned_plan_batch(Run& r, SchedulePlan& plan) {
+// Learned schedule, in two halves around the first sweep round.  begin
+// launches the tile features, the network and the read-back of its
+// predictions, and records an event; the first sweep round (needed by every
+// schedule that sweeps at all, and harmless otherwise) is launched behind it;
+// end waits for the event, so the host plans while that round runs, and
+// fixes the remaining rounds.  Every tile of every image is one network
+// sample; image i runs R_i = round(area-weighted mean of its tiles'
+// predicted rounds) rounds in total.
+
+// Tile counts per image (prefix), into pinned staging after the predictions.
+struct LearnedLayout {
+    int tile = 0, total = 0;
+    std::vector<uint32_t> base;  // n + 1
+};
+
+LearnedLayout learned_layout(const Run& r, int tile) {
+    LearnedLayout L;
+    const int n = (int)r.h.size();
+    L.tile = tile;
+    L.base.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        const ImgDesc& d = r.h[i];
+        L.base[i + 1] = L.base[i] + (uint32_t)(((d.W + tile - 1) / tile) * ((d.H + tile - 1) / tile));
+    }
+    L.total = (int)L.base[n];
+    return L;
+}
+
+// Pinned staging: predictions (2 floats per tile), then per kind: the batch
+// keeps tile bases and the descriptors; one image keeps tile order, starts
+// and per-round masks.
+size_t learned_staging_bytes(const Run& r, int total) {
+    const size_t n = r.h.size();
+    return (size_t)total * 2 * sizeof(float) + (n + 1) * 4 + n * sizeof(ImgDesc) +
+           (size_t)(2 * total + 1) * 4 + (size_t)4 * total + 256;
+}
+
+int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
+    *pending = false;
     iwpp_workspace* ws = r.ws;
     bool have = false;
     int rc = ensure_model(ws, &have);
-    if (rc || !have) return rc;
-    const int n = (int)r.h.size(), tile = plan.tile;
-    std::vector<uint32_t> base(n + 1, 0);
-    bool v16 = true;
-    for (int i = 0; i < n; ++i) {
-        const ImgDesc& d = r.h[i];
-        base[i + 1] = base[i] + (uint32_t)(((d.W + tile - 1) / tile) * ((d.H + tile - 1) / tile));
-        v16 = v16 && (uintptr_t)d.J % 16 == 0 && (uintptr_t)d.I % 16 == 0 && d.pj % 16 == 0 &&
-              d.pi % 16 == 0;
-    }
-    const int total = (int)base[n];
-    rc = ensure_host_plan(ws, (size_t)total * 2 * sizeof(float) + (size_t)(n + 1) * 4 + 64);
+    if (rc || !have) return rc;  // no model: the plain heuristic
+    rc = ensure_events(ws);
+    if (rc) return rc;
+    const int n = (int)r.h.size();
+    const LearnedLayout L = learned_layout(r, plan.tile);
+    rc = ensure_host_plan(ws, learned_staging_bytes(r, L.total));
     if (rc) return rc;
     float* pred = (float*)ws->host_plan;
-    uint32_t* h_base = (uint32_t*)(pred + (size_t)total * 2);
-    std::copy(base.begin(), base.end(), h_base);
-    HIP_TRY(grow(ws->order, ws->order_n, (size_t)n + 1));
-    HIP_TRY(hipMemcpyAsync(ws->order, h_base, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, r.s));
     HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
-                 (size_t)total * features_slabs(tile, kFeatureSub) * 16));
-    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)total * 16));
-    HIP_TRY(grow(ws->feat, ws->feat_n, (size_t)total * 16));
-    HIP_TRY(grow(ws->pred, ws->pred_n, (size_t)total * 2));
-    HIP_TRY(launch_tile_features_batch(r.ds, ws->order, total, v16, r.dtype, r.conn, tile,
-                                       kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat, r.s));
-    HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, total, ws->pred, nullptr, r.s));
-    HIP_TRY(hipMemcpyAsync(pred, ws->pred, (size_t)total * 2 * sizeof(float),
+                 (size_t)L.total * features_slabs(L.tile, kFeatureSub) * 16));
+    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)L.total * 16));
+    HIP_TRY(grow(ws->feat, ws->feat_n, (size_t)L.total * 16));
+    HIP_TRY(grow(ws->pred, ws->pred_n, (size_t)L.total * 2));
+    if (n == 1) {
+        const ImgDesc& d = r.h[0];
+        HIP_TRY(launch_tile_features(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.conn,
+                                     L.tile, kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat,
+                                     r.s));
+    } else {
+        bool v16 = true;
+        for (const ImgDesc& d : r.h)
+            v16 = v16 && (uintptr_t)d.J % 16 == 0 && (uintptr_t)d.I % 16 == 0 && d.pj % 16 == 0 &&
+                  d.pi % 16 == 0;
+        uint32_t* h_base = (uint32_t*)(pred + (size_t)L.total * 2);
+        std::copy(L.base.begin(), L.base.end(), h_base);
+        HIP_TRY(grow(ws->order, ws->order_n, (size_t)n + 1));
+        HIP_TRY(hipMemcpyAsync(ws->order, h_base, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, r.s));
+        HIP_TRY(launch_tile_features_batch(r.ds, ws->order, L.total, v16, r.dtype, r.conn, L.tile,
+                                           kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat,
+                                           r.s));
+    }
+    HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, L.total, ws->pred, nullptr, r.s));
+    HIP_TRY(hipMemcpyAsync(pred, ws->pred, (size_t)L.total * 2 * sizeof(float),
                            hipMemcpyDeviceToHost, r.s));
-    HIP_TRY(hipStreamSynchronize(r.s));
+    HIP_TRY(hipEventRecord(ws->ev[4], r.s));
+    *pending = true;
+    return IWPP_OK;
+}
+
+// Area-weighted mean of the predicted rounds of image i's tiles.
+int image_rounds(const ImgDesc& d, const float* pred, uint32_t b0, uint32_t b1, int tile) {
+    const int ntx = (d.W + tile - 1) / tile;
+    double wsum = 0, wy = 0;
+    for (uint32_t g = b0; g < b1; ++g) {
+        const int t = (int)(g - b0), tx = t % ntx, ty = t / ntx;
+        const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
+                            (double)(std::min(d.H, (ty + 1) * tile) - ty * tile);
+        const float v = pred[(size_t)g * 2];
+        wy += area * (std::isfinite(v) ? std::min(4.f, std::max(0.f, v)) : 0.f);
+        wsum += area;
+    }
+    return clamp_rounds((float)(wy / wsum));
+}
+
+// Batch: the sweep kernels skip image i from round R_i on; seeding keeps
+// index order.  R_i is capped at the batch's plain (or autotuned) rounds: the
+// network's rounds come from images timed one at a time, where the queue
+// phase of a small image costs a launch of its own; in a batch the images
+// share it, and more sweeps than the batch's measured choice were found to
+// cost more than they save.  The descriptors with their rounds go up from
+// pinned staging (nothing waits for the copy: the staging is not written
+// again before the call's final synchronisation).
+int learned_plan_end_batch(Run& r, SchedulePlan& plan) {
+    iwpp_workspace* ws = r.ws;
+    const int n = (int)r.h.size();
+    const LearnedLayout L = learned_layout(r, plan.tile);
+    HIP_TRY(hipEventSynchronize(ws->ev[4]));
+    const float* pred = (const float*)ws->host_plan;
+    const int cap = std::max(1, plan.sweep_rounds);
     int rmax = 0;
     long long skipped = 0;
-    std::vector<int> R(n);
     for (int i = 0; i < n; ++i) {
-        const ImgDesc& d = r.h[i];
-        const int ntx = (d.W + tile - 1) / tile;
-        double wsum = 0, wy = 0;
-        for (uint32_t g = base[i]; g < base[i + 1]; ++g) {
-            const int t = (int)(g - base[i]), tx = t % ntx, ty = t / ntx;
-            const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
-                                (double)(std::min(d.H, (ty + 1) * tile) - ty * tile);
-            const float v = pred[(size_t)g * 2];
-            wy += area * (std::isfinite(v) ? std::min(4.f, std::max(0.f, v)) : 0.f);
-            wsum += area;
-        }
-        R[i] = clamp_rounds((float)(wy / wsum));
-        rmax = std::max(rmax, R[i]);
+        r.h[i].rounds = std::min(cap, image_rounds(r.h[i], pred, L.base[i], L.base[i + 1], L.tile));
+        rmax = std::max(rmax, r.h[i].rounds);
     }
-    for (int i = 0; i < n; ++i) {
-        r.h[i].rounds = R[i];
-        skipped += (long long)(rmax - R[i]) * (base[i + 1] - base[i]);
-    }
-    HIP_TRY(hipMemcpyAsync(ws->descs, r.h.data(), sizeof(ImgDesc) * n, hipMemcpyHostToDevice,
-                           r.s));
-    HIP_TRY(hipStreamSynchronize(r.s));  // r.h may change before the copy has read it
+    for (int i = 0; i < n; ++i) skipped += (long long)(rmax - r.h[i].rounds) * (L.base[i + 1] - L.base[i]);
+    ImgDesc* h_desc = (ImgDesc*)round_up((uintptr_t)(pred + (size_t)L.total * 2) + (n + 1) * 4, 64);
+    std::copy(r.h.begin(), r.h.end(), h_desc);
+    HIP_TRY(hipMemcpyAsync(ws->descs, h_desc, sizeof(ImgDesc) * n, hipMemcpyHostToDevice, r.s));
     plan.sweep_rounds = rmax;
     r.st.schedule = IWPP_SCHED_LEARNED;
     r.st.sweeps_skipped_tiles = (int)std::min<long long>(skipped, 0x7FFFFFFF);
     return IWPP_OK;
 }
 
-int learned_plan(Run& r, SchedulePlan& plan) {
+// One image: R = area-weighted mean of the predicted rounds; a tile whose own
+// prediction is at least two rounds lower skips the rounds from its
+// prediction on (never the first, which has already been launched); the
+// seeding pass visits tiles by decreasing predicted priority.
+int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     iwpp_workspace* ws = r.ws;
-    bool have = false;
-    int rc0 = ensure_model(ws, &have);
-    if (rc0 || !have) return rc0;  // no model: the plain heuristic
     const ImgDesc& d = r.h[0];
     const int tile = plan.tile;
     const int ntx = (d.W + tile - 1) / tile, nty = (d.H + tile - 1) / tile, nt = ntx * nty;
-    // pinned staging: predictions in, tile order + starts + masks out
-    rc0 = ensure_host_plan(ws, (size_t)nt * 2 * sizeof(float) + (size_t)(2 * nt + 1) * 4 +
-                                   (size_t)4 * nt + 64);
-    if (rc0) return rc0;
-    float* pred = (float*)ws->host_plan;
-    uint32_t* h_order = (uint32_t*)(pred + (size_t)nt * 2);  // tiles[nt], starts[nt+1]
+    HIP_TRY(hipEventSynchronize(ws->ev[4]));
+    const float* pred = (const float*)ws->host_plan;
+    uint32_t* h_order = (uint32_t*)round_up((uintptr_t)(pred + (size_t)nt * 2) + 2 * 4 +
+                                                sizeof(ImgDesc),
+                                            64);  // tiles[nt], starts[nt+1]
     uint8_t* h_masks = (uint8_t*)(h_order + 2 * nt + 1);
-    HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
-                 (size_t)nt * features_slabs(tile, kFeatureSub) * 16));
-    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)nt * 16));
-    HIP_TRY(grow(ws->feat, ws->feat_n, (size_t)nt * 16));
-    HIP_TRY(grow(ws->pred, ws->pred_n, (size_t)nt * 2));
-    HIP_TRY(launch_tile_features(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.conn, tile,
-                                 kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat, r.s));
-    HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, nt, ws->pred, nullptr, r.s));
-    HIP_TRY(hipMemcpyAsync(pred, ws->pred, (size_t)nt * 2 * sizeof(float),
-                           hipMemcpyDeviceToHost, r.s));
-    HIP_TRY(hipStreamSynchronize(r.s));
-    // The image runs R = round(area-weighted mean of the predicted rounds)
-    // rounds; a tile whose own prediction is at least two rounds lower skips
-    // the rounds from its prediction on.
-    double wsum = 0, wy = 0;
-    for (int t = 0; t < nt; ++t) {
-        const int tx = t % ntx, ty = t / ntx;
-        const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
-                            (double)(std::min(d.H, (ty + 1) * tile) - ty * tile);
-        const float v = pred[(size_t)t * 2];
-        wy += area * (std::isfinite(v) ? std::min(4.f, std::max(0.f, v)) : 0.f);
-        wsum += area;
-    }
-    const int rounds = clamp_rounds((float)(wy / wsum));
+    const int rounds = image_rounds(d, pred, 0, (uint32_t)nt, tile);
     std::vector<int> k(nt);
     for (int t = 0; t < nt; ++t) {
-        const int kt = clamp_rounds(pred[(size_t)t * 2]);
+        const int kt = std::max(1, clamp_rounds(pred[(size_t)t * 2]));
         k[t] = kt <= rounds - 2 ? kt : rounds;
     }
     plan.sweep_rounds = rounds;
     r.st.schedule = IWPP_SCHED_LEARNED;
     int skipped = 0;
-    for (int t = 0; t < nt; ++t) skipped += rounds - k[t];
+    for (int t = 0; t < nt; ++t) skipped += std::max(0, rounds - k[t]);
     r.st.sweeps_skipped_tiles = skipped;
     if (nt == 1) return IWPP_OK;  // one tile: uniform rounds, nothing to order
-    // seeding order: tiles by decreasing priority (the kernel expands a tile
-    // into its megapixels)
     std::vector<int> tiles(nt);
     for (int t = 0; t < nt; ++t) tiles[t] = t;
     std::stable_sort(tiles.begin(), tiles.end(), [&](int a, int b) {
@@ -734,12 +770,10 @@ int learned_plan(Run& r, SchedulePlan& plan) {
     r.order_mpt = mpt;
     if (skipped > 0) {  // per-round tile masks, only when some tile stops early
         const size_t nm = (size_t)rounds * nt;
-        uint8_t* mk = nm <= (size_t)4 * nt ? h_masks : nullptr;
-        if (!mk) return fail(IWPP_ERR_ARG, "internal: mask staging too small");
         for (int rr = 0; rr < rounds; ++rr)
-            for (int t = 0; t < nt; ++t) mk[(size_t)rr * nt + t] = k[t] > rr ? 1 : 0;
+            for (int t = 0; t < nt; ++t) h_masks[(size_t)rr * nt + t] = k[t] > rr ? 1 : 0;
         HIP_TRY(grow(ws->tiles, ws->tiles_n, nm));
-        HIP_TRY(hipMemcpyAsync(ws->tiles, mk, nm, hipMemcpyHostToDevice, r.s));
+        HIP_TRY(hipMemcpyAsync(ws->tiles, h_masks, nm, hipMemcpyHostToDevice, r.s));
         r.round_masks = ws->tiles;
         r.sched_tile = tile;
         r.sched_ntx = ntx;
@@ -780,12 +814,10 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
         return fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
     rc = setup_images(r, imgs, n);
     if (rc) return rc;
+    bool planning = false;
     if (schedule == IWPP_SCHED_LEARNED && !r.graph) {
-        const auto t0 = std::chrono::steady_clock::now();
-        rc = n == 1 ? learned_plan(r, plan) : learned_plan_batch(r, plan);
+        rc = learned_plan_begin(r, plan, &planning);
         if (rc) return rc;
-        r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
-                           std::chrono::steady_clock::now() - t0).count();
     }
     if (r.graph) {
         rc = setup_queue(r);
@@ -797,7 +829,18 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
         if (rc) return rc;
         HIP_TRY(hipEventRecord(r.ws->ev[0], r.s));
     }
-    rc = sweep_phase(r, plan.sweep_rounds);
+    if (planning) {
+        rc = sweep_phase(r, 0, 1);  // launched behind the network, before its answer
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = n == 1 ? learned_plan_end_single(r, plan) : learned_plan_end_batch(r, plan);
+        if (rc) return rc;
+        r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - t0).count();
+        rc = sweep_phase(r, 1, plan.sweep_rounds);
+    } else {
+        rc = sweep_phase(r, 0, plan.sweep_rounds);
+    }
     if (rc) return rc;
     if (r.graph) return queue_launches(r);
     rc = queue_phase(r);

@@ -187,29 +187,41 @@ DEV void normalise(const float* r, float M, float tile_px, int tile, int dtype, 
     f[15] = r[14] / fmaxf(tile_px, 1.f);
 }
 
-// One workgroup per image: its tiles' raw statistics from the slab partials,
-// M = the image's largest mask value, then its normalised features.
-__global__ __launch_bounds__(256) void k_features_finish(const float* part, int nslab,
-                                                          const uint32_t* tile_base, int ntiles,
+// Raw statistics of each tile from its slab partials: one wave per tile,
+// lane = (slab group g = lane / 16, statistic k = lane % 16); each lane folds
+// every fourth slab, then the four groups are folded across lanes (a fixed
+// order: the result does not depend on scheduling).
+__global__ __launch_bounds__(256) void k_features_tiles(const float* part, int nslab, int ntiles,
+                                                         float* raw) {
+    const int lane = threadIdx.x & 63;
+    const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (t >= ntiles) return;  // whole wave
+    const int k = lane & 15, g = lane >> 4;
+    const bool mx = is_max_stat(k);
+    float v = 0.f;  // identity of both folds (statistics are >= 0)
+    for (int sl = g; sl < nslab; sl += 4) {
+        const float o = part[((size_t)t * nslab + sl) * kStats + k];
+        v = mx ? fmaxf(v, o) : v + o;
+    }
+    const float o16 = __shfl_xor(v, 16);
+    v = mx ? fmaxf(v, o16) : v + o16;
+    const float o32 = __shfl_xor(v, 32);
+    v = mx ? fmaxf(v, o32) : v + o32;
+    if (g == 0) raw[(size_t)t * kStats + k] = v;
+}
+
+// One workgroup per image: M = the image's largest mask value over its tiles'
+// raw statistics, then its tiles' normalised features.
+__global__ __launch_bounds__(256) void k_features_finish(const uint32_t* tile_base, int ntiles,
                                                           int tile, int sub, int dtype, int conn,
-                                                          float* raw, float* feat) {
+                                                          const float* raw, float* feat) {
     __shared__ float smax[256];
     const int t0 = tile_base ? (int)tile_base[blockIdx.x] : 0;
     const int t1 = tile_base ? (int)tile_base[blockIdx.x + 1] : ntiles;
     float m = 0.f;
-    for (int i = t0 * kStats + (int)threadIdx.x; i < t1 * kStats; i += blockDim.x) {
-        const int t = i / kStats, k = i % kStats;
-        const bool mx = is_max_stat(k);
-        float v = part[((size_t)t * nslab) * kStats + k];
-        for (int sl = 1; sl < nslab; ++sl) {
-            const float o = part[((size_t)t * nslab + sl) * kStats + k];
-            v = mx ? fmaxf(v, o) : v + o;
-        }
-        raw[i] = v;
-        if (k == 15) m = fmaxf(m, v);
-    }
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) m = fmaxf(m, raw[(size_t)t * kStats + 15]);
     smax[threadIdx.x] = m;
-    __syncthreads();  // also orders this workgroup's raw[] writes before the reads below
+    __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
         __syncthreads();
@@ -383,8 +395,9 @@ hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base
         if (v16) hipLaunchKernelGGL((k_tile_features_part<uint16_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
         else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
     }
-    hipLaunchKernelGGL(k_features_finish, tile_base ? ds.n : 1, 256, 0, s, part, nslab,
-                       tile_base, ntiles, tile, sub, dtype, conn, raw, feat);
+    hipLaunchKernelGGL(k_features_tiles, (ntiles + 3) / 4, 256, 0, s, part, nslab, ntiles, raw);
+    hipLaunchKernelGGL(k_features_finish, tile_base ? ds.n : 1, 256, 0, s, tile_base, ntiles, tile,
+                       sub, dtype, conn, raw, feat);
     return hipGetLastError();
 }
 

@@ -44,7 +44,7 @@ MODEL_PATH = os.path.join(MODEL_DIR, "sched_mlp.f32")
 META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
 
 MAX_ROUNDS = 4
-FEATURE_SUB = 8  # row subsampling of the features (the library uses the same)
+FEATURE_SUB = 16  # row subsampling of the features (the library uses the same)
 
 
 def schedule_tile(width, height):
@@ -80,10 +80,20 @@ def bf16_bits(w):
 
 
 def load_model(path=None):
+    """Weights of a model file.  Its .json beside it, when present, must name
+    the feature subsampling the library computes (a model trained on other
+    features is refused)."""
     path = path or MODEL_PATH
     w = np.fromfile(path, dtype="<f4")
     if w.size != MLP_PARAMS:
         raise ValueError(f"{path}: {w.size} values, expected {MLP_PARAMS}")
+    meta_path = os.path.splitext(path)[0] + ".json"
+    if os.path.exists(meta_path):
+        with open(meta_path) as f:
+            sub = json.load(f).get("feature_sub")
+        if sub is not None and sub != FEATURE_SUB:
+            raise ValueError(f"{path}: trained on features of every {sub}th row, "
+                             f"the library samples every {FEATURE_SUB}th")
     return w.astype(np.float32)
 
 
