@@ -33,6 +33,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profile_traffic(workload, prefixes):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3
+    summary of this workload (profiles/<workload>_kernels.md: FETCH_SIZE and
+    WRITE_SIZE passes, calibrated to bytes there), averaged over the family's
+    dispatches; (None, reason) when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_kernels.md")
+    if not os.path.exists(path):
+        return None, f"no {os.path.relpath(path, ROOT)}"
+    num = den = 0.0
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) < 8 or not cells[0].startswith("`"):
+            continue
+        name = cells[0].strip("`")
+        if not name.startswith(prefixes) or cells[7] == "-":
+            continue
+        calls = float(cells[1])
+        num += calls * float(cells[7]) * 1e6
+        den += calls
+    if den == 0:
+        return None, f"no {'/'.join(prefixes)} rows in {os.path.relpath(path, ROOT)}"
+    return num / den, (f"{os.path.relpath(path, ROOT)}: (2.0 x FETCH_SIZE + WRITE_SIZE) per "
+                       f"dispatch, mean over the {'/'.join(prefixes)} dispatches")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,21 +110,25 @@ def main():
             iwpp.set_model(None)
 
         def step():
-            work.buf.copy_(marker0.buf)
             return iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue,
                                     schedule, opt, ws)
 
         for i in range(args.warmup):
+            work.buf.copy_(marker0.buf)
             st = step()
             log(f"{schedule} warmup {i}: {st}")
-        torch.cuda.synchronize()
+        # Each step reconstructs a fresh copy of the marker (the reconstruction
+        # is in place); the copy is made before the step's timed bracket.
         stats = []
-        t0 = time.perf_counter()
+        tot = 0.0
         for _ in range(args.steps):
+            work.buf.copy_(marker0.buf)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
             stats.append(step())
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        ms = (t1 - t0) * 1e3 / args.steps
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t0
+        ms = tot * 1e3 / args.steps
         log(f"{schedule}: timed {args.steps} steps: {ms:.3f} ms/step, "
             f"{px / (ms * 1e-3) / 1e6:.1f} Mpixel/s")
         return ms, stats, work.numpy()
@@ -143,19 +172,25 @@ def main():
     used_queue = {0: "mutex", 1: "dq", 2: "uq"}[stats[-1]["queue"]]
     rounds = counters["sweep_rounds"]
     visits = statistics.mean(s["tile_visits"] for s in stats)
-    # Per reconstruction: device time of each kernel family, its launches, and
-    # the algorithmic bytes one launch must move (3*sizeof(T) per pixel it owns:
-    # read marker, read mask, write marker; a megapixel visit owns 64*64 pixels).
+    # Per reconstruction: device time of each kernel family inside the timed
+    # steps (the library's HIP events on its own stream), its launches, and
+    # the algorithmic bytes one launch must move: 3*sizeof(T) per pixel a
+    # sweep pass owns (read marker, read mask, write marker; row and column
+    # passes alike), 3*sizeof(T) per pixel of each megapixel visit (64*64
+    # pixels) in the queue phase, 2*sizeof(T) per pixel for the seeding scan.
+    queue_name = "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>")
     kernels = {
-        "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>"):
-            (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize),
-        "k_col_pass": (2 * rounds * ms_col, 2 * rounds, bytes_sweep),
-        "k_row_pass": (rounds * ms_row, rounds, bytes_sweep),
+        "sweep passes (k_row_pass*, k_col_pass)":
+            (phases["ms_sweep"], 3 * rounds, bytes_sweep,
+             ("k_row_pass", "k_col_pass")),
+        queue_name: (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize, ("k_queue_",)),
+        "k_unstable_scan": (phases["ms_seed"], 1, 2 * tsize * px, ("k_unstable_scan",)),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
-    tot_ms, launches, alg_bytes = kernels[dom]
+    tot_ms, launches, alg_bytes, prefixes = kernels[dom]
     avg_ms = tot_ms / max(1, launches)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    traffic, traffic_src = profile_traffic(case.name, prefixes)
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -163,11 +198,12 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": None,
+        "traffic": traffic,
         "algorithmic_bytes_per_launch": alg_bytes,
         "avg_launch_ms": avg_ms,
-        "basis": "3*sizeof(T) bytes per owned pixel per launch / average launch time "
-                 "(HIP events on the library's stream)",
+        "basis": "algorithmic bytes per launch / average launch time inside the timed steps "
+                 "(the library's HIP events on its stream)",
+        "traffic_basis": traffic_src,
     }
     sweep_kernels = {
         "row_pass_ms": ms_row,
@@ -177,6 +213,7 @@ def main():
         "col_pass_GBps": bytes_sweep / (ms_col * 1e-3) / 1e9,
         "col_pass_frac": bytes_sweep / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "per_reconstruction_ms": {k: v[0] for k, v in kernels.items()},
+        "basis": "row_pass / col_pass: one isolated launch each over the clipped marker, HIP events",
     }
 
     # --- CPU baseline: the oracle (a port of the sequential hybrid), one core
@@ -353,14 +390,17 @@ def bench_batch(args):
     rounds = max(1, batch_st["sweep_rounds"])
     kernels = {
         "k_queue_" + ("mutex" if batch_st["queue"] == 0 else "bsp"):
-            (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * cases[0].mask.itemsize),
-        "k_row_pass_batch+k_col_pass": (phases["ms_sweep"], 3 * rounds,
-                                        3 * cases[0].mask.itemsize * px),
+            (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * cases[0].mask.itemsize,
+             ("k_queue_",)),
+        "sweep passes (k_row_pass_wave, k_col_pass)":
+            (phases["ms_sweep"], 3 * rounds, 3 * cases[0].mask.itemsize * px,
+             ("k_row_pass", "k_col_pass")),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
-    tot, launches, alg = kernels[dom]
+    tot, launches, alg, prefixes = kernels[dom]
     avg = tot / max(1, launches)
     achieved = alg / (avg * 1e-3) / 1e9 if avg > 0 else None
+    traffic, traffic_src = profile_traffic("c4_batch_256_u8", prefixes)
     out = {
         "metric": METRIC,
         "value": px / best_ms / 1e3,
@@ -378,7 +418,9 @@ def bench_batch(args):
                    "connectivity": 8, "form": form, "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": None, "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg},
+                     "traffic": traffic, "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg,
+                     "basis": "batched call's phase times (the library's HIP events), per launch",
+                     "traffic_basis": traffic_src},
         "cpu_baseline": cpu,
         "exact": exact,
         "per_image_loop": {"ms": ms_loop, "mpixel_s": px / ms_loop / 1e3},
