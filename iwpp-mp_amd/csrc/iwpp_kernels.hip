@@ -794,7 +794,8 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
                    const uint8_t* __restrict__ I, size_t ps, size_t pd, size_t pi, int W, int H,
                    int R, int strip, int seg, const TileMask& tm, uint32_t* changed) {
     using V = VecIO<T, VB>;
-    constexpr int NW = V::NW, NPX = V::NPX;
+    using L = LanePx<T, VB>;
+    constexpr int NW = V::NW, NP = L::NP;
     const int lane = lane_id();
     const int rowbytes = W * (int)sizeof(T);
     const int xb0 = strip * 64 * VB;  // first byte of the strip
@@ -810,7 +811,9 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
     // then see an older value of the same pixels, a valid lower bound (the
     // plane is zeroed before the first scheduled round).
     if (tm.active && !wave_any(on)) return;
-    uint32_t prev[NPX];
+    // The previous output row, as the lane's pixel registers (16-bit pairs
+    // with 16-byte lanes: pixel k and k + NP in one register).
+    uint32_t prev[NP];
     {
         const int yh = DIR > 0 ? y0 - 1 : y1;
         uint32_t w[NW];
@@ -819,7 +822,7 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
 #pragma unroll
             for (int i = 0; i < NW; ++i) w[i] = 0;
         }
-        V::unpack(w, prev);
+        L::split(w, prev);
     }
     bool any_change = false;
     // Rows go in groups of CU: the group's loads (source, mask and, for 8-
@@ -852,36 +855,45 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
             if (i0 + u >= n) break;  // the same in every lane
             const int i = i0 + u;
             const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
-            uint32_t v[NPX], c[NPX], m[NPX];
-            V::unpack(jw[u], v);
-            V::unpack(iw[u], c);
+            uint32_t v[NP], c[NP], m[NP];
+            L::split(jw[u], v);
+            L::split(iw[u], c);
             if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
 #pragma unroll
-                for (int k = 0; k < NPX; ++k) c[k] = 0;
+                for (int k = 0; k < NP; ++k) c[k] = 0;
             }
             if constexpr (CONN == 8) {
-                const uint32_t left = from_left(prev[NPX - 1], le[u]);
-                const uint32_t right = from_right(prev[0], re[u]);
+                // left neighbour of the lane's first register, right neighbour
+                // of its last: the adjacent lane's pixel (outside the strip:
+                // the loaded one) joined with this lane's own pixel
+                uint32_t left, right;
+                if constexpr (L::PAIRS) {
+                    left = from_left(prev[NP - 1] >> 16, le[u]) | (prev[NP - 1] << 16);
+                    right = (prev[0] >> 16) | (from_right(prev[0] & 0xFFFFu, re[u]) << 16);
+                } else {
+                    left = from_left(prev[NP - 1], le[u]);
+                    right = from_right(prev[0], re[u]);
+                }
 #pragma unroll
-                for (int k = 0; k < NPX; ++k) {
+                for (int k = 0; k < NP; ++k) {
                     const uint32_t a = k > 0 ? prev[k - 1] : left;
-                    const uint32_t b = k < NPX - 1 ? prev[k + 1] : right;
-                    m[k] = max(prev[k], max(a, b));
+                    const uint32_t b = k < NP - 1 ? prev[k + 1] : right;
+                    m[k] = L::vmax(prev[k], L::vmax(a, b));
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < NPX; ++k) m[k] = prev[k];
+                for (int k = 0; k < NP; ++k) m[k] = prev[k];
             }
             bool d = false;
 #pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                const uint32_t out = max(v[k], min(m[k], c[k]));
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t out = L::vmax(v[k], L::vmin(m[k], c[k]));
                 d |= out != v[k];
                 prev[k] = out;
             }
             any_change |= d;
             uint32_t ow[NW];
-            V::pack(prev, ow);
+            L::join(prev, ow);
             V::store(D + (size_t)y * pd, o, rowbytes, ow);
         }
     }

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
 // Normalised features from raw statistics.  M = max mask value over all tiles
 // (0 -> 1); tile_px = sampled pixels of a full tile.  Features 1 and 14 carry
 // the connectivity (8 -> 1, 4 -> 0) and the pixel type (uint16 -> 1, uint8 -> 0).
-DEV void normalise(const float* r, float M, float tile_px, int tile, int dtype, int conn,
+DEV void normalise(const float* r, float M, int tile, int dtype, int conn, float log2px,
                    float* f) {
     const float n = fmaxf(r[14], 1.f);
     const float inv = 1.f / fmaxf(M, 1.f);
@@ -184,7 +184,9 @@ DEV void normalise(const float* r, float M, float tile_px, int tile, int dtype, 
     f[12] = r[12] / side * inv;
     f[13] = r[13] / side * inv;
     f[14] = dtype == 2 ? 1.f : 0.f;
-    f[15] = r[14] / fmaxf(tile_px, 1.f);
+    // the image's size (what a size rule would key on), at most 2^28 pixels:
+    // larger images than the training set's largest are seen as that size
+    f[15] = fminf(log2px, 28.f) / 32.f;
 }
 
 // Raw statistics of each tile from its slab partials: one wave per tile,
@@ -212,9 +214,11 @@ __global__ __launch_bounds__(256) void k_features_tiles(const float* part, int n
 
 // One workgroup per image: M = the image's largest mask value over its tiles'
 // raw statistics, then its tiles' normalised features.
-__global__ __launch_bounds__(256) void k_features_finish(const uint32_t* tile_base, int ntiles,
-                                                          int tile, int sub, int dtype, int conn,
+__global__ __launch_bounds__(256) void k_features_finish(Descs ds, const uint32_t* tile_base,
+                                                          int ntiles, int tile, int dtype, int conn,
                                                           const float* raw, float* feat) {
+    const ImgDesc dd = ds.d ? ds.d[blockIdx.x] : ds.one;
+    const float log2px = log2f((float)dd.W * (float)dd.H);
     __shared__ float smax[256];
     const int t0 = tile_base ? (int)tile_base[blockIdx.x] : 0;
     const int t1 = tile_base ? (int)tile_base[blockIdx.x + 1] : ntiles;
@@ -227,9 +231,8 @@ __global__ __launch_bounds__(256) void k_features_finish(const uint32_t* tile_ba
         __syncthreads();
     }
     const float M = smax[0];
-    const float tile_px = (float)tile * (float)((tile + sub - 1) / sub);
     for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
-        normalise(raw + (size_t)t * 16, M, tile_px, tile, dtype, conn, feat + (size_t)t * 16);
+        normalise(raw + (size_t)t * 16, M, tile, dtype, conn, log2px, feat + (size_t)t * 16);
 }
 
 // ------------------------------------------------------------------ MLP ----
@@ -396,8 +399,8 @@ hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base
         else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
     }
     hipLaunchKernelGGL(k_features_tiles, (ntiles + 3) / 4, 256, 0, s, part, nslab, ntiles, raw);
-    hipLaunchKernelGGL(k_features_finish, tile_base ? ds.n : 1, 256, 0, s, tile_base, ntiles, tile,
-                       sub, dtype, conn, raw, feat);
+    hipLaunchKernelGGL(k_features_finish, tile_base ? ds.n : 1, 256, 0, s, ds, tile_base, ntiles,
+                       tile, dtype, conn, raw, feat);
     return hipGetLastError();
 }
 

@@ -16,7 +16,7 @@ Traces.  For one training image, with the schedule's tile size:
   * the tile's mean reconstructed value.
 Labels.  For every training image the whole reconstruction is timed with
 k = 0..4 uniform sweep rounds; every tile of the image gets the fastest k,
-(the fewest rounds among those within 3% of it), and priority_t = mean
+(the fewest rounds among those within 5% of it), and priority_t = mean
 reconstructed value of t / max mask value, so tiles
 whose flood carries the highest values are seeded first (their propagation
 overwrites lower floods, which would otherwise be propagated and then
@@ -45,6 +45,7 @@ META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
 
 MAX_ROUNDS = 4
 FEATURE_SUB = 16  # row subsampling of the features (the library uses the same)
+FEATURE_SET = 2   # feature definitions (2: feature 15 = log2(image pixels) / 32)
 
 
 def schedule_tile(width, height):
@@ -90,10 +91,14 @@ def load_model(path=None):
     meta_path = os.path.splitext(path)[0] + ".json"
     if os.path.exists(meta_path):
         with open(meta_path) as f:
-            sub = json.load(f).get("feature_sub")
+            meta = json.load(f)
+        sub = meta.get("feature_sub")
         if sub is not None and sub != FEATURE_SUB:
             raise ValueError(f"{path}: trained on features of every {sub}th row, "
                              f"the library samples every {FEATURE_SUB}th")
+        if meta.get("feature_set", 1) != FEATURE_SET:
+            raise ValueError(f"{path}: trained on feature set {meta.get('feature_set', 1)}, "
+                             f"the library computes set {FEATURE_SET}")
     return w.astype(np.float32)
 
 
@@ -177,7 +182,7 @@ def trace(mask, marker, connectivity, rounds=MAX_ROUNDS, tile=None):
             "final_mean": sums / px, "maxval": float(mask.max()), "tile": tile}
 
 
-def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=3, workspace=None):
+def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=5, workspace=None):
     """Wall-clock ms of a whole reconstruction (launches, host work and
     synchronisation included) with k uniform sweep rounds, k = 0..rounds,
     best of `trials`, on the GPU."""
@@ -198,7 +203,7 @@ def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=3, workspace=Non
     return out
 
 
-TIE = 1.03  # timings within 3% count as equal (the smaller k wins)
+TIE = 1.05  # timings within 5% count as equal (the smaller k wins)
 
 
 def labels(tr, timings):
@@ -248,9 +253,23 @@ def training_cases(counts=None, seed_base=None):
     from . import U8, U16, synth_mask, synth_marker_hdome, synth_marker_seeds
     from . import workloads as wl
     base = seed_base if seed_base is not None else wl.S0 + 1000
-    n = {"c1": 4, "c2": 6, "c3": 2, "c4": 96, "seeds8k": 6, "hdome8k": 3}
+    n = {"c1": 4, "c2": 6, "c3": 2, "c4": 96, "seeds8k": 6, "hdome8k": 3, "sizes": 1}
     n.update(counts or {})
     out = []
+    # The h-dome (first workload) and seeds (second) generators over a range
+    # of sides, and the uint16 disk (third) and seeds-with-disks (fifth)
+    # generators at smaller sides: the best round count depends on the size.
+    for rep in range(n["sizes"]):
+        for side in (2048, 3072, 6144, 8192, 12288):
+            for fam, gen in (("c1", wl.c1), ("c2", wl.c2)):
+                c = gen(side=side, seed=base + 100 + 31 * rep + side // 512 + (7 if fam == "c2" else 0))
+                out.append((f"{c.name}_{side}", c.mask, c.marker, c.connectivity))
+        for side in (8192, 12288):
+            c = wl.c3(side=side, seed=base + 200 + 31 * rep + side // 1024, ndisks=200 * side * side // 16384 ** 2)
+            out.append((f"{c.name}_{side}", c.mask, c.marker, c.connectivity))
+        for side in (8192, 16384):
+            c = wl.c5(side=side, seed=base + 300 + 31 * rep + side // 1024, ndisks=1000 * side * side // 32768 ** 2)
+            out.append((f"{c.name}_{side}", c.mask, c.marker, c.connectivity))
     for i in range(n["c1"]):
         c = wl.c1(seed=base + 10 + i)
         out.append((c.name, c.mask, c.marker, c.connectivity))
@@ -329,6 +348,7 @@ def main(argv=None):
     for s, L in hist[:: max(1, len(hist) // 10)] + hist[-1:]:
         log(f"step {s}: loss {L:.5f}")
     meta = {"params": MLP_PARAMS, "features": MLP_FEATURES, "feature_sub": FEATURE_SUB,
+            "feature_set": FEATURE_SET,
             "max_rounds": MAX_ROUNDS, "samples": int(X.shape[0]), "images": info,
             "steps": args.steps, "lr": args.lr, "loss_first": hist[0][1],
             "loss_last": hist[-1][1],

@@ -140,10 +140,11 @@ def tile_features_raw(J, I, tile, sub):
     return out
 
 
-def normalise(raw, tile, sub, dtype, conn):
+def normalise(raw, tile, sub, dtype, conn, pixels):
     """The model's inputs from raw statistics; M = the largest max I over all
     tiles; features 1 and 14 carry the connectivity (8 -> 1) and the pixel
-    type (uint16 -> 1)."""
+    type (uint16 -> 1), feature 15 the image's size, min(log2(pixels), 28) / 32.
+    (`sub` no longer enters: kept so that calls name the sampling used.)"""
     raw = np.asarray(raw, dtype=np.float64)
     M = max(raw[:, 15].max(), 1.0)
     n = np.maximum(raw[:, 14], 1.0)
@@ -161,5 +162,5 @@ def normalise(raw, tile, sub, dtype, conn):
     for k in range(10, 14):
         f[:, k] = raw[:, k] / tile / M
     f[:, 14] = 1.0 if np.dtype(dtype).itemsize == 2 else 0.0
-    f[:, 15] = raw[:, 14] / (tile * (-(-tile // sub)))
+    f[:, 15] = min(np.log2(float(pixels)), 28.0) / 32.0
     return f
