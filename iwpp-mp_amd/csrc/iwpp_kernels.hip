@@ -1265,7 +1265,8 @@ template <class T, int VB, int CONN>
 __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_all,
                                                         uint32_t* count) {
     using V = VecIO<T, VB>;
-    constexpr int NW = V::NW, NPX = V::NPX;
+    using L = LanePx<T, VB>;
+    constexpr int NW = V::NW, NPX = V::NPX, NP = L::NP;
     constexpr int LPM = kMP / NPX;          // lanes per megapixel
     constexpr int MPS = 64 / LPM;           // megapixels per strip
     const int lane = lane_id();
@@ -1283,19 +1284,32 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
     const int o = xb0 + lane * VB;
     const int xs0 = xb0 / (int)sizeof(T), xs1 = (xb0 + 64 * VB) / (int)sizeof(T);
     const int y0 = my * kMP, y1 = min(im.H, y0 + kMP);
-    auto load_row = [&](const uint8_t* base, size_t pitch, int y, uint32_t (&px)[NPX]) {
+    // Pixels as the lane's registers (16-bit pairs with 16-byte lanes, pixel k
+    // and k + NP in one register).
+    auto load_row = [&](const uint8_t* base, size_t pitch, int y, uint32_t (&px)[NP]) {
         uint32_t w[NW];
         if (y >= 0 && y < im.H) V::load(base + (size_t)y * pitch, o, rowbytes, w);
         else {
 #pragma unroll
             for (int i = 0; i < NW; ++i) w[i] = 0;
         }
-        V::unpack(w, px);
+        L::split(w, px);
     };
     auto edge = [&](int y, int x) -> uint32_t {
         return (y >= 0 && y < im.H && x >= 0 && x < im.W) ? ldp<T>(im.J, im.pj, x, y) : 0u;
     };
-    uint32_t up[NPX], cur[NPX];
+    // Neighbour of the lane's first register to the left / last to the right:
+    // the adjacent lane's pixel (outside the strip: `fill`) joined with this
+    // lane's own pixel.
+    auto left_of = [&](const uint32_t (&r)[NP], uint32_t fill) -> uint32_t {
+        if constexpr (L::PAIRS) return from_left(r[NP - 1] >> 16, fill) | (r[NP - 1] << 16);
+        else return from_left(r[NP - 1], fill);
+    };
+    auto right_of = [&](const uint32_t (&r)[NP], uint32_t fill) -> uint32_t {
+        if constexpr (L::PAIRS) return (r[0] >> 16) | (from_right(r[0] & 0xFFFFu, fill) << 16);
+        else return from_right(r[0], fill);
+    };
+    uint32_t up[NP], cur[NP];
     load_row(im.J, im.pj, y0 - 1, up);
     load_row(im.J, im.pj, y0, cur);
     uint32_t upL = lane == 0 ? edge(y0 - 1, xs0 - 1) : 0u, upR = lane == 63 ? edge(y0 - 1, xs1) : 0u;
@@ -1323,31 +1337,38 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
             if (g + u >= y1) break;  // the same in every lane
-            uint32_t dn[NPX], msk[NPX];
-            V::unpack(dnw[u], dn);
-            V::unpack(mkw[u], msk);
+            uint32_t dn[NP], msk[NP], m[NP];
+            L::split(dnw[u], dn);
+            L::split(mkw[u], msk);
             const uint32_t dnL = dL[u], dnR = dR[u];
-            // neighbours left of pixel 0 and right of pixel NPX-1 in rows y-1, y, y+1
-            const uint32_t lc = from_left(cur[NPX - 1], cuL), rc = from_right(cur[0], cuR);
-            uint32_t lu = 0, ru = 0, ld = 0, rd = 0;
+            // m = the largest neighbour: 8-connectivity, the 3 x 3 block's
+            // column maxima (the pixel itself included: it cannot raise
+            // itself) maxed over left, centre and right; 4-connectivity, up,
+            // down, left and right.
             if constexpr (CONN == 8) {
-                lu = from_left(up[NPX - 1], upL);
-                ru = from_right(up[0], upR);
-                ld = from_left(dn[NPX - 1], dnL);
-                rd = from_right(dn[0], dnR);
+                uint32_t vcol[NP];
+#pragma unroll
+                for (int k = 0; k < NP; ++k) vcol[k] = L::vmax(up[k], L::vmax(cur[k], dn[k]));
+                const uint32_t vl = left_of(vcol, max(upL, max(cuL, dnL)));
+                const uint32_t vr = right_of(vcol, max(upR, max(cuR, dnR)));
+#pragma unroll
+                for (int k = 0; k < NP; ++k)
+                    m[k] = L::vmax(vcol[k], L::vmax(k > 0 ? vcol[k - 1] : vl, k < NP - 1 ? vcol[k + 1] : vr));
+            } else {
+                const uint32_t cl = left_of(cur, cuL), cr = right_of(cur, cuR);
+#pragma unroll
+                for (int k = 0; k < NP; ++k)
+                    m[k] = L::vmax(L::vmax(up[k], dn[k]),
+                                   L::vmax(k > 0 ? cur[k - 1] : cl, k < NP - 1 ? cur[k + 1] : cr));
+            }
+            // unstable where min(m, mask) > marker in some pixel
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t t = L::vmin(m[k], msk[k]);
+                un |= L::vmax(t, cur[k]) != cur[k];
             }
 #pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                uint32_t m = max(up[k], dn[k]);
-                m = max(m, max(k > 0 ? cur[k - 1] : lc, k < NPX - 1 ? cur[k + 1] : rc));
-                if constexpr (CONN == 8) {
-                    m = max(m, max(k > 0 ? up[k - 1] : lu, k < NPX - 1 ? up[k + 1] : ru));
-                    m = max(m, max(k > 0 ? dn[k - 1] : ld, k < NPX - 1 ? dn[k + 1] : rd));
-                }
-                un |= min(m, msk[k]) > cur[k];
-            }
-#pragma unroll
-            for (int k = 0; k < NPX; ++k) {
+            for (int k = 0; k < NP; ++k) {
                 up[k] = cur[k];
                 cur[k] = dn[k];
             }
