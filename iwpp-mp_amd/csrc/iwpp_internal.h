@@ -33,6 +33,8 @@ enum Ctl : int {
     C_PH_CONVERGE,   //   local sweeps, write-back, notification
     C_PH_STORE,
     C_PH_NOTIFY,
+    C_SEEDED,        // mutex ring: entries put in by the seeding pass (tickets 0..C_SEEDED-1)
+    C_SCLAIM,        // mutex ring: next seeded ticket to claim (claimed in batches)
     C_NSLOTS
 };
 

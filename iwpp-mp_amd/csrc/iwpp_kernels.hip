@@ -1031,20 +1031,30 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
     wave_lds_sync();
 }
 
-// Row sweep over the interior, lane = row.  DIR +1: left to right.
+// Row sweep over the interior, lane = row.  DIR +1: left to right.  The
+// lane's 66 words (halo included) are read into registers first, so the
+// reads issue back to back and the dependent chain runs on registers; the
+// interior goes back to LDS when it changed.  A lane owns its row in a row
+// sweep (its column in a column sweep): no other lane touches those words.
 template <int DIR>
 DEV bool sweep_rows(uint32_t* s, int lane) {
-    int idx = DIR > 0 ? tix(lane, -1) : tix(lane, 64);
-    uint32_t p = s[idx];
+    const int base = tix(lane, -1);
+    uint32_t w[66];
+#pragma unroll
+    for (int c = 0; c < 66; ++c) w[c] = s[base + c];
+    uint32_t p = DIR > 0 ? w[0] : w[65];
     bool ch = false;
-#pragma unroll 16
+#pragma unroll
     for (int k = 0; k < kMP; ++k) {
-        idx += DIR;
-        const uint32_t w = s[idx];
-        const uint32_t nw = upd(w, p);
-        ch |= nw != w;
-        s[idx] = nw;
+        const int c = DIR > 0 ? 1 + k : kMP - k;
+        const uint32_t nw = upd(w[c], p);
+        ch |= nw != w[c];
+        w[c] = nw;
         p = nw;
+    }
+    if (ch) {
+#pragma unroll
+        for (int c = 1; c <= kMP; ++c) s[base + c] = w[c];
     }
     return ch;
 }
@@ -1054,25 +1064,29 @@ DEV bool sweep_rows(uint32_t* s, int lane) {
 // and 63 taking them from the halo columns.
 template <int CONN, int DIR>
 DEV bool sweep_cols(uint32_t* s, int lane) {
-    int idx = DIR > 0 ? tix(-1, lane) : tix(64, lane);
-    int hl = DIR > 0 ? tix(-1, -1) : tix(64, -1);
-    int hr = DIR > 0 ? tix(-1, 64) : tix(64, 64);
-    uint32_t p = s[idx];
+    const int base = tix(-1, lane);
+    uint32_t w[66];
+#pragma unroll
+    for (int r = 0; r < 66; ++r) w[r] = s[base + r * TS];
+    uint32_t p = DIR > 0 ? w[0] : w[65];
     bool ch = false;
-#pragma unroll 16
+#pragma unroll
     for (int k = 0; k < kMP; ++k) {
-        idx += DIR * TS;
+        const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
+        const int rp = r - DIR;                    // ... and of the row above it in sweep order
         uint32_t q = p;
         if constexpr (CONN == 8) {
-            q = pmax(q, pmax(from_left(p, s[hl]), from_right(p, s[hr])));
-            hl += DIR * TS;
-            hr += DIR * TS;
+            const uint32_t hl = s[rp * TS], hr = s[rp * TS + 65];  // halo columns of row rp
+            q = pmax(q, pmax(from_left(p, hl), from_right(p, hr)));
         }
-        const uint32_t w = s[idx];
-        const uint32_t nw = upd(w, q);
-        ch |= nw != w;
-        s[idx] = nw;
+        const uint32_t nw = upd(w[r], q);
+        ch |= nw != w[r];
+        w[r] = nw;
         p = nw;
+    }
+    if (ch) {
+#pragma unroll
+        for (int r = 1; r <= kMP; ++r) s[base + r * TS] = w[r];
     }
     return ch;
 }
@@ -1518,40 +1532,77 @@ DEV bool ring_push_many(const QueueArgs& q, bool mine, uint32_t value) {
     return !wave_any(!ok);
 }
 
+// Taking work.  The seeding pass fills tickets 0..S-1 before the kernel
+// starts; they are claimed in batches of B through their own counter (every
+// one is filled, nothing waits), read and their slots freed at once, and
+// visited one after another -- one atomic on a shared counter per B visits
+// instead of one per visit, which dominated the queue phase when thousands of
+// waves start on a full ring.  Tickets from S on are popped one at a time from
+// C_HEAD as before.  A wave's decrements of `pending` for finished megapixels
+// are held back while it still has claimed work and flushed before it waits
+// for a single pop or leaves: the count is never below the true number of
+// queued and running megapixels, so no wave leaves early.
 template <class T, int CONN>
 __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
     extern __shared__ uint32_t lds[];
+    constexpr int BMAX = 8;
     const int lane = lane_id();
     uint32_t* s = lds + (threadIdx.x >> 6) * (WAVE_LDS / 4);
     uint8_t* rose = reinterpret_cast<uint8_t*>(s + TILE_WORDS);
     unsigned long long visits = 0;
     PhaseTimer pt(q.trace != 0);
+    const uint32_t seeded = a_load(ctl_at(q.ctl, C_SEEDED));
+    const uint32_t nwaves = gridDim.x * (kWG / 64);
+    // batch: about four batches per wave over the seeded region, 1..BMAX
+    const uint32_t B = max(1u, min((uint32_t)BMAX, seeded / (4u * nwaves)));
+    uint32_t claimed = 0;  // lane k < nclaimed holds the k-th claimed megapixel
+    int nclaimed = 0, next = 0;
+    bool seeds_left = seeded > 0;
+    uint32_t owed = 0;     // finished megapixels not yet taken off `pending`
     while (true) {
-        // pop
-        uint32_t h = 0;
-        if (lane == 0) h = a_add(ctl_at(q.ctl, C_HEAD), 1u);
-        h = first_lane(h);
-        const uint32_t slot = h & (q.cap - 1u);
+        uint32_t m = 0;
         int state = 0;  // 1 = got an entry, 2 = done, 3 = timeout
-        uint32_t got = 0;
-        if (lane == 0) {
-            Waiter w(q.timeout_ticks);
-            while (true) {
-                const unsigned long long e = a_load64(&q.ring[slot]);
-                if ((uint32_t)(e >> 32) == h + 1u) { state = 1; got = (uint32_t)e; break; }
-                if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
-                if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u || !w.step(q.ctl)) { state = 3; break; }
+        if (next >= nclaimed && seeds_left) {  // claim the next batch of seeded tickets
+            uint32_t c = 0;
+            if (lane == 0) c = a_add(ctl_at(q.ctl, C_SCLAIM), B);
+            c = first_lane(c);
+            nclaimed = c >= seeded ? 0 : (int)min(B, seeded - c);
+            seeds_left = c + B < seeded;
+            next = 0;
+            claimed = 0;
+            if (lane < nclaimed) {
+                const uint32_t t = c + (uint32_t)lane;
+                const uint32_t slot = t & (q.cap - 1u);
+                claimed = (uint32_t)a_load64(&q.ring[slot]);  // filled before the launch
+                a_store64(&q.ring[slot], (unsigned long long)(t + q.cap) << 32);
             }
         }
-        state = (int)first_lane((uint32_t)state);
-        if (state != 1) break;
-        uint32_t m = 0;
-        if (lane == 0) {
-            m = got;
-            a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
-            a_xchg(&q.st[m], ST_RUNNING);
+        if (next < nclaimed) {
+            m = rdlane(claimed, next++);
+            state = 1;
+        } else {
+            if (owed && lane == 0) a_add(ctl_at(q.ctl, C_PENDING), 0u - owed);
+            owed = 0;
+            uint32_t h = 0;
+            if (lane == 0) h = a_add(ctl_at(q.ctl, C_HEAD), 1u);
+            h = first_lane(h);
+            const uint32_t slot = h & (q.cap - 1u);
+            uint32_t got = 0;
+            if (lane == 0) {
+                Waiter w(q.timeout_ticks);
+                while (true) {
+                    const unsigned long long e = a_load64(&q.ring[slot]);
+                    if ((uint32_t)(e >> 32) == h + 1u) { state = 1; got = (uint32_t)e; break; }
+                    if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
+                    if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u || !w.step(q.ctl)) { state = 3; break; }
+                }
+                if (state == 1) a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
+            }
+            state = (int)first_lane((uint32_t)state);
+            if (state != 1) break;
+            m = first_lane(got);
         }
-        m = first_lane(m);
+        if (lane == 0) a_xchg(&q.st[m], ST_RUNNING);
         mem_wait();
         pt.lap(0);
         const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
@@ -1584,14 +1635,12 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
         uint32_t old = 0;
         if (lane == 0) old = a_and(&q.st[m], ~ST_RUNNING);
         old = first_lane(old);
-        if (old & ST_PENDING) {
-            ok = ring_push_many(q, lane == 0, m) && ok;
-        } else if (lane == 0) {
-            a_add(ctl_at(q.ctl, C_PENDING), 0xFFFFFFFFu);
-        }
+        if (old & ST_PENDING) ok = ring_push_many(q, lane == 0, m) && ok;
+        else ++owed;
         pt.lap(4);
         if (!ok) break;
     }
+    if (owed && lane == 0) a_add(ctl_at(q.ctl, C_PENDING), 0u - owed);
     pt.lap(0);
     pt.flush(q.ctl);
     if (lane == 0 && visits) {
@@ -1725,6 +1774,8 @@ __global__ void k_prepare_queue(QueueArgs q) {
         a_store(ctl_at(q.ctl, C_PENDING), k);
         if (n > q.cap) a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
         a_store(ctl_at(q.ctl, C_TAIL), k);
+        a_store(ctl_at(q.ctl, C_SEEDED), k);  // mutex: seeded tickets are claimed apart,
+        a_store(ctl_at(q.ctl, C_HEAD), k);    // single pops start after them
     }
 }
 
