@@ -400,10 +400,6 @@ int sweep_phase(Run& r, int k0, int k1) {
         }
         return IWPP_OK;
     }
-    if (r.round_masks && k0 == 0) {  // blocks may be skipped: start the scratch plane at 0
-        const ImgDesc& d = r.h[0];  // (after a full round it holds lower bounds already)
-        HIP_TRY(launch_fill_u32((uint32_t*)d.P, 0u, d.pp * d.H / 4, r.s));
-    }
     for (int k = k0; k < k1; ++k) {
         if (r.round_masks) {
             const TileMask tm{r.round_masks + (size_t)k * r.sched_nt, r.sched_tile, r.sched_tile,
@@ -419,10 +415,10 @@ int sweep_phase(Run& r, int k0, int k1) {
             HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0, r.row16,
                                           r.row_wave, nullptr, r.s));
         }
-        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1, true,
-                                      r.wide, nullptr, r.s));
-        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1, false,
-                                      r.wide, nullptr, r.s));
+        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
+                                      COL_IN_PLACE, r.wide, nullptr, r.s));
+        HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
+                                      COL_IN_PLACE, r.wide, nullptr, r.s));
     }
     r.st.sweep_rounds += std::max(0, k1 - k0);
     return IWPP_OK;
@@ -1242,7 +1238,7 @@ extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, siz
     d.R = block_rows > 0 ? block_rows : col_auto_rows(w, h, dtype, wide);
     d.tm = TileMask{nullptr, kMP, kMP, 1};
     ds.total = d.strips * ((h + d.R - 1) / d.R);
-    HIP_TRY(launch_col_pass_batch(ds, dtype, conn, dir, true, wide, (uint32_t*)changed,
+    HIP_TRY(launch_col_pass_batch(ds, dtype, conn, dir, COL_J_TO_P, wide, (uint32_t*)changed,
                                   (hipStream_t)stream));
     return IWPP_OK;
 }

@@ -127,9 +127,11 @@ hipError_t launch_row_pass(const Image& im, int dtype, bool clip, const TileMask
 hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wide,
                                  int wave_rowbytes, uint32_t* changed, hipStream_t s);
 // Column pass over every image's column blocks (ds.total = all waves).
-// j_to_p: read J write P (else read P write J); dir +1 down, -1 up; wide =
-// 16-byte lanes (every image 16-byte aligned), else 4-byte lanes.
-hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, bool j_to_p,
+// mode: COL_J_TO_P read J write P, COL_P_TO_J read P write J, COL_IN_PLACE on
+// J; dir +1 down, -1 up; wide = 16-byte lanes (every image 16-byte aligned),
+// else 4-byte lanes.
+enum ColMode : int { COL_J_TO_P = 0, COL_P_TO_J = 1, COL_IN_PLACE = 2 };
+hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, int mode,
                                  bool wide, uint32_t* changed, hipStream_t s);
 bool col_wide(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
               size_t pi, int W, int dtype);

@@ -789,10 +789,10 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
 // reading src and writing dst (out of place).  Inside the block each row
 // uses the block's previous output row; neighbours outside the block come
 // from src, so the pass is deterministic.
-template <class T, int VB, int CONN, int DIR>
-DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
-                   const uint8_t* __restrict__ I, size_t ps, size_t pd, size_t pi, int W, int H,
-                   int R, int strip, int seg, const TileMask& tm, uint32_t* changed) {
+template <class T, int VB, int CONN, int DIR, bool INPLACE>
+DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, size_t ps,
+                   size_t pd, size_t pi, int W, int H, int R, int strip, int seg,
+                   const TileMask& tm, uint32_t* changed) {
     using V = VecIO<T, VB>;
     using L = LanePx<T, VB>;
     constexpr int NW = V::NW, NP = L::NP;
@@ -807,9 +807,9 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
     const bool on = o < rowbytes ? tile_on(tm, o / (int)sizeof(T), y0) : false;
     // With a tile schedule, a block whose tiles are all off (at its first row)
     // is skipped in both directions of a round (the same test in both), so its
-    // marker keeps its value; blocks reading its rows from the scratch plane
-    // then see an older value of the same pixels, a valid lower bound (the
-    // plane is zeroed before the first scheduled round).
+    // marker keeps its value, which neighbouring blocks read as a lower bound.
+    // (Out of place, a skipped block leaves the destination plane as it was:
+    // only the in-place mode is used with tile schedules.)
     if (tm.active && !wave_any(on)) return;
     // The previous output row, as the lane's pixel registers (16-bit pairs
     // with 16-byte lanes: pixel k and k + NP in one register).
@@ -892,17 +892,25 @@ DEV void col_block(const uint8_t* __restrict__ S, uint8_t* __restrict__ D,
                 prev[k] = out;
             }
             any_change |= d;
-            uint32_t ow[NW];
-            L::join(prev, ow);
-            V::store(D + (size_t)y * pd, o, rowbytes, ow);
+            if (!INPLACE || d) {  // in place: only vectors that rose are written
+                uint32_t ow[NW];
+                L::join(prev, ow);
+                V::store(D + (size_t)y * pd, o, rowbytes, ow);
+            }
         }
     }
     if (any_change && changed) *changed = 1u;
 }
 
-// Global wave w -> (image, strip, segment).  JP: read J write P, else read P
-// write J.
-template <class T, int VB, int CONN, int DIR, bool JP>
+// Global wave w -> (image, strip, segment).  MODE 0: read J write P (every
+// pixel: the result of the blocked pass, neighbours outside a block from J);
+// 1: read P write J; 2: in place on J, writing only the vectors that rose.
+// In place, the rows just outside a block and the pixels just outside a
+// strip belong to other waves and may be read before or after those waves
+// raise them: either value is a lower bound of the reconstruction, so the
+// reconstruction is unchanged (only the order of propagation is), and every
+// unchanged vector saves its write.
+template <class T, int VB, int CONN, int DIR, int MODE>
 __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ds.total) return;
@@ -910,8 +918,12 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
     if (ds.round >= d.rounds) return;  // this image's sweep rounds are done
     const int local = w - d.cw_base;
     const int strip = local % d.strips, seg = local / d.strips;
-    if (JP) col_block<T, VB, CONN, DIR>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
-    else col_block<T, VB, CONN, DIR>(d.P, d.J, d.I, d.pp, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+    if constexpr (MODE == 0)
+        col_block<T, VB, CONN, DIR, false>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+    else if constexpr (MODE == 1)
+        col_block<T, VB, CONN, DIR, false>(d.P, d.J, d.I, d.pp, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+    else
+        col_block<T, VB, CONN, DIR, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
@@ -1942,31 +1954,35 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
     return hipGetLastError();
 }
 
-template <class T, int VB>
-static void col_pass_t(const Descs& ds, int conn, int dir, bool jp, uint32_t* changed,
-                       hipStream_t s) {
+template <class T, int VB, int CONN, int DIR>
+static void col_pass_dir(const Descs& ds, int mode, uint32_t* changed, hipStream_t s) {
     const int grid = (ds.total + 3) / 4;
-#define IWPP_COL(C, D, JP) \
-    hipLaunchKernelGGL((k_col_pass<T, VB, C, D, JP>), grid, kWG, 0, s, ds, changed)
-    if (conn == 8) {
-        if (dir > 0) { if (jp) IWPP_COL(8, 1, true); else IWPP_COL(8, 1, false); }
-        else { if (jp) IWPP_COL(8, -1, true); else IWPP_COL(8, -1, false); }
-    } else {
-        if (dir > 0) { if (jp) IWPP_COL(4, 1, true); else IWPP_COL(4, 1, false); }
-        else { if (jp) IWPP_COL(4, -1, true); else IWPP_COL(4, -1, false); }
-    }
-#undef IWPP_COL
+    if (mode == 0) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 0>), grid, kWG, 0, s, ds, changed);
+    else if (mode == 1) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 1>), grid, kWG, 0, s, ds, changed);
+    else hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 2>), grid, kWG, 0, s, ds, changed);
 }
 
-hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, bool j_to_p,
+template <class T, int VB>
+static void col_pass_t(const Descs& ds, int conn, int dir, int mode, uint32_t* changed,
+                       hipStream_t s) {
+    if (conn == 8) {
+        if (dir > 0) col_pass_dir<T, VB, 8, 1>(ds, mode, changed, s);
+        else col_pass_dir<T, VB, 8, -1>(ds, mode, changed, s);
+    } else {
+        if (dir > 0) col_pass_dir<T, VB, 4, 1>(ds, mode, changed, s);
+        else col_pass_dir<T, VB, 4, -1>(ds, mode, changed, s);
+    }
+}
+
+hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, int mode,
                                  bool wide, uint32_t* changed, hipStream_t s) {
     if (ds.total <= 0) return hipSuccess;
     if (dtype == 1) {
-        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, j_to_p, changed, s);
-        else col_pass_t<uint8_t, 4>(ds, conn, dir, j_to_p, changed, s);
+        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, mode, changed, s);
+        else col_pass_t<uint8_t, 4>(ds, conn, dir, mode, changed, s);
     } else {
-        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, j_to_p, changed, s);
-        else col_pass_t<uint16_t, 4>(ds, conn, dir, j_to_p, changed, s);
+        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, mode, changed, s);
+        else col_pass_t<uint16_t, 4>(ds, conn, dir, mode, changed, s);
     }
     return hipGetLastError();
 }
