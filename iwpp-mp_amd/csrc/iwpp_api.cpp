@@ -1,6 +1,6 @@
 // Host driver and C ABI (include/iwpp.h).  Orchestrates, on one HIP stream:
-//   clip (folded into the first row pass) -> sweep rounds (row pass in place,
-//   column pass down into a scratch plane, column pass up back) -> seeding
+//   clip (folded into the first row pass) -> sweep rounds (row pass, column
+//   pass down, column pass up, all in place) -> seeding
 //   (unstable megapixels compacted into the queue) -> queue phase (mutex /
 //   DQ / UQ) -> overflow restart from the partially reconstructed marker.
 // The restart mirrors the reference's do { ... } while (memLeak) loop
@@ -72,8 +72,6 @@ hipError_t grow(P*& p, size_t& have, size_t need_elems) {
 
 struct iwpp_workspace {
     int device = -1;
-    uint8_t* plane = nullptr;   // scratch plane for the out-of-place column passes
-    size_t plane_n = 0;
     uint32_t* st = nullptr;     // per-megapixel state
     size_t st_n = 0;
     uint8_t* flags = nullptr;   // per-megapixel unstable flags
@@ -131,8 +129,11 @@ extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtyp
         return nullptr;
     }
     if (max_pixels && max_dtype >= 1 && max_dtype <= 2) {
-        if (grow(ws->plane, ws->plane_n, max_pixels * (size_t)max_dtype + 4096) != hipSuccess) {
-            delete ws;
+        // per-megapixel state, flags and the ring for a single image of
+        // max_pixels pixels (edge megapixels of any shape included)
+        const size_t nmp = max_pixels / ((size_t)kMP * kMP) + 2 * (size_t)std::sqrt((double)max_pixels) / kMP + 4;
+        if (grow(ws->st, ws->st_n, nmp) != hipSuccess || grow(ws->flags, ws->flags_n, nmp) != hipSuccess) {
+            iwpp_workspace_destroy(ws);
             return nullptr;
         }
     }
@@ -141,7 +142,6 @@ extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtyp
 
 extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     if (!ws) return;
-    (void)hipFree(ws->plane);
     (void)hipFree(ws->st);
     (void)hipFree(ws->flags);
     (void)hipFree(ws->qa);
@@ -266,19 +266,10 @@ unsigned long long timeout_ticks(double seconds) {
     return (unsigned long long)(seconds * khz * 1000.0);
 }
 
-// Builds descriptors, scratch planes and the device-side image tables.
+// Builds descriptors and the device-side image tables.
 int setup_images(Run& r, const iwpp_image* imgs, int n) {
     iwpp_workspace* ws = r.ws;
     r.h.resize(n);
-    // scratch planes for the column passes, one after the other
-    size_t plane_bytes = 0;
-    std::vector<size_t> poff(n);
-    for (int i = 0; i < n; ++i) {
-        poff[i] = plane_bytes;
-        plane_bytes += round_up((size_t)imgs[i].width * r.dtype, 256) * imgs[i].height + 256;
-    }
-    HIP_TRY(grow(ws->plane, ws->plane_n, plane_bytes + 256));
-    uint8_t* plane0 = (uint8_t*)round_up((uintptr_t)ws->plane, 256);
     r.wide = true;
     bool aligned = true;  // every plane 16-byte aligned (base and pitch)
     double useful = 0, lanes16 = 0, chunks16 = 0;
@@ -288,18 +279,18 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         d = ImgDesc{};
         d.J = (uint8_t*)imgs[i].marker;
         d.I = (const uint8_t*)imgs[i].mask;
-        d.P = plane0 + poff[i];
+        d.P = nullptr;  // the column passes of a reconstruction run in place
         d.pj = imgs[i].pitch_marker;
         d.pi = imgs[i].pitch_mask;
-        d.pp = round_up((size_t)imgs[i].width * r.dtype, 256);
+        d.pp = 0;
         d.W = imgs[i].width;
         d.H = imgs[i].height;
         d.nmx = (d.W + kMP - 1) / kMP;
         d.nmy = (d.H + kMP - 1) / kMP;
         d.tm = TileMask{nullptr, kMP, kMP, 1};
-        r.wide = r.wide && col_wide(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, r.dtype);
+        r.wide = r.wide && col_wide(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, r.dtype);
         aligned = aligned && (uintptr_t)d.J % 16 == 0 && (uintptr_t)d.I % 16 == 0 &&
-                  (uintptr_t)d.P % 16 == 0 && d.pj % 16 == 0 && d.pi % 16 == 0 && d.pp % 16 == 0;
+                  d.pj % 16 == 0 && d.pi % 16 == 0;
         const double rb = (double)d.W * r.dtype;
         useful += rb * d.H;
         lanes16 += std::ceil(rb / 1024.0) * 1024.0 * d.H;  // 64 lanes x 16 B per column strip
