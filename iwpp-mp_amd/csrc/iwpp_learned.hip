@@ -74,32 +74,47 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
 #pragma unroll
     for (int k = 0; k < kStats; ++k) acc[k] = 0.f;
     const int items = kSlab * nvec;
-    for (int i = threadIdx.x; i < items; i += blockDim.x) {
-        const int r = i / nvec, v = i % nvec;
-        const int y = y0 + (sl * kSlab + r) * sub;
-        if (y >= y1) continue;
+    // A thread's items (at most IPT: tile <= 2048 pixels of <= 2 bytes, 16-byte
+    // vectors, 8 rows, 256 threads) are all loaded before any is used, from
+    // addresses clamped into the row, the bytes past the row cleared after:
+    // no branch around a load, so the loads are in flight together.
+    constexpr int IPT = kSlab * (2048 * 2 / VB) / 256;
+    for (int base = 0; base < items; base += IPT * (int)blockDim.x) {  // once for tiles <= 2048
+    uint32_t jv[IPT][VB / 4], iv[IPT][VB / 4];
+    int ov[IPT];
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) {
+        const int i = base + threadIdx.x + u * (int)blockDim.x;
+        const int r = i / max(nvec, 1), v = i - r * max(nvec, 1);
+        const int y = min(y0 + (sl * kSlab + r) * sub, im.H - 1);
         const int o = bx0 + v * VB;  // VB-aligned (tiles start at multiples of 64 px)
-        uint32_t jw[VB / 4], iw[VB / 4];
-        const uint8_t* jr = im.J + (size_t)y * im.pj + o;
-        const uint8_t* ir = im.I + (size_t)y * im.pi + o;
-        if (o + VB > rowbytes) {  // row end: pixel by pixel, nothing past the row
-#pragma unroll
-            for (int w = 0; w < VB / 4; ++w) jw[w] = iw[w] = 0;
-#pragma unroll
-            for (int k = 0; k < NPX; ++k) {
-                if (o + (k + 1) * (int)sizeof(T) <= rowbytes) {
-                    jw[k / PPW] |= (uint32_t)*reinterpret_cast<const T*>(jr + k * sizeof(T)) << (BITS * (k % PPW));
-                    iw[k / PPW] |= (uint32_t)*reinterpret_cast<const T*>(ir + k * sizeof(T)) << (BITS * (k % PPW));
-                }
-            }
-        } else if constexpr (VB == 16) {
+        const bool live = i < items && y0 + (sl * kSlab + r) * sub < y1;
+        ov[u] = live ? o : -1;
+        const int oc = (live && o < rowbytes) ? o : 0;
+        const uint8_t* jr = im.J + (size_t)y * im.pj + oc;
+        const uint8_t* ir = im.I + (size_t)y * im.pi + oc;
+        if constexpr (VB == 16) {
             const uint4 a = *reinterpret_cast<const uint4*>(jr);
             const uint4 b = *reinterpret_cast<const uint4*>(ir);
-            jw[0] = a.x; jw[1] = a.y; jw[2] = a.z; jw[3] = a.w;
-            iw[0] = b.x; iw[1] = b.y; iw[2] = b.z; iw[3] = b.w;
+            jv[u][0] = a.x; jv[u][1] = a.y; jv[u][2] = a.z; jv[u][3] = a.w;
+            iv[u][0] = b.x; iv[u][1] = b.y; iv[u][2] = b.z; iv[u][3] = b.w;
         } else {
-            jw[0] = *reinterpret_cast<const uint32_t*>(jr);
-            iw[0] = *reinterpret_cast<const uint32_t*>(ir);
+            jv[u][0] = *reinterpret_cast<const uint32_t*>(jr);
+            iv[u][0] = *reinterpret_cast<const uint32_t*>(ir);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) {
+        const int o = ov[u];
+        if (o < 0) continue;
+        uint32_t jw[VB / 4], iw[VB / 4];
+        const int valid = rowbytes - o;  // bytes of this vector inside the row
+#pragma unroll
+        for (int w = 0; w < VB / 4; ++w) {
+            const int vb = valid - 4 * w;
+            const uint32_t keep = vb >= 4 ? 0xFFFFFFFFu : vb <= 0 ? 0u : (0xFFFFFFFFu >> (8 * (4 - vb)));
+            jw[w] = jv[u][w] & keep;
+            iw[w] = iv[u][w] & keep;
         }
         const int xv = o / (int)sizeof(T);  // first pixel of the vector
         float prev_i = 0.f;
@@ -123,6 +138,7 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
             acc[14] += 1.f;
             acc[15] = fmaxf(acc[15], c);
         }
+    }
     }
     // border inflow: rows y0 / y1-1 in the first / last slab, the left and
     // right columns over this slab's band of image rows
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
 // Normalised features from raw statistics.  M = max mask value over all tiles
 // (0 -> 1); tile_px = sampled pixels of a full tile.  Features 1 and 14 carry
 // the connectivity (8 -> 1, 4 -> 0) and the pixel type (uint16 -> 1, uint8 -> 0).
-DEV void normalise(const float* r, float M, int tile, int dtype, int conn, float log2px,
+DEV void normalise(const float* r, float M, int tile, int sub, int dtype, int conn, float log2px,
                    float* f) {
     const float n = fmaxf(r[14], 1.f);
     const float inv = 1.f / fmaxf(M, 1.f);
@@ -178,6 +194,7 @@ DEV void normalise(const float* r, float M, int tile, int dtype, int conn, float
     f[7] = r[7] / n * 100.f;
     f[8] = r[8] * inv;
     f[9] = r[9] / n * inv * 10.f;
+    (void)sub;
     const float side = (float)tile;
     f[10] = r[10] / side * inv;
     f[11] = r[11] / side * inv;
@@ -215,8 +232,8 @@ __global__ __launch_bounds__(256) void k_features_tiles(const float* part, int n
 // One workgroup per image: M = the image's largest mask value over its tiles'
 // raw statistics, then its tiles' normalised features.
 __global__ __launch_bounds__(256) void k_features_finish(Descs ds, const uint32_t* tile_base,
-                                                          int ntiles, int tile, int dtype, int conn,
-                                                          const float* raw, float* feat) {
+                                                          int ntiles, int tile, int sub, int dtype,
+                                                          int conn, const float* raw, float* feat) {
     const ImgDesc dd = ds.d ? ds.d[blockIdx.x] : ds.one;
     const float log2px = log2f((float)dd.W * (float)dd.H);
     __shared__ float smax[256];
@@ -232,7 +249,7 @@ __global__ __launch_bounds__(256) void k_features_finish(Descs ds, const uint32_
     }
     const float M = smax[0];
     for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
-        normalise(raw + (size_t)t * 16, M, tile, dtype, conn, log2px, feat + (size_t)t * 16);
+        normalise(raw + (size_t)t * 16, M, tile, sub, dtype, conn, log2px, feat + (size_t)t * 16);
 }
 
 // ------------------------------------------------------------------ MLP ----
@@ -400,7 +417,7 @@ hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base
     }
     hipLaunchKernelGGL(k_features_tiles, (ntiles + 3) / 4, 256, 0, s, part, nslab, ntiles, raw);
     hipLaunchKernelGGL(k_features_finish, tile_base ? ds.n : 1, 256, 0, s, ds, tile_base, ntiles,
-                       tile, dtype, conn, raw, feat);
+                       tile, sub, dtype, conn, raw, feat);
     return hipGetLastError();
 }
 
