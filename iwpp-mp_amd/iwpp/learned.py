@@ -324,27 +324,105 @@ def build_dataset(cases, rounds=MAX_ROUNDS, log=None):
     return np.concatenate(X), np.concatenate(Y), info
 
 
+def validation_cases(seed_base=None):
+    """Held-out images for choosing between trained models: the h-dome and
+    seeds generators at 6144^2 and 12288^2, the uint16 disk generator at
+    12288^2 and the seeds-with-disks generator at 16384^2, with seeds that
+    neither the training set nor the tests use."""
+    from . import workloads as wl
+    base = seed_base if seed_base is not None else wl.S0 + 3000
+    out = []
+    for side in (6144, 12288):
+        out.append(wl.c1(side=side, seed=base + side // 1024))
+        out.append(wl.c2(side=side, seed=base + 50 + side // 1024))
+    out.append(wl.c3(side=12288, seed=base + 100, ndisks=112))
+    out.append(wl.c5(side=16384, seed=base + 101, ndisks=250))
+    return out
+
+
+def validation_ms(w, cases, reps=3):
+    """Total wall-clock ms of reconstructing `cases` under the learned
+    schedule with weights w (median of reps per image)."""
+    torch = _torch()
+    import statistics
+    from . import Workspace as _W
+    set_model(w)
+    ws = _W()
+    total = 0.0
+    for c in cases:
+        I = DeviceImage.from_numpy(c.mask)
+        J0 = DeviceImage.from_numpy(c.marker)
+        J = DeviceImage(J0.h, J0.w, J0.dtype, J0.pitch)
+        ts = []
+        for _ in range(reps + 1):
+            J.buf.copy_(J0.buf)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reconstruct(J, I, connectivity=c.connectivity, queue="auto", schedule="learned",
+                        workspace=ws)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        total += statistics.median(ts[1:])
+    set_model(None)
+    return total
+
+
 def main(argv=None):
     """Train the schedule model on the GPU and write the model files:
-        python -m iwpp.learned [--out DIR] [--steps N] [--lr LR]"""
+        python -m iwpp.learned [--out DIR] [--steps N] [--lr LR] [--inits K]
+    With K > 1, K networks from different initial weights are trained on the
+    same data and the one with the lowest held-out reconstruction time
+    (validation_cases) is kept."""
     import argparse
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--out", default=MODEL_DIR)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--c4", type=int, default=96)
+    ap.add_argument("--inits", type=int, default=1)
+    ap.add_argument("--relabel", default=None,
+                    help="reuse DIR/sched_dataset.npz and the per-image timings of "
+                         "DIR/sched_mlp.json (labels recomputed with the current rule)")
     args = ap.parse_args(argv)
 
     def log(msg):
         print(msg, flush=True)
 
     t0 = time.time()
-    cases = training_cases({"c4": args.c4})
-    log(f"{len(cases)} training images generated in {time.time() - t0:.1f} s")
-    X, Y, info = build_dataset(cases, log=lambda m: None if " c4_" in m else log(m))
+    if args.relabel:
+        d = np.load(os.path.join(args.relabel, "sched_dataset.npz"))
+        with open(os.path.join(args.relabel, "sched_mlp.json")) as f:
+            info = json.load(f)["images"]
+        X, Y = d["X"], d["Y"].copy()
+        pos = 0
+        for im in info:
+            t = np.asarray(im["ms_by_rounds"], dtype=np.float64)
+            k = int(np.flatnonzero(t <= TIE * t.min())[0])
+            Y[pos:pos + im["tiles"], 0] = k
+            im["k"] = k
+            pos += im["tiles"]
+        assert pos == X.shape[0], "dataset and timings disagree"
+        log(f"relabelled {len(info)} images from {args.relabel}")
+    else:
+        cases = training_cases({"c4": args.c4})
+        log(f"{len(cases)} training images generated in {time.time() - t0:.1f} s")
+        X, Y, info = build_dataset(cases, log=lambda m: None if " c4_" in m else log(m))
     log(f"dataset: {X.shape[0]} tiles, k* histogram "
         f"{np.bincount(Y[:, 0].astype(int), minlength=MAX_ROUNDS + 1).tolist()}")
-    w, hist = train(X, Y, steps=args.steps, lr=args.lr)
+    candidates = []
+    for seed in range(max(1, args.inits)):
+        w, hist = train(X, Y, steps=args.steps, lr=args.lr, seed=seed)
+        log(f"init {seed}: loss {hist[0][1]:.5f} -> {hist[-1][1]:.5f}")
+        candidates.append((seed, w, hist))
+    chosen, val = 0, None
+    if len(candidates) > 1:
+        vcases = validation_cases()
+        scores = [validation_ms(w, vcases) for _, w, _ in candidates]
+        for (seed, _, _), v in zip(candidates, scores):
+            log(f"init {seed}: held-out reconstruction time {v:.3f} ms")
+        chosen = int(np.argmin(scores))
+        val = [round(v, 4) for v in scores]
+    seed, w, hist = candidates[chosen]
     for s, L in hist[:: max(1, len(hist) // 10)] + hist[-1:]:
         log(f"step {s}: loss {L:.5f}")
     meta = {"params": MLP_PARAMS, "features": MLP_FEATURES, "feature_sub": FEATURE_SUB,
@@ -353,7 +431,8 @@ def main(argv=None):
             "steps": args.steps, "lr": args.lr, "loss_first": hist[0][1],
             "loss_last": hist[-1][1],
             "label_k_histogram": np.bincount(Y[:, 0].astype(int),
-                                             minlength=MAX_ROUNDS + 1).tolist()}
+                                             minlength=MAX_ROUNDS + 1).tolist(),
+            "init_seed": seed, "validation_ms_per_init": val}
     save_model(w, meta, os.path.join(args.out, "sched_mlp.f32"))
     np.savez_compressed(os.path.join(args.out, "sched_dataset.npz"), X=X, Y=Y)
     log(f"model written to {args.out} ({time.time() - t0:.1f} s)")
