@@ -78,8 +78,8 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
     // vectors, 8 rows, 256 threads) are all loaded before any is used, from
     // addresses clamped into the row, the bytes past the row cleared after:
     // no branch around a load, so the loads are in flight together.
-    constexpr int IPT = kSlab * (2048 * 2 / VB) / 256;
-    for (int base = 0; base < items; base += IPT * (int)blockDim.x) {  // once for tiles <= 2048
+    constexpr int IPT = kSlab * (2048 * (int)sizeof(T) / VB) / 256;  // 2048-pixel tiles in one go
+    for (int base = 0; base < items; base += IPT * (int)blockDim.x) {  // once for tiles <= 2048 px
     uint32_t jv[IPT][VB / 4], iv[IPT][VB / 4];
     int ov[IPT];
 #pragma unroll
