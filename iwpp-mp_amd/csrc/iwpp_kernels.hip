@@ -986,36 +986,45 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
     const int wl = lane % WPR, rl = lane / WPR;
     const int xw = x0 + wl * PPW;
     const bool xin = xw < im.W;
-    // All loads first, unconditionally from clamped (always valid) addresses so
-    // that they issue back to back; out-of-image words are zeroed afterwards.
+    // All loads of a batch first, unconditionally from clamped (always valid)
+    // addresses so that they issue back to back; out-of-image words are
+    // zeroed afterwards.  uint16 tiles take two batches (one would hold 66
+    // registers of loads and halve the waves per SIMD).
     constexpr int NIT = (66 + RPI - 1) / RPI;
-    uint32_t gj[NIT], gi[NIT];
+    constexpr int NB = sizeof(T) == 1 ? 1 : 2, BIT = (NIT + NB - 1) / NB;
     const size_t xoff = (size_t)(xin ? xw : x0) * sizeof(T);
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int y = y0 - 1 + it * RPI + rl;
-        const int yc = min(max(y, 0), im.H - 1);
-        gj[it] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
-        gi[it] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff));
-    }
+    for (int b = 0; b < NB; ++b) {
+        uint32_t gj[BIT], gi[BIT];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int row = -1 + it * RPI + rl;
-        const int y = y0 + row;
-        const bool valid = row <= 64 && xin && y >= 0 && y < im.H;
-        gj[it] = valid ? gj[it] : 0u;
-        gi[it] = valid ? gi[it] : 0u;
-    }
+        for (int u = 0; u < BIT; ++u) {
+            const int it = b * BIT + u;
+            const int y = y0 - 1 + it * RPI + rl;
+            const int yc = min(max(y, 0), im.H - 1);
+            gj[u] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
+            gi[u] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff));
+        }
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int row = -1 + it * RPI + rl;
-        if (row > 64) continue;
+        for (int u = 0; u < BIT; ++u) {
+            const int it = b * BIT + u;
+            const int row = -1 + it * RPI + rl;
+            const int y = y0 + row;
+            const bool valid = it < NIT && row <= 64 && xin && y >= 0 && y < im.H;
+            gj[u] = valid ? gj[u] : 0u;
+            gi[u] = valid ? gi[u] : 0u;
+        }
 #pragma unroll
-        for (int k = 0; k < PPW; ++k) {
-            const bool in = xw + k < im.W;
-            const uint32_t j = in ? (gj[it] >> (BITS * k)) & PM : 0u;
-            const uint32_t m = in ? (gi[it] >> (BITS * k)) & PM : 0u;
-            s[tix(row, wl * PPW + k)] = j | (m << 16);
+        for (int u = 0; u < BIT; ++u) {
+            const int it = b * BIT + u;
+            const int row = -1 + it * RPI + rl;
+            if (it >= NIT || row > 64) continue;
+#pragma unroll
+            for (int k = 0; k < PPW; ++k) {
+                const bool in = xw + k < im.W;
+                const uint32_t j = in ? (gj[u] >> (BITS * k)) & PM : 0u;
+                const uint32_t m = in ? (gi[u] >> (BITS * k)) & PM : 0u;
+                s[tix(row, wl * PPW + k)] = j | (m << 16);
+            }
         }
     }
     // halo columns x0-1 and x0+64, rows -1..64 (loads clamped the same way)
@@ -1043,83 +1052,98 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
     wave_lds_sync();
 }
 
-// Row sweep over the interior, lane = row.  DIR +1: left to right.  The
-// lane's 66 words (halo included) are read into registers first, so the
-// reads issue back to back and the dependent chain runs on registers; the
-// interior goes back to LDS when it changed.  A lane owns its row in a row
-// sweep (its column in a column sweep): no other lane touches those words.
-template <int DIR>
-DEV bool sweep_rows(uint32_t* s, int lane) {
+// Row sweeps over the interior, lane = row: left to right, then right to
+// left.  The lane's 66 words (halo included) are read into registers first,
+// so the reads issue back to back, both dependent chains run on registers,
+// and the interior goes back to LDS once, if it changed.  A lane owns its row
+// in a row sweep (its column in a column sweep): no other lane touches those
+// words.  ch[0], ch[1]: whether each direction changed a pixel.
+DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2]) {
     const int base = tix(lane, -1);
     uint32_t w[66];
 #pragma unroll
     for (int c = 0; c < 66; ++c) w[c] = s[base + c];
-    uint32_t p = DIR > 0 ? w[0] : w[65];
-    bool ch = false;
+    ch[0] = ch[1] = false;
+    uint32_t p = w[0];
 #pragma unroll
-    for (int k = 0; k < kMP; ++k) {
-        const int c = DIR > 0 ? 1 + k : kMP - k;
+    for (int c = 1; c <= kMP; ++c) {
         const uint32_t nw = upd(w[c], p);
-        ch |= nw != w[c];
+        ch[0] |= nw != w[c];
         w[c] = nw;
         p = nw;
     }
-    if (ch) {
+    p = w[65];
+#pragma unroll
+    for (int c = kMP; c >= 1; --c) {
+        const uint32_t nw = upd(w[c], p);
+        ch[1] |= nw != w[c];
+        w[c] = nw;
+        p = nw;
+    }
+    if (ch[0] || ch[1]) {
 #pragma unroll
         for (int c = 1; c <= kMP; ++c) s[base + c] = w[c];
     }
-    return ch;
 }
 
-// Column sweep over the interior, lane = column.  DIR +1: top to bottom.
-// 8-connectivity adds the previous row's left/right neighbours (DPP), lanes 0
-// and 63 taking them from the halo columns.
-template <int CONN, int DIR>
-DEV bool sweep_cols(uint32_t* s, int lane) {
+// Column sweeps over the interior, lane = column: top to bottom, then bottom
+// to top, the same way.  8-connectivity adds the previous row's left/right
+// neighbours (DPP), lanes 0 and 63 taking them from the halo columns.
+template <int CONN>
+DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2]) {
     const int base = tix(-1, lane);
     uint32_t w[66];
 #pragma unroll
     for (int r = 0; r < 66; ++r) w[r] = s[base + r * TS];
-    uint32_t p = DIR > 0 ? w[0] : w[65];
-    bool ch = false;
+    ch[0] = ch[1] = false;
 #pragma unroll
-    for (int k = 0; k < kMP; ++k) {
-        const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
-        const int rp = r - DIR;                    // ... and of the row above it in sweep order
-        uint32_t q = p;
-        if constexpr (CONN == 8) {
-            const uint32_t hl = s[rp * TS], hr = s[rp * TS + 65];  // halo columns of row rp
-            q = pmax(q, pmax(from_left(p, hl), from_right(p, hr)));
+    for (int d = 0; d < 2; ++d) {
+        const int DIR = d == 0 ? 1 : -1;
+        uint32_t p = d == 0 ? w[0] : w[65];
+#pragma unroll
+        for (int k = 0; k < kMP; ++k) {
+            const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
+            const int rp = r - DIR;                    // ... and of the row before it
+            uint32_t q = p;
+            if constexpr (CONN == 8) {
+                const uint32_t hl = s[rp * TS], hr = s[rp * TS + 65];  // halo columns of row rp
+                q = pmax(q, pmax(from_left(p, hl), from_right(p, hr)));
+            }
+            const uint32_t nw = upd(w[r], q);
+            ch[d] |= nw != w[r];
+            w[r] = nw;
+            p = nw;
+            // keep the scheduler from hoisting every halo read of the sweep
+            // (128 live values would halve the waves per SIMD)
+            if constexpr (CONN == 8)
+                if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
         }
-        const uint32_t nw = upd(w[r], q);
-        ch |= nw != w[r];
-        w[r] = nw;
-        p = nw;
     }
-    if (ch) {
+    if (ch[0] || ch[1]) {
 #pragma unroll
         for (int r = 1; r <= kMP; ++r) s[base + r * TS] = w[r];
     }
-    return ch;
 }
 
-// Sweeps until the four directions in a row change nothing (a sweep is
-// idempotent, so three quiet sweeps after a change suffice), or max_sweeps.
+// Sweeps until the four directions in a row change nothing, or max_sweeps:
+// rows forward, rows backward, columns forward, columns backward, in turn.
+// Each sweep is idempotent, so three quiet sweeps after the last change mean
+// every direction leaves the tile as it is.  Two sweeps share one pass over
+// the registers.
 template <int CONN>
 DEV bool tile_converge(uint32_t* s, int max_sweeps) {
     const int lane = lane_id();
     int quiet = -1;
-    for (int k = 0; k < max_sweeps; ++k) {
-        bool ch;
-        switch (k & 3) {
-            case 0: ch = sweep_rows<+1>(s, lane); break;
-            case 1: ch = sweep_cols<CONN, +1>(s, lane); break;
-            case 2: ch = sweep_rows<-1>(s, lane); break;
-            default: ch = sweep_cols<CONN, -1>(s, lane); break;
-        }
+    for (int k = 0; k < max_sweeps; k += 2) {
+        bool ch[2];
+        if ((k & 2) == 0) sweep_rows_pair(s, lane, ch);
+        else sweep_cols_pair<CONN>(s, lane, ch);
         wave_lds_sync();
-        if (wave_any(ch)) quiet = 0;
-        else if (++quiet >= 3) return true;
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            if (wave_any(ch[d])) quiet = 0;
+            else if (++quiet >= 3) return true;
+        }
     }
     return false;
 }
