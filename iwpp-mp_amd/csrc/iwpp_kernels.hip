@@ -1129,11 +1129,12 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2]) {
 // rows forward, rows backward, columns forward, columns backward, in turn.
 // Each sweep is idempotent, so three quiet sweeps after the last change mean
 // every direction leaves the tile as it is.  Two sweeps share one pass over
-// the registers.
+// the registers.  `changed`: some sweep raised a pixel.
 template <int CONN>
-DEV bool tile_converge(uint32_t* s, int max_sweeps) {
+DEV bool tile_converge(uint32_t* s, int max_sweeps, bool& changed) {
     const int lane = lane_id();
     int quiet = -1;
+    changed = false;
     for (int k = 0; k < max_sweeps; k += 2) {
         bool ch[2];
         if ((k & 2) == 0) sweep_rows_pair(s, lane, ch);
@@ -1141,11 +1142,79 @@ DEV bool tile_converge(uint32_t* s, int max_sweeps) {
         wave_lds_sync();
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            if (wave_any(ch[d])) quiet = 0;
-            else if (++quiet >= 3) return true;
+            if (wave_any(ch[d])) {
+                quiet = 0;
+                changed = true;
+            } else if (++quiet >= 3) {
+                return true;
+            }
         }
     }
     return false;
+}
+
+// The fetched marker values of the megapixel's border pixels, taken right
+// after tile_fetch: lo/hi halves = top/bottom row at column `lane` (tb), and
+// left/right column at row `lane` (lr).
+struct Border {
+    uint32_t tb, lr;
+};
+DEV Border tile_border(const uint32_t* s) {
+    const int lane = lane_id();
+    return Border{(s[tix(0, lane)] & 0xFFFFu) | (s[tix(kMP - 1, lane)] << 16),
+                  (s[tix(lane, 0)] & 0xFFFFu) | (s[tix(lane, kMP - 1)] << 16)};
+}
+
+template <class T>
+DEV void st_px_coh(uint8_t* p, uint32_t v) {
+    __hip_atomic_store(gp(reinterpret_cast<T*>(p)), (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Write-back for a wave that is the megapixel's only writer (mutex queue, UQ):
+// memory still holds what the wave fetched, so nothing is read back.  When the
+// visit raised any pixel, every pixel of the megapixel is written (32-bit
+// words; pixel stores for a word that runs past the image's right edge, whose
+// other bytes are not the image's); otherwise nothing.  The rose flags (same
+// layout as tile_store) come from the border values taken after the fetch.
+// Returns without waiting for the stores.
+template <class T>
+DEV void tile_store_excl(const uint32_t* s, uint8_t* rose, Border ob, bool changed,
+                         const Image& im, int mx, int my) {
+    constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
+    constexpr int BITS = 8 * (int)sizeof(T);
+    const int lane = lane_id();
+    if (!changed) {
+        reinterpret_cast<uint32_t*>(rose)[lane] = 0u;
+        wave_lds_sync();
+        return;
+    }
+    rose[lane] = (s[tix(0, lane)] & 0xFFFFu) != (ob.tb & 0xFFFFu);
+    rose[64 + lane] = (s[tix(kMP - 1, lane)] & 0xFFFFu) != (ob.tb >> 16);
+    rose[128 + lane] = (s[tix(lane, 0)] & 0xFFFFu) != (ob.lr & 0xFFFFu);
+    rose[192 + lane] = (s[tix(lane, kMP - 1)] & 0xFFFFu) != (ob.lr >> 16);
+    const int x0 = mx * kMP, y0 = my * kMP;
+    const int wl = lane % WPR, rl = lane / WPR;
+    const int xw = x0 + wl * PPW;
+    if (xw < im.W) {
+        const bool whole = xw + PPW <= im.W;
+#pragma unroll
+        for (int it = 0; it < kMP / RPI; ++it) {
+            const int row = it * RPI + rl;
+            const int y = y0 + row;
+            if (y >= im.H) continue;
+            uint8_t* wp = im.J + (size_t)y * im.pj + (size_t)xw * sizeof(T);
+            if (whole) {
+                uint32_t f = 0;
+#pragma unroll
+                for (int k = 0; k < PPW; ++k) f |= (s[tix(row, wl * PPW + k)] & 0xFFFFu) << (BITS * k);
+                st_coh(wp, f);
+            } else {
+                for (int k = 0; k < PPW && xw + k < im.W; ++k)
+                    st_px_coh<T>(wp + k * sizeof(T), s[tix(row, wl * PPW + k)] & 0xFFFFu);
+            }
+        }
+    }
+    wave_lds_sync();
 }
 
 // Neighbour bits of a megapixel.
@@ -1168,14 +1237,13 @@ DEV int neighbour_index(int m, int nmx, uint32_t bit) {
     }
 }
 
-// Writes back the pixels that rose, reading the marker again (in the
-// exclusive variants it still holds the fetched values).  EXCL: this wave is
-// the only writer of the megapixel (plain 32-bit stores).  Otherwise a
-// duplicate visitor may write concurrently: 32-bit compare-and-swap of the
-// per-pixel max.  Records which border pixels rose (rose[0..63] top row,
-// [64..127] bottom row, [128..191] left column, [192..255] right column).
-// Returns true when some pixel in memory rose.
-template <class T, bool EXCL>
+// Writes back the pixels that rose when a duplicate visitor of the same
+// megapixel may write concurrently (DQ): reads the marker again and merges
+// with a 32-bit compare-and-swap of the per-pixel max.  Records which border
+// pixels rose (rose[0..63] top row, [64..127] bottom row, [128..191] left
+// column, [192..255] right column).  Returns true when some pixel in memory
+// rose.
+template <class T>
 DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, int my) {
     constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
     constexpr int BITS = 8 * (int)sizeof(T);
@@ -1213,22 +1281,18 @@ DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, i
                 }
             }
             if (up) {
-                if constexpr (EXCL) {
-                    st_coh(wp, f);
-                } else {
-                    uint32_t cur = g;
-                    while (true) {
-                        uint32_t want = cur;
+                uint32_t cur = g;
+                while (true) {
+                    uint32_t want = cur;
 #pragma unroll
-                        for (int k = 0; k < PPW; ++k) {
-                            const uint32_t a = (cur >> (BITS * k)) & PM, b = (f >> (BITS * k)) & PM;
-                            if (b > a) want = (want & ~(PM << (BITS * k))) | (b << (BITS * k));
-                        }
-                        if (want == cur) break;
-                        const uint32_t seen = a_cas(wp, cur, want);
-                        if (seen == cur) break;
-                        cur = seen;
+                    for (int k = 0; k < PPW; ++k) {
+                        const uint32_t a = (cur >> (BITS * k)) & PM, b = (f >> (BITS * k)) & PM;
+                        if (b > a) want = (want & ~(PM << (BITS * k))) | (b << (BITS * k));
                     }
+                    if (want == cur) break;
+                    const uint32_t seen = a_cas(wp, cur, want);
+                    if (seen == cur) break;
+                    cur = seen;
                 }
                 any = true;
 #pragma unroll
@@ -1568,6 +1632,37 @@ DEV bool ring_push_many(const QueueArgs& q, bool mine, uint32_t value) {
     return !wave_any(!ok);
 }
 
+// Ticket value that tells a waiting wave the queue phase is over.
+constexpr uint32_t RING_DONE = 0xFFFFFFFFu;
+
+// Called by the wave whose decrement brought `pending` to zero: nothing is
+// queued or running, so nothing will be pushed again and every ticket below
+// the tail has been consumed.  A waiting wave holds one ticket at or after the
+// tail and takes at most one more, so the tickets [tail, tail + nwaves) (all
+// free: the ring holds more than nmp >= nwaves entries) are every ticket any
+// wave can still wait on; each gets RING_DONE.  Waiting waves then poll only
+// their own slot.
+DEV void ring_close(const QueueArgs& q, uint32_t nwaves) {
+    const int lane = lane_id();
+    uint32_t tail = 0;
+    if (lane == 0) tail = a_load(ctl_at(q.ctl, C_TAIL));
+    tail = first_lane(tail);
+    for (uint32_t i = (uint32_t)lane; i < nwaves; i += 64) {
+        const uint32_t t = tail + i;
+        a_store64(&q.ring[t & (q.cap - 1u)], ((unsigned long long)(t + 1u) << 32) | RING_DONE);
+    }
+}
+
+// Takes `owed` finished megapixels off `pending`; the wave that reaches zero
+// closes the ring.
+DEV void pending_settle(const QueueArgs& q, uint32_t& owed, uint32_t nwaves) {
+    if (!owed) return;
+    uint32_t before = 0;
+    if (lane_id() == 0) before = a_add(ctl_at(q.ctl, C_PENDING), 0u - owed);
+    if (first_lane(before) == owed) ring_close(q, nwaves);
+    owed = 0;
+}
+
 // Taking work.  The seeding pass fills tickets 0..S-1 before the kernel
 // starts; they are claimed in batches of B through their own counter (every
 // one is filled, nothing waits), read and their slots freed at once, and
@@ -1578,6 +1673,15 @@ DEV bool ring_push_many(const QueueArgs& q, bool mine, uint32_t value) {
 // are held back while it still has claimed work and flushed before it waits
 // for a single pop or leaves: the count is never below the true number of
 // queued and running megapixels, so no wave leaves early.
+//
+// A visit's memory round trips overlap where the protocol allows: the next
+// claimed megapixel m2 is taken (st[m2] = RUNNING) while this visit's stores
+// drain; after one wait, the notifications of this visit and the release of
+// m go out as one round of atomics, and m2's tile loads are issued behind
+// them.  m2 is not notified by this visit: its fetch comes after this visit's
+// stores are complete.  Taking m2 before m is released only holds two
+// megapixels at once; a notification that reaches m2 while it is taken
+// queues it again, as for any running megapixel.
 template <class T, int CONN>
 __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
     extern __shared__ uint32_t lds[];
@@ -1595,19 +1699,30 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
     // batches of 16), until too few waves share the work (measured
     // optimum: 8-16 at c5, 4 at c3, 1-2 at c2).
     const uint32_t B = max(1u, min(16u, seeded / (3u * nwaves)));
+    if (seeded == 0) return;  // `pending` starts at zero: nothing to do
     uint32_t claimed = 0;  // lane k < nclaimed holds the k-th claimed megapixel
     int nclaimed = 0, next = 0;
-    bool seeds_left = seeded > 0;
+    bool seeds_left = true;
     uint32_t owed = 0;     // finished megapixels not yet taken off `pending`
-    while (true) {
-        uint32_t m = 0;
-        int state = 0;  // 1 = got an entry, 2 = done, 3 = timeout
-        if (next >= nclaimed && seeds_left) {  // claim the next batch of seeded tickets
-            uint32_t c = 0;
-            if (lane == 0) c = a_add(ctl_at(q.ctl, C_SCLAIM), B);
-            c = first_lane(c);
+    // Next claimed megapixel, if any (claims a new batch when this one is used
+    // up).  Wave w's first batch is tickets [wB, wB + B), taken without an
+    // atomic (thousands of waves starting on one counter queue up); later
+    // batches come from the counter, after the first nwaves * B tickets.
+    bool first_claim = true;
+    auto take_claimed = [&](uint32_t& out) -> bool {
+        if (next >= nclaimed && seeds_left) {
+            uint32_t c;
+            if (first_claim) {
+                c = (blockIdx.x * (kWG / 64) + first_lane(threadIdx.x >> 6)) * B;
+                seeds_left = nwaves * B < seeded;  // tickets left for the counter
+                first_claim = false;
+            } else {
+                uint32_t cc = 0;
+                if (lane == 0) cc = a_add(ctl_at(q.ctl, C_SCLAIM), B);
+                c = nwaves * B + first_lane(cc);
+                seeds_left = c + B < seeded;
+            }
             nclaimed = c >= seeded ? 0 : (int)min(B, seeded - c);
-            seeds_left = c + B < seeded;
             next = 0;
             claimed = 0;
             if (lane < nclaimed) {
@@ -1617,70 +1732,103 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
                 a_store64(&q.ring[slot], (unsigned long long)(t + q.cap) << 32);
             }
         }
-        if (next < nclaimed) {
-            m = rdlane(claimed, next++);
-            state = 1;
-        } else {
-            if (owed && lane == 0) a_add(ctl_at(q.ctl, C_PENDING), 0u - owed);
-            owed = 0;
-            uint32_t h = 0;
-            if (lane == 0) h = a_add(ctl_at(q.ctl, C_HEAD), 1u);
-            h = first_lane(h);
-            const uint32_t slot = h & (q.cap - 1u);
-            uint32_t got = 0;
-            if (lane == 0) {
-                Waiter w(q.timeout_ticks);
-                while (true) {
-                    const unsigned long long e = a_load64(&q.ring[slot]);
-                    if ((uint32_t)(e >> 32) == h + 1u) { state = 1; got = (uint32_t)e; break; }
-                    if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
-                    if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u || !w.step(q.ctl)) { state = 3; break; }
+        if (next >= nclaimed) return false;
+        out = rdlane(claimed, next++);
+        return true;
+    };
+    uint32_t m = 0;
+    bool have = false;  // m is taken and its tile is in LDS
+    while (true) {
+        if (!have) {
+            int state = 0;  // 1 = got an entry, 2 = done, 3 = timeout
+            if (take_claimed(m)) {
+                state = 1;
+            } else {
+                pending_settle(q, owed, nwaves);
+                uint32_t h = 0;
+                if (lane == 0) h = a_add(ctl_at(q.ctl, C_HEAD), 1u);
+                h = first_lane(h);
+                const uint32_t slot = h & (q.cap - 1u);
+                uint32_t got = 0;
+                if (lane == 0) {
+                    // The wave polls its own ring slot, which gets an entry or
+                    // RING_DONE.  Reads of the shared counters, which every
+                    // working wave updates, queue up with those updates and
+                    // slow the waves that still work: the timeout flag (and
+                    // `pending`, a second way out) are read on every 256th
+                    // poll only.
+                    Waiter w(q.timeout_ticks);
+                    for (uint32_t k = 0;; ++k) {
+                        const unsigned long long e = a_load64(&q.ring[slot]);
+                        if ((uint32_t)(e >> 32) == h + 1u) {
+                            got = (uint32_t)e;
+                            state = got == RING_DONE ? 2 : 1;
+                            break;
+                        }
+                        if ((k & 255u) == 255u) {
+                            if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u) { state = 3; break; }
+                            if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
+                        }
+                        if (!w.step(q.ctl)) { state = 3; break; }
+                    }
+                    if (state == 1) a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
                 }
-                if (state == 1) a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
+                state = (int)first_lane((uint32_t)state);
+                if (state != 1) break;
+                m = first_lane(got);
             }
-            state = (int)first_lane((uint32_t)state);
-            if (state != 1) break;
-            m = first_lane(got);
+            if (lane == 0) a_xchg(&q.st[m], ST_RUNNING);
+            mem_wait();
+            pt.lap(0);
+            const ImgDesc d0 = desc(ds, find_mp(ds, (int)m));
+            const int ml0 = (int)m - d0.mp_base;
+            tile_fetch<T>(image_of(d0), ml0 % d0.nmx, ml0 / d0.nmx, s);
+            pt.lap(1);
         }
-        if (lane == 0) a_xchg(&q.st[m], ST_RUNNING);
-        mem_wait();
-        pt.lap(0);
         const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
         const Image im = image_of(dsc);
         const int ml = (int)m - dsc.mp_base;
         const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
-        tile_fetch<T>(im, mx, my, s);
-        pt.lap(1);
-        const bool conv = tile_converge<CONN>(s, 4 * q.max_iters);
+        const Border ob = tile_border(s);
+        bool changed;
+        const bool conv = tile_converge<CONN>(s, 4 * q.max_iters, changed);
         pt.lap(2);
-        tile_store<T, true>(s, rose, im, mx, my);
-        mem_wait();
-        pt.lap(3);
+        tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
+        uint32_t m2 = 0;
+        const bool have2 = take_claimed(m2);
+        if (have2 && lane == 0) a_xchg(&q.st[m2], ST_RUNNING);
         uint32_t nb = tile_notify_mask<CONN>(s, rose, im, mx, my);
-        wave_lds_sync();
         if (!conv) nb |= NB_SELF;
+        mem_wait();  // this visit's stores are visible and m2 is taken
+        pt.lap(3);
         ++visits;
-        // notify neighbours: lane b handles bit b
-        bool push = false;
-        uint32_t target = 0;
+        // notify neighbours (lane b handles bit b) and release m (lane 9)
+        bool tried = false;
+        uint32_t target = 0, old = 0;
         if (lane < 9 && ((nb >> lane) & 1u)) {
             target = (uint32_t)(dsc.mp_base + neighbour_index(ml, dsc.nmx, 1u << lane));
-            const uint32_t old = a_or(&q.st[target], ST_PENDING);
-            push = old == 0u;
+            tried = !(have2 && target == m2);
+            if (tried) old = a_or(&q.st[target], ST_PENDING);
         }
+        if (lane == 9) old = a_and(&q.st[m], ~ST_RUNNING);
+        if (have2) {  // m2's loads go out behind the atomics
+            const ImgDesc d2 = desc(ds, find_mp(ds, (int)m2));
+            const int ml2 = (int)m2 - d2.mp_base;
+            tile_fetch<T>(image_of(d2), ml2 % d2.nmx, ml2 / d2.nmx, s);
+        }
+        const bool push = tried && old == 0u;
+        const bool again = (rdlane(old, 9) & ST_PENDING) != 0u;
         const uint32_t npush = (uint32_t)__popcll(ballot(push));
         if (npush && lane == 0) a_add(ctl_at(q.ctl, C_PENDING), npush);
         bool ok = ring_push_many(q, push, target);
-        // finish
-        uint32_t old = 0;
-        if (lane == 0) old = a_and(&q.st[m], ~ST_RUNNING);
-        old = first_lane(old);
-        if (old & ST_PENDING) ok = ring_push_many(q, lane == 0, m) && ok;
+        if (again) ok = ring_push_many(q, lane == 0, m) && ok;
         else ++owed;
         pt.lap(4);
         if (!ok) break;
+        m = m2;
+        have = have2;
     }
-    if (owed && lane == 0) a_add(ctl_at(q.ctl, C_PENDING), 0u - owed);
+    pending_settle(q, owed, nwaves);
     pt.lap(0);
     pt.flush(q.ctl);
     if (lane == 0 && visits) {
@@ -1749,10 +1897,14 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
             const int ml = (int)m - dsc.mp_base;
             const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
             tile_fetch<T>(im, mx, my, s);
+            const Border ob = tile_border(s);
             pt.lap(1);
-            const bool conv = tile_converge<CONN>(s, 4 * q.max_iters);
+            bool changed;
+            const bool conv = tile_converge<CONN>(s, 4 * q.max_iters, changed);
             pt.lap(2);
-            const bool rose_any = tile_store<T, EXCL>(s, rose, im, mx, my);
+            bool rose_any = changed;
+            if constexpr (EXCL) tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
+            else rose_any = tile_store<T>(s, rose, im, mx, my);
             mem_wait();
             pt.lap(3);
             uint32_t nb = tile_notify_mask<CONN>(s, rose, im, mx, my);
