@@ -9,6 +9,7 @@
 // rises and never passes the reconstruction, so any partial marker is a valid
 // restart point.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -97,14 +98,15 @@ struct iwpp_workspace {
     size_t feat_part_n = 0;
     float* feat_raw = nullptr;  // raw tile statistics
     size_t feat_raw_n = 0;
-    float* feat = nullptr;      // normalised tile features
-    size_t feat_n = 0;
-    float* pred = nullptr;      // MLP outputs per tile
-    size_t pred_n = 0;
     uint32_t* order = nullptr;  // seeding order: tile ranks then tile starts
     size_t order_n = 0;
     void* host_plan = nullptr;  // pinned staging for the learned plan (both ways)
     size_t host_plan_n = 0;
+    float* host_plan_dev = nullptr;  // the same memory as the device sees it
+    uint32_t* plan_flag = nullptr;   // pinned, mapped: the network's "outputs are in" word
+    uint32_t* plan_flag_dev = nullptr;
+    uint32_t* plan_count = nullptr;  // device word (zero between calls) for that kernel
+    uint32_t plan_seq = 0;
     uint16_t* model = nullptr;  // BF16 MLP weights on this device
     int model_version = -1;
     hipStream_t own = nullptr;  // used when the caller passes the NULL stream
@@ -153,12 +155,12 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->img_of_mp);
     (void)hipFree(ws->feat_part);
     (void)hipFree(ws->feat_raw);
-    (void)hipFree(ws->feat);
-    (void)hipFree(ws->pred);
     (void)hipFree(ws->order);
     (void)hipFree(ws->model);
     if (ws->host_ctl) (void)hipHostFree(ws->host_ctl);
     if (ws->host_plan) (void)hipHostFree(ws->host_plan);
+    if (ws->plan_flag) (void)hipHostFree(ws->plan_flag);
+    (void)hipFree(ws->plan_count);
     if (ws->own) (void)hipStreamDestroy(ws->own);
     for (hipEvent_t e : ws->ev)
         if (e) (void)hipEventDestroy(e);
@@ -243,7 +245,15 @@ struct Run {
     int sched_tile = 0, sched_ntx = 0, sched_nt = 0;
     const uint32_t* order_tiles = nullptr, *order_start = nullptr;
     int order_nt = 0, order_ntx = 0, order_mpt = 0;
+    // learned schedule, between the feature pass and the host's plan
+    bool plan_predict = false;  // the network is still to be launched (after the first rows)
+    bool plan_order = false;    // ... and writes the seeding order to ws->order
+    int plan_tiles = 0, plan_tile = 0;
+    const uint32_t* plan_tile_base = nullptr;
+    uint32_t plan_seq = 0;
 };
+
+int learned_plan_predict(Run& r);
 
 int ensure_events(iwpp_workspace* ws) {
     for (hipEvent_t& e : ws->ev)
@@ -415,6 +425,10 @@ int sweep_phase(Run& r, int k0, int k1) {
             HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0, r.row16,
                                           r.row_wave, nullptr, r.s));
         }
+        // learned plan: the network goes between the first row pass and the
+        // column passes, so the host plans while those run
+        const int rp = learned_plan_predict(r);
+        if (rp) return rp;
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
                                       COL_IN_PLACE, r.wide, nullptr, r.s));
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
@@ -579,8 +593,60 @@ int ensure_host_plan(iwpp_workspace* ws, size_t bytes) {
     if (ws->host_plan) HIP_TRY(hipHostFree(ws->host_plan));
     ws->host_plan = nullptr;
     ws->host_plan_n = 0;
-    HIP_TRY(hipHostMalloc(&ws->host_plan, bytes, 0));
+    // mapped and coherent: the network's outputs are written here by the
+    // device and read by the host once the stream's event has passed
+    HIP_TRY(hipHostMalloc(&ws->host_plan, bytes, hipHostMallocMapped | hipHostMallocCoherent));
     ws->host_plan_n = bytes;
+    HIP_TRY(hipHostGetDevicePointer((void**)&ws->host_plan_dev, ws->host_plan, 0));
+    return IWPP_OK;
+}
+
+int ensure_plan_flag(iwpp_workspace* ws) {
+    if (ws->plan_flag) return IWPP_OK;
+    HIP_TRY(hipMalloc((void**)&ws->plan_count, 4));
+    HIP_TRY(hipMemset(ws->plan_count, 0, 4));
+    HIP_TRY(hipHostMalloc((void**)&ws->plan_flag, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    *(volatile uint32_t*)ws->plan_flag = 0;
+    ws->plan_seq = 0;
+    HIP_TRY(hipHostGetDevicePointer((void**)&ws->plan_flag_dev, ws->plan_flag, 0));
+    return IWPP_OK;
+}
+
+// Launches the network of the learned plan (after the first row pass: it
+// reads only the feature partials of the input marker).
+int learned_plan_predict(Run& r) {
+    if (!r.plan_predict) return IWPP_OK;
+    r.plan_predict = false;
+    iwpp_workspace* ws = r.ws;
+    Descs one{};
+    one.n = 1;
+    one.one = r.h[0];
+    HIP_TRY(launch_plan_predict(r.h.size() == 1 ? one : r.ds, r.plan_tile_base, r.plan_tiles,
+                                r.dtype, r.conn, r.plan_tile, kFeatureSub, ws->feat_part,
+                                ws->feat_raw, ws->model, ws->host_plan_dev, ws->plan_count,
+                                ws->plan_flag_dev, r.plan_seq,
+                                r.plan_order ? ws->order : nullptr, r.s));
+    return IWPP_OK;
+}
+
+// Waits (polling the pinned flag) until the network's outputs are in host
+// memory; a failed stream or the time limit ends the wait with an error.
+int learned_plan_wait(Run& r) {
+    volatile uint32_t* flag = r.ws->plan_flag;
+    const auto t0 = std::chrono::steady_clock::now();
+    const double limit = r.opt.timeout_s > 0 ? r.opt.timeout_s : 60.0;
+    for (uint32_t k = 1; *flag != r.plan_seq; ++k) {
+        if ((k & 1023u) == 0) {
+            const hipError_t e = hipStreamQuery(r.s);
+            if (e != hipSuccess && e != hipErrorNotReady)
+                return fail(IWPP_ERR_HIP, "learned plan: %s", hipGetErrorString(e));
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > limit)
+                return fail(IWPP_ERR_TIMEOUT, "learned plan: the network's outputs did not arrive");
+            if (e == hipSuccess && *flag != r.plan_seq)
+                return fail(IWPP_ERR_HIP, "learned plan: stream idle without the network's outputs");
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     return IWPP_OK;
 }
 
@@ -637,17 +703,24 @@ int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
     const LearnedLayout L = learned_layout(r, plan.tile);
     rc = ensure_host_plan(ws, learned_staging_bytes(r, L.total));
     if (rc) return rc;
+    rc = ensure_plan_flag(ws);
+    if (rc) return rc;
     float* pred = (float*)ws->host_plan;
     HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
                  (size_t)L.total * features_slabs(L.tile, kFeatureSub) * 16));
     HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)L.total * 16));
-    HIP_TRY(grow(ws->feat, ws->feat_n, (size_t)L.total * 16));
-    HIP_TRY(grow(ws->pred, ws->pred_n, (size_t)L.total * 2));
     if (n == 1) {
         const ImgDesc& d = r.h[0];
-        HIP_TRY(launch_tile_features(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, r.conn,
-                                     L.tile, kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat,
-                                     r.s));
+        Descs one{};
+        one.n = 1;
+        one.one = d;
+        const bool v16 = (uintptr_t)d.J % 16 == 0 && (uintptr_t)d.I % 16 == 0 && d.pj % 16 == 0 &&
+                         d.pi % 16 == 0;
+        HIP_TRY(launch_plan_features(one, nullptr, L.total, v16, r.dtype, L.tile, kFeatureSub,
+                                     ws->feat_part, r.s));
+        r.plan_tile_base = nullptr;
+        r.plan_order = L.total > 1 && L.total <= kOrderMax;
+        if (r.plan_order) HIP_TRY(grow(ws->order, ws->order_n, (size_t)2 * L.total + 1));
     } else {
         bool v16 = true;
         for (const ImgDesc& d : r.h)
@@ -657,14 +730,15 @@ int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
         std::copy(L.base.begin(), L.base.end(), h_base);
         HIP_TRY(grow(ws->order, ws->order_n, (size_t)n + 1));
         HIP_TRY(hipMemcpyAsync(ws->order, h_base, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, r.s));
-        HIP_TRY(launch_tile_features_batch(r.ds, ws->order, L.total, v16, r.dtype, r.conn, L.tile,
-                                           kFeatureSub, ws->feat_part, ws->feat_raw, ws->feat,
-                                           r.s));
+        HIP_TRY(launch_plan_features(r.ds, ws->order, L.total, v16, r.dtype, L.tile,
+                                     kFeatureSub, ws->feat_part, r.s));
+        r.plan_tile_base = ws->order;
     }
-    HIP_TRY(launch_mlp_forward(ws->model, true, ws->feat, L.total, ws->pred, nullptr, r.s));
-    HIP_TRY(hipMemcpyAsync(pred, ws->pred, (size_t)L.total * 2 * sizeof(float),
-                           hipMemcpyDeviceToHost, r.s));
-    HIP_TRY(hipEventRecord(ws->ev[4], r.s));
+    r.plan_predict = true;
+    r.plan_tiles = L.total;
+    r.plan_tile = L.tile;
+    r.plan_seq = ++ws->plan_seq;
+    if (r.plan_seq == 0) r.plan_seq = ++ws->plan_seq;  // 0 is the flag's initial value
     *pending = true;
     return IWPP_OK;
 }
@@ -696,7 +770,8 @@ int learned_plan_end_batch(Run& r, SchedulePlan& plan) {
     iwpp_workspace* ws = r.ws;
     const int n = (int)r.h.size();
     const LearnedLayout L = learned_layout(r, plan.tile);
-    HIP_TRY(hipEventSynchronize(ws->ev[4]));
+    const int rw = learned_plan_wait(r);
+    if (rw) return rw;
     const float* pred = (const float*)ws->host_plan;
     const int cap = std::max(1, plan.sweep_rounds);
     int rmax = 0;
@@ -724,7 +799,8 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     const ImgDesc& d = r.h[0];
     const int tile = plan.tile;
     const int ntx = (d.W + tile - 1) / tile, nty = (d.H + tile - 1) / tile, nt = ntx * nty;
-    HIP_TRY(hipEventSynchronize(ws->ev[4]));
+    const int rw = learned_plan_wait(r);
+    if (rw) return rw;
     const float* pred = (const float*)ws->host_plan;
     uint32_t* h_order = (uint32_t*)round_up((uintptr_t)(pred + (size_t)nt * 2) + 2 * 4 +
                                                 sizeof(ImgDesc),
@@ -742,12 +818,18 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     for (int t = 0; t < nt; ++t) skipped += std::max(0, rounds - k[t]);
     r.st.sweeps_skipped_tiles = skipped;
     if (nt == 1) return IWPP_OK;  // one tile: uniform rounds, nothing to order
+    const int mpt = tile / kMP;   // megapixels per tile side
+    r.order_tiles = ws->order;
+    r.order_start = ws->order + nt;
+    r.order_nt = nt;
+    r.order_ntx = ntx;
+    r.order_mpt = mpt;
+    if (!r.plan_order) {  // more tiles than the network kernel orders: sort here
     std::vector<int> tiles(nt);
     for (int t = 0; t < nt; ++t) tiles[t] = t;
     std::stable_sort(tiles.begin(), tiles.end(), [&](int a, int b) {
         return pred[(size_t)a * 2 + 1] > pred[(size_t)b * 2 + 1];
     });
-    const int mpt = tile / kMP;  // megapixels per tile side
     uint32_t pos = 0;
     for (int rk = 0; rk < nt; ++rk) {
         const int t = tiles[rk], tx = t % ntx, ty = t / ntx;
@@ -761,9 +843,7 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
                            r.s));
     r.order_tiles = ws->order;
     r.order_start = ws->order + nt;
-    r.order_nt = nt;
-    r.order_ntx = ntx;
-    r.order_mpt = mpt;
+    }
     if (skipped > 0) {  // per-round tile masks, only when some tile stops early
         const size_t nm = (size_t)rounds * nt;
         for (int rr = 0; rr < rounds; ++rr)

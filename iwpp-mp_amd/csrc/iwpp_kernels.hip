@@ -953,6 +953,10 @@ DEV uint32_t upd(uint32_t w, uint32_t p) {
 DEV uint32_t pmax(uint32_t a, uint32_t b) {
     return as1(__builtin_elementwise_max(as2(a), as2(b)));
 }
+// Maximum of the low halves (one v_max3_u16); the high half is left zero.
+DEV uint32_t max3_lo(uint32_t a, uint32_t b, uint32_t c) {
+    return (uint32_t)max(max((uint16_t)a, (uint16_t)b), (uint16_t)c);
+}
 // min(J(p), I(q)) > J(q) for a pixel value p and a packed neighbour word q.
 DEV bool raises(uint32_t p, uint32_t q) { return min(p, q >> 16) > (q & 0xFFFFu); }
 
@@ -1105,9 +1109,9 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2]) {
             const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
             const int rp = r - DIR;                    // ... and of the row before it
             uint32_t q = p;
-            if constexpr (CONN == 8) {
+            if constexpr (CONN == 8) {  // upd reads only q's low half
                 const uint32_t hl = s[rp * TS], hr = s[rp * TS + 65];  // halo columns of row rp
-                q = pmax(q, pmax(from_left(p, hl), from_right(p, hr)));
+                q = max3_lo(p, from_left(p, hl), from_right(p, hr));
             }
             const uint32_t nw = upd(w[r], q);
             ch[d] |= nw != w[r];

@@ -8,6 +8,7 @@ namespace iwpp {
 constexpr int kMlpParams = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2;  // 5378
 constexpr int kMlpIn = 16, kMlpOut = 2;
 constexpr int kFeatureSub = 16;  // row subsampling of the tile features in a reconstruction
+constexpr int kOrderMax = 1024;  // most tiles whose seeding order the network kernel computes
 
 // BF16 round to nearest even from the IEEE-754 binary32 layout (NaN kept quiet).
 __host__ __device__ inline uint16_t f32_to_bf16(float f) {
@@ -33,6 +34,19 @@ hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, 
 hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base, int ntiles,
                                       bool v16, int dtype, int conn, int tile, int sub,
                                       float* part, float* raw, float* feat, hipStream_t s);
+// The learned plan's two launches: the feature pass (slab partials of the
+// input marker), then the network, which reads only the partials and may run
+// after the marker has changed.  `out` (two floats per tile) and `flag` are
+// device-visible pinned host memory; `flag` gets `seq` once every output is
+// there; `count` is a zeroed device word.  `order` (device, 2 ntiles + 1
+// words, one image of at most kOrderMax tiles, else null): the seeding order
+// by decreasing predicted priority and each tile's first queue position.
+hipError_t launch_plan_features(const Descs& ds, const uint32_t* tile_base, int ntiles, bool v16,
+                                int dtype, int tile, int sub, float* part, hipStream_t s);
+hipError_t launch_plan_predict(const Descs& ds, const uint32_t* tile_base, int ntiles, int dtype,
+                               int conn, int tile, int sub, const float* part, float* raw,
+                               const void* params, float* out, uint32_t* count, uint32_t* flag,
+                               uint32_t seq, uint32_t* order, hipStream_t s);
 // y: n x 2; acts (optional): n x 128 hidden activations for training.
 hipError_t launch_mlp_forward(const void* params, bool bf16, const float* x, int n, float* y,
                               float* acts, hipStream_t s);

@@ -17,14 +17,16 @@ namespace iwpp {
 //  0 sum I, 1 sum J, 2 sum (I-J), 3 sum (I-J)^2, 4 max(0, max (I-J)),
 //  5 count J==I, 6 count J==0, 7 count seeds (J==I, J>0), 8 max J,
 //  9 sum |I(x+1)-I(x)| over pixel pairs (x even, x+1 in the tile),
-//  10..13 border inflow top/bottom/left/right: sum over every border pixel p
-//  of max(0, min(J(q), I(p)) - J(p)) with q the pixel across the border,
+//  10..13 border inflow top/bottom/left/right: sum of max(0, min(J(q), I(p))
+//  - J(p)) over border pixels p, q the pixel across the border: every pixel
+//  of the top and bottom rows, the left and right columns on the sampled rows,
 //  14 pixels sampled, 15 max I.
 // Work split: one workgroup per (tile, slab of kSlab sampled rows); partial
 // statistics go to part[tile][slab][16]; a second kernel combines and
 // normalises them per image.
 // Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
-constexpr int kSlab = 8;
+constexpr int kSlab = 4;
+constexpr int kRawLds = 256;  // tiles of one image whose statistics the plan keeps in LDS
 constexpr int kStats = 16;
 
 DEV bool is_max_stat(int k) { return k == 4 || k == 8 || k == 15; }
@@ -140,25 +142,43 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
         }
     }
     }
-    // border inflow: rows y0 / y1-1 in the first / last slab, the left and
-    // right columns over this slab's band of image rows
+    // border inflow: rows y0 / y1-1 in the first slab, the left and right
+    // columns on this slab's sampled rows (a column read touches one cache
+    // line per row: on every row it cost more than the rest of the kernel)
     auto J = [&](int x, int y) -> float {
         return (float)*reinterpret_cast<const T*>(im.J + (size_t)y * im.pj + (size_t)x * sizeof(T));
     };
     auto I = [&](int x, int y) -> float {
         return (float)*reinterpret_cast<const T*>(im.I + (size_t)y * im.pi + (size_t)x * sizeof(T));
     };
-    if (sl == 0) {
-        for (int x = x0 + (int)threadIdx.x; x < x1; x += blockDim.x) {
-            if (y0 > 0) acc[10] += fmaxf(0.f, fminf(J(x, y0 - 1), I(x, y0)) - J(x, y0));
-            if (y1 < im.H) acc[11] += fmaxf(0.f, fminf(J(x, y1), I(x, y1 - 1)) - J(x, y1 - 1));
+    if (sl == 0) {  // groups of 8 columns per thread, loads (clamped) before use
+        const int ya = max(y0 - 1, 0), yb = min(y1, im.H - 1);
+        for (int xb = x0; xb < x1; xb += 8 * (int)blockDim.x) {
+            float a[8][6];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int x = min(xb + (int)threadIdx.x + u * (int)blockDim.x, x1 - 1);
+                a[u][0] = J(x, ya);
+                a[u][1] = I(x, y0);
+                a[u][2] = J(x, y0);
+                a[u][3] = J(x, yb);
+                a[u][4] = I(x, y1 - 1);
+                a[u][5] = J(x, y1 - 1);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (xb + (int)threadIdx.x + u * (int)blockDim.x >= x1) continue;
+                if (y0 > 0) acc[10] += fmaxf(0.f, fminf(a[u][0], a[u][1]) - a[u][2]);
+                if (y1 < im.H) acc[11] += fmaxf(0.f, fminf(a[u][3], a[u][4]) - a[u][5]);
+            }
         }
     }
-    const int band0 = y0 + sl * kSlab * sub;
-    const int band1 = sl == nslab - 1 ? y1 : min(y1, band0 + kSlab * sub);
-    for (int y = band0 + (int)threadIdx.x; y < band1; y += blockDim.x) {
-        if (x0 > 0) acc[12] += fmaxf(0.f, fminf(J(x0 - 1, y), I(x0, y)) - J(x0, y));
-        if (x1 < im.W) acc[13] += fmaxf(0.f, fminf(J(x1, y), I(x1 - 1, y)) - J(x1 - 1, y));
+    if ((int)threadIdx.x < kSlab) {
+        const int y = y0 + (sl * kSlab + (int)threadIdx.x) * sub;
+        if (y < y1) {
+            if (x0 > 0) acc[12] += fmaxf(0.f, fminf(J(x0 - 1, y), I(x0, y)) - J(x0, y));
+            if (x1 < im.W) acc[13] += fmaxf(0.f, fminf(J(x1, y), I(x1 - 1, y)) - J(x1 - 1, y));
+        }
     }
     __shared__ float red[kStats][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -194,12 +214,11 @@ DEV void normalise(const float* r, float M, int tile, int sub, int dtype, int co
     f[7] = r[7] / n * 100.f;
     f[8] = r[8] * inv;
     f[9] = r[9] / n * inv * 10.f;
-    (void)sub;
     const float side = (float)tile;
     f[10] = r[10] / side * inv;
     f[11] = r[11] / side * inv;
-    f[12] = r[12] / side * inv;
-    f[13] = r[13] / side * inv;
+    f[12] = r[12] * (float)sub / side * inv;  // columns sampled on every sub-th row
+    f[13] = r[13] * (float)sub / side * inv;
     f[14] = dtype == 2 ? 1.f : 0.f;
     // the image's size (what a size rule would key on), at most 2^28 pixels:
     // larger images than the training set's largest are seen as that size
@@ -210,23 +229,36 @@ DEV void normalise(const float* r, float M, int tile, int sub, int dtype, int co
 // lane = (slab group g = lane / 16, statistic k = lane % 16); each lane folds
 // every fourth slab, then the four groups are folded across lanes (a fixed
 // order: the result does not depend on scheduling).
-__global__ __launch_bounds__(256) void k_features_tiles(const float* part, int nslab, int ntiles,
-                                                         float* raw) {
+// Writes the tile's 16 statistics to out[0..15].  A lane folds every fourth
+// slab in order; its loads go out in groups of 8 before any is used.
+DEV void fold_tile(const float* part, int nslab, int t, float* out) {
     const int lane = threadIdx.x & 63;
-    const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (t >= ntiles) return;  // whole wave
     const int k = lane & 15, g = lane >> 4;
     const bool mx = is_max_stat(k);
     float v = 0.f;  // identity of both folds (statistics are >= 0)
-    for (int sl = g; sl < nslab; sl += 4) {
-        const float o = part[((size_t)t * nslab + sl) * kStats + k];
-        v = mx ? fmaxf(v, o) : v + o;
+    constexpr int U = 8;
+    for (int s0 = g; s0 < nslab; s0 += 4 * U) {
+        float o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int sl = s0 + 4 * u;
+            o[u] = sl < nslab ? part[((size_t)t * nslab + sl) * kStats + k] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v = mx ? fmaxf(v, o[u]) : v + o[u];
     }
     const float o16 = __shfl_xor(v, 16);
     v = mx ? fmaxf(v, o16) : v + o16;
     const float o32 = __shfl_xor(v, 32);
     v = mx ? fmaxf(v, o32) : v + o32;
-    if (g == 0) raw[(size_t)t * kStats + k] = v;
+    if (g == 0) out[k] = v;
+}
+
+__global__ __launch_bounds__(256) void k_features_tiles(const float* part, int nslab, int ntiles,
+                                                         float* raw) {
+    const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (t >= ntiles) return;  // whole wave
+    fold_tile(part, nslab, t, raw + (size_t)t * kStats);
 }
 
 // One workgroup per image: M = the image's largest mask value over its tiles'
@@ -264,49 +296,210 @@ static_assert(O_B3 + 2 == kMlpParams, "MLP layout");
 // two W3 entries of column j).  Inputs and layer-1 activations reach every
 // lane through readlane; the outputs are wave sums.  FP32 accumulation.
 // acts (optional, training): per sample h1[64] then h2[64] (after ReLU).
+// A store to the host's pinned memory that no device cache holds.
+DEV void st_host(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Lane j's share of the weights: W1 row j, W2 row j, the two W3 entries of
+// column j, and the biases.
+struct MlpLane {
+    float w1[16], w2[64], b1, b2, w30, w31, b30, b31;
+};
+// N consecutive weights from element `first` (16-byte aligned: the rows of W1
+// and W2 start at multiples of 8 elements), as 16-byte loads.
+template <bool BF16, int N>
+DEV void load_row(const void* params, int first, float* w) {
+    constexpr int PER = BF16 ? 8 : 4;  // weights per 16 bytes
+    const uint4* p = reinterpret_cast<const uint4*>(
+        reinterpret_cast<const char*>(params) + (size_t)first * (BF16 ? 2 : 4));
+    uint4 v[N / PER];
+#pragma unroll
+    for (int i = 0; i < N / PER; ++i) v[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < N / PER; ++i) {
+        const uint32_t q[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if constexpr (BF16) {
+                w[i * 8 + 2 * j] = __builtin_bit_cast(float, q[j] << 16);
+                w[i * 8 + 2 * j + 1] = __builtin_bit_cast(float, q[j] & 0xFFFF0000u);
+            } else {
+                w[i * 4 + j] = __builtin_bit_cast(float, q[j]);
+            }
+        }
+    }
+}
+
+template <bool BF16>
+DEV MlpLane mlp_lane(const void* params, int lane) {
+    auto W = [&](int i) -> float {
+        return BF16 ? bf16_to_f32(reinterpret_cast<const uint16_t*>(params)[i])
+                    : reinterpret_cast<const float*>(params)[i];
+    };
+    static_assert(O_W1 % 8 == 0 && O_W2 % 8 == 0, "weight rows 16-byte aligned");
+    MlpLane m;
+    load_row<BF16, 16>(params, O_W1 + lane * 16, m.w1);
+    load_row<BF16, 64>(params, O_W2 + lane * 64, m.w2);
+    m.b1 = W(O_B1 + lane);
+    m.b2 = W(O_B2 + lane);
+    m.w30 = W(O_W3 + lane);
+    m.w31 = W(O_W3 + 64 + lane);
+    m.b30 = W(O_B3);
+    m.b31 = W(O_B3 + 1);
+    return m;
+}
+// One sample: lane k < 16 holds input k in xv.  Returns both outputs in every
+// lane, and the lane's hidden units h1, h2 (after ReLU).
+DEV void mlp_sample(const MlpLane& m, float xv, float& y0, float& y1, float& h1, float& h2) {
+    float a = m.b1;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        a = fmaf(m.w1[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), k)), a);
+    h1 = fmaxf(a, 0.f);
+    a = m.b2;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+        a = fmaf(m.w2[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k)), a);
+    h2 = fmaxf(a, 0.f);
+    y0 = m.w30 * h2;
+    y1 = m.w31 * h2;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        y0 += __shfl_xor(y0, 32 >> i, 64);
+        y1 += __shfl_xor(y1, 32 >> i, 64);
+    }
+    y0 += m.b30;
+    y1 += m.b31;
+}
+
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_mlp_forward(const void* params, const float* x, int n,
                                                       float* y, float* acts) {
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-    auto W = [&](int i) -> float {
-        return BF16 ? bf16_to_f32(reinterpret_cast<const uint16_t*>(params)[i])
-                    : reinterpret_cast<const float*>(params)[i];
-    };
-    float w1[16], w2[64];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) w1[k] = W(O_W1 + lane * 16 + k);
-#pragma unroll
-    for (int k = 0; k < 64; ++k) w2[k] = W(O_W2 + lane * 64 + k);
-    const float b1 = W(O_B1 + lane), b2 = W(O_B2 + lane);
-    const float w30 = W(O_W3 + lane), w31 = W(O_W3 + 64 + lane);
-    const float b30 = W(O_B3), b31 = W(O_B3 + 1);
+    const MlpLane m = mlp_lane<BF16>(params, lane);
     for (int s = wave; s < n; s += nwaves) {
         const float xv = lane < 16 ? x[(size_t)s * 16 + lane] : 0.f;
-        float a = b1;
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            a = fmaf(w1[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), k)), a);
-        const float h1 = fmaxf(a, 0.f);
-        a = b2;
-#pragma unroll
-        for (int k = 0; k < 64; ++k)
-            a = fmaf(w2[k], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), k)), a);
-        const float h2 = fmaxf(a, 0.f);
-        float y0 = w30 * h2, y1 = w31 * h2;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            y0 += __shfl_xor(y0, 32 >> i, 64);
-            y1 += __shfl_xor(y1, 32 >> i, 64);
-        }
+        float y0, y1, h1, h2;
+        mlp_sample(m, xv, y0, y1, h1, h2);
         if (acts) {
             acts[(size_t)s * 128 + lane] = h1;
             acts[(size_t)s * 128 + 64 + lane] = h2;
         }
         if (lane == 0) {
-            y[(size_t)s * 2] = y0 + b30;
-            y[(size_t)s * 2 + 1] = y1 + b31;
+            y[(size_t)s * 2] = y0;
+            y[(size_t)s * 2 + 1] = y1;
+        }
+    }
+}
+
+// The plan's network outputs from the slab partials, one workgroup (16
+// waves) per image: fold the slabs of each tile (as k_features_tiles), the
+// image's largest mask value, each tile's normalised features (as
+// k_features_finish) and the network's two outputs (as k_mlp_forward with
+// BF16 weights), written to `out` -- the host's pinned staging, mapped.  The
+// last workgroup to finish writes `seq` to `flag` (host memory as well), which
+// the host polls: no copy and no event between the network and the host's
+// plan.  `count` (device, zero) counts finished workgroups and is left zero.
+// With `order` (one image of at most kOrderMax tiles), the seeding order goes
+// there as well: order[r] = the tile of rank r by decreasing priority (ties
+// by tile index, NaN last), order[nt + r] = the first queue position of that
+// tile's megapixels, order[2 nt] = their total.
+__global__ __launch_bounds__(1024) void k_plan_predict(Descs ds, const uint32_t* tile_base,
+                                                        int ntiles, int nslab, int tile, int sub,
+                                                        int dtype, int conn, const float* part,
+                                                        float* raw, const void* params,
+                                                        float* out, uint32_t* count,
+                                                        uint32_t* flag, uint32_t seq,
+                                                        uint32_t* order) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const ImgDesc dd = ds.d ? ds.d[blockIdx.x] : ds.one;
+    const int t0 = tile_base ? (int)tile_base[blockIdx.x] : 0;
+    const int t1 = tile_base ? (int)tile_base[blockIdx.x + 1] : ntiles;
+    const MlpLane m = mlp_lane<true>(params, lane);  // in flight during the fold
+    // raw statistics of the image's tiles: in LDS when they fit
+    __shared__ float sraw[kRawLds * kStats];
+    float* rw = t1 - t0 <= kRawLds ? sraw : raw + (size_t)t0 * kStats;
+    for (int t = t0 + wv; t < t1; t += nw) fold_tile(part, nslab, t, rw + (size_t)(t - t0) * kStats);
+    __syncthreads();
+    __shared__ float smax[16];
+    __shared__ float feat[16][16];
+    float mx = 0.f;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) mx = fmaxf(mx, rw[(size_t)(t - t0) * kStats + 15]);
+    mx = wave_reduce(mx, true);
+    if (lane == 0) smax[wv] = mx;
+    __syncthreads();
+    float M = 0.f;
+    for (int w = 0; w < nw; ++w) M = fmaxf(M, smax[w]);
+    const float log2px = log2f((float)dd.W * (float)dd.H);
+    __shared__ float key[kOrderMax];
+    __shared__ uint32_t cnt[kOrderMax];
+    __shared__ uint32_t wsum[16];
+    for (int t = t0 + wv; t < t1; t += nw) {
+        if (lane == 0) normalise(rw + (size_t)(t - t0) * kStats, M, tile, sub, dtype, conn, log2px, feat[wv]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const float xv = lane < 16 ? feat[wv][lane] : 0.f;
+        float y0, y1, h1, h2;
+        mlp_sample(m, xv, y0, y1, h1, h2);
+        if (lane == 0) {
+            st_host(out + (size_t)t * 2, y0);
+            st_host(out + (size_t)t * 2 + 1, y1);
+            if (order) key[t - t0] = isnan(y1) ? -INFINITY : y1;
+        }
+        __builtin_amdgcn_wave_barrier();  // feat[wv] is rewritten for the next tile
+    }
+    if (order) {
+        const int nt = t1 - t0, ntx = (dd.W + tile - 1) / tile, mpt = tile / 64;
+        __syncthreads();
+        // rank of tile t = tiles before it in the order; megapixels by rank
+        const int t = (int)threadIdx.x;  // nt <= kOrderMax = blockDim.x
+        int rk = 0;
+        if (t < nt) {
+            const float p = key[t];
+            for (int u = 0; u < nt; ++u) {
+                const float q = key[u];
+                rk += (q > p) || (q == p && u < t);
+            }
+            const int tx = t % ntx, ty = t / ntx;
+            cnt[rk] = (uint32_t)(min(mpt, dd.nmx - tx * mpt) * min(mpt, dd.nmy - ty * mpt));
+            order[rk] = (uint32_t)t;
+        }
+        __syncthreads();
+        // exclusive prefix sum of cnt over ranks: lanes, then waves
+        const uint32_t c = t < nt ? cnt[t] : 0u;
+        uint32_t v = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(v, d, 64);
+            if (lane >= d) v += o;
+        }
+        if (lane == 63) wsum[wv] = v;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int w = 0; w < wv; ++w) before += wsum[w];
+        if (t < nt) order[nt + t] = before + v - c;
+        if (t == nt - 1) order[2 * nt] = before + v;
+    }
+    // The outputs went out as system-scope stores, which no cache keeps:
+    // waiting until they have completed puts them before the flag.  (A
+    // system-scope release fence would do it too, but writes back the whole
+    // L2 -- dirty with the row pass just before -- and cost ~10 us.)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t done = gridDim.x == 1 ? 1u
+                                             : __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (done == gridDim.x) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
@@ -436,6 +629,33 @@ hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, 
                      im.pi % 16 == 0;
     return launch_tile_features_batch(ds, nullptr, nt, v16, dtype, conn, tile, sub, part, raw,
                                       feat, s);
+}
+
+hipError_t launch_plan_features(const Descs& ds, const uint32_t* tile_base, int ntiles, bool v16,
+                                int dtype, int tile, int sub, float* part, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    const int nslab = features_slabs(tile, sub);
+    const int grid = ntiles * nslab;
+    if (dtype == 1) {
+        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint8_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+        else hipLaunchKernelGGL((k_tile_features_part<uint8_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+    } else {
+        if (v16) hipLaunchKernelGGL((k_tile_features_part<uint16_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+        else hipLaunchKernelGGL((k_tile_features_part<uint16_t, 4>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_predict(const Descs& ds, const uint32_t* tile_base, int ntiles, int dtype,
+                               int conn, int tile, int sub, const float* part, float* raw,
+                               const void* params, float* out, uint32_t* count, uint32_t* flag,
+                               uint32_t seq, uint32_t* order, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    if (order && (tile_base || ntiles > kOrderMax)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_plan_predict, tile_base ? ds.n : 1, 1024, 0, s, ds, tile_base, ntiles,
+                       features_slabs(tile, sub), tile, sub, dtype, conn, part, raw, params, out,
+                       count, flag, seq, order);
+    return hipGetLastError();
 }
 
 hipError_t launch_mlp_forward(const void* params, bool bf16, const float* x, int n, float* y,

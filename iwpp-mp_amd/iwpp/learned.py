@@ -16,7 +16,7 @@ Traces.  For one training image, with the schedule's tile size:
   * the tile's mean reconstructed value.
 Labels.  For every training image the whole reconstruction is timed with
 k = 0..4 uniform sweep rounds; every tile of the image gets the fastest k,
-(the fewest rounds among those within 5% of it), and priority_t = mean
+(the fewest rounds among those within 2% of it), and priority_t = mean
 reconstructed value of t / max mask value, so tiles
 whose flood carries the highest values are seeded first (their propagation
 overwrites lower floods, which would otherwise be propagated and then
@@ -45,7 +45,8 @@ META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
 
 MAX_ROUNDS = 4
 FEATURE_SUB = 16  # row subsampling of the features (the library uses the same)
-FEATURE_SET = 2   # feature definitions (2: feature 15 = log2(image pixels) / 32)
+FEATURE_SET = 4   # feature definitions (2: feature 15 = log2(image pixels) / 32;
+                  # 4: left/right border inflow on the sampled rows only)
 
 
 def schedule_tile(width, height):
@@ -203,7 +204,7 @@ def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=5, workspace=Non
     return out
 
 
-TIE = 1.05  # timings within 5% count as equal (the smaller k wins)
+TIE = 1.02  # timings within 2% count as equal (the smaller k wins)
 
 
 def labels(tr, timings):

@@ -102,8 +102,9 @@ def tile_features_raw(J, I, tile, sub):
     6 #(J==0), 7 #(J==I>0), 8 max J, 9 sum |I(x+1)-I(x)| over pairs with x even
     and x+1 in the tile,
     10..13 top/bottom/left/right border inflow sum max(0, min(J(out), I(in)) - J(in)),
-    14 pixels sampled, 15 max I.  Statistics 0-9, 14, 15 use every `sub`-th
-    row from the tile's first; the border terms use every border pixel."""
+    14 pixels sampled, 15 max I.  Statistics 0-9, 12-15 use every `sub`-th
+    row from the tile's first; the top and bottom border terms use every
+    pixel of the row."""
     J = np.asarray(J, dtype=np.float64)
     I = np.asarray(I, dtype=np.float64)
     H, W = J.shape
@@ -131,10 +132,11 @@ def tile_features_raw(J, I, tile, sub):
                 r[11] = np.maximum(0, np.minimum(J[y1, x0:x1], I[y1 - 1, x0:x1])
                                    - J[y1 - 1, x0:x1]).sum()
             if x0 > 0:
-                r[12] = np.maximum(0, np.minimum(J[y0:y1, x0 - 1], I[y0:y1, x0]) - J[y0:y1, x0]).sum()
+                r[12] = np.maximum(0, np.minimum(J[y0:y1:sub, x0 - 1], I[y0:y1:sub, x0])
+                                   - J[y0:y1:sub, x0]).sum()
             if x1 < W:
-                r[13] = np.maximum(0, np.minimum(J[y0:y1, x1], I[y0:y1, x1 - 1])
-                                   - J[y0:y1, x1 - 1]).sum()
+                r[13] = np.maximum(0, np.minimum(J[y0:y1:sub, x1], I[y0:y1:sub, x1 - 1])
+                                   - J[y0:y1:sub, x1 - 1]).sum()
             r[14] = j.size
             r[15] = i.max()
     return out
@@ -144,7 +146,8 @@ def normalise(raw, tile, sub, dtype, conn, pixels):
     """The model's inputs from raw statistics; M = the largest max I over all
     tiles; features 1 and 14 carry the connectivity (8 -> 1) and the pixel
     type (uint16 -> 1), feature 15 the image's size, min(log2(pixels), 28) / 32.
-    (`sub` no longer enters: kept so that calls name the sampling used.)"""
+    The left/right border sums, taken on every `sub`-th row, are scaled by
+    `sub` to the size of the top/bottom ones."""
     raw = np.asarray(raw, dtype=np.float64)
     M = max(raw[:, 15].max(), 1.0)
     n = np.maximum(raw[:, 14], 1.0)
@@ -160,7 +163,7 @@ def normalise(raw, tile, sub, dtype, conn, pixels):
     f[:, 8] = raw[:, 8] / M
     f[:, 9] = raw[:, 9] / n / M * 10.0
     for k in range(10, 14):
-        f[:, k] = raw[:, k] / tile / M
+        f[:, k] = raw[:, k] * (sub if k >= 12 else 1) / tile / M
     f[:, 14] = 1.0 if np.dtype(dtype).itemsize == 2 else 0.0
     f[:, 15] = min(np.log2(float(pixels)), 28.0) / 32.0
     return f
