@@ -708,7 +708,7 @@ int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
     float* pred = (float*)ws->host_plan;
     HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
                  (size_t)L.total * features_slabs(L.tile, kFeatureSub) * 16));
-    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)L.total * 16));
+    HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)L.total * 19));  // statistics, outputs, priorities
     if (n == 1) {
         const ImgDesc& d = r.h[0];
         Descs one{};

@@ -26,7 +26,10 @@ namespace iwpp {
 // normalises them per image.
 // Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
 constexpr int kSlab = 4;
-constexpr int kRawLds = 256;  // tiles of one image whose statistics the plan keeps in LDS
+constexpr int kRawLds = 256;  // tiles of one unit whose statistics the plan keeps in LDS
+constexpr int kPlanThreads = 512;  // workgroup of the plan's network: 8 waves, 256 VGPRs each
+constexpr int kPlanTiles = kPlanThreads / 64;  // tiles per workgroup (one image): one per wave
+static_assert(kOrderMax <= kPlanThreads, "the seeding order takes one tile per thread");
 constexpr int kStats = 16;
 
 DEV bool is_max_stat(int k) { return k == 4 || k == 8 || k == 15; }
@@ -396,71 +399,117 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const void* params, const f
     }
 }
 
-// The plan's network outputs from the slab partials, one workgroup (16
-// waves) per image: fold the slabs of each tile (as k_features_tiles), the
-// image's largest mask value, each tile's normalised features (as
-// k_features_finish) and the network's two outputs (as k_mlp_forward with
-// BF16 weights), written to `out` -- the host's pinned staging, mapped.  The
-// last workgroup to finish writes `seq` to `flag` (host memory as well), which
-// the host polls: no copy and no event between the network and the host's
-// plan.  `count` (device, zero) counts finished workgroups and is left zero.
-// With `order` (one image of at most kOrderMax tiles), the seeding order goes
-// there as well: order[r] = the tile of rank r by decreasing priority (ties
-// by tile index, NaN last), order[nt + r] = the first queue position of that
-// tile's megapixels, order[2 nt] = their total.
-__global__ __launch_bounds__(1024) void k_plan_predict(Descs ds, const uint32_t* tile_base,
+// The plan's network outputs from the slab partials.  A workgroup (8 waves)
+// takes one unit: in a batch one image, for one image kPlanTiles of its
+// tiles.  For each tile: fold its slabs (as k_features_tiles), normalise with
+// the image's largest mask value (the maximum of statistic 15 over the
+// image's partials: the same value k_features_finish takes from the folded
+// tiles), run the network (BF16 weights, as k_mlp_forward) and write the two
+// outputs to `out` -- the host's pinned staging, mapped.  The last workgroup
+// to finish writes `seq` to `flag` (host memory as well), which the host
+// polls: no copy and no event between the network and the host's plan.
+// `count` (device, zero) counts finished workgroups and is left zero.
+// With `order` (one image of at most kOrderMax tiles) the last workgroup also
+// writes the seeding order: order[r] = the tile of rank r by decreasing
+// priority (ties by tile index, NaN last), order[nt + r] = the first queue
+// position of that tile's megapixels, order[2 nt] = their total; `prio`
+// (device, nt floats) carries the priorities between workgroups.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_predict(Descs ds, const uint32_t* tile_base,
                                                         int ntiles, int nslab, int tile, int sub,
                                                         int dtype, int conn, const float* part,
                                                         float* raw, const void* params,
                                                         float* out, uint32_t* count,
                                                         uint32_t* flag, uint32_t seq,
-                                                        uint32_t* order) {
+                                                        float* prio, uint32_t* order) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const ImgDesc dd = ds.d ? ds.d[blockIdx.x] : ds.one;
-    const int t0 = tile_base ? (int)tile_base[blockIdx.x] : 0;
-    const int t1 = tile_base ? (int)tile_base[blockIdx.x + 1] : ntiles;
+    const bool one = tile_base == nullptr;
+    const int img = one ? 0 : (int)blockIdx.x;
+    const ImgDesc dd = ds.d ? ds.d[img] : ds.one;
+    const int i0 = one ? 0 : (int)tile_base[img], i1 = one ? ntiles : (int)tile_base[img + 1];
+    const int t0 = one ? (int)blockIdx.x * kPlanTiles : i0;
+    const int t1 = one ? min(ntiles, t0 + kPlanTiles) : i1;
     const MlpLane m = mlp_lane<true>(params, lane);  // in flight during the fold
-    // raw statistics of the image's tiles: in LDS when they fit
-    __shared__ float sraw[kRawLds * kStats];
-    float* rw = t1 - t0 <= kRawLds ? sraw : raw + (size_t)t0 * kStats;
-    for (int t = t0 + wv; t < t1; t += nw) fold_tile(part, nslab, t, rw + (size_t)(t - t0) * kStats);
-    __syncthreads();
     __shared__ float smax[16];
-    __shared__ float feat[16][16];
-    float mx = 0.f;
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) mx = fmaxf(mx, rw[(size_t)(t - t0) * kStats + 15]);
-    mx = wave_reduce(mx, true);
-    if (lane == 0) smax[wv] = mx;
-    __syncthreads();
-    float M = 0.f;
-    for (int w = 0; w < nw; ++w) M = fmaxf(M, smax[w]);
-    const float log2px = log2f((float)dd.W * (float)dd.H);
     __shared__ float key[kOrderMax];
     __shared__ uint32_t cnt[kOrderMax];
     __shared__ uint32_t wsum[16];
-    for (int t = t0 + wv; t < t1; t += nw) {
-        if (lane == 0) normalise(rw + (size_t)(t - t0) * kStats, M, tile, sub, dtype, conn, log2px, feat[wv]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const float xv = lane < 16 ? feat[wv][lane] : 0.f;
-        float y0, y1, h1, h2;
-        mlp_sample(m, xv, y0, y1, h1, h2);
-        if (lane == 0) {
-            st_host(out + (size_t)t * 2, y0);
-            st_host(out + (size_t)t * 2 + 1, y1);
-            if (order) key[t - t0] = isnan(y1) ? -INFINITY : y1;
+    __shared__ int last;
+    // the image's largest mask value, from its partials
+    float mx = 0.f;
+    for (int i = i0 * nslab + (int)threadIdx.x; i < i1 * nslab; i += blockDim.x)
+        mx = fmaxf(mx, part[(size_t)i * kStats + 15]);
+    mx = wave_reduce(mx, true);
+    if (lane == 0) smax[wv] = mx;
+    const float log2px = log2f((float)dd.W * (float)dd.H);
+    // This unit's tiles: fold, features and network.  rw holds their raw
+    // statistics, y their outputs; both in LDS when they fit (instantiated
+    // apart, so that the compiler sees LDS accesses).
+    auto tiles = [&](float* rw, float* y) {
+        for (int tb = t0 + wv; tb < t1; tb += 4 * nw) {  // four tiles' partials in flight
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = tb + u * nw;
+                if (t < t1) fold_tile(part, nslab, t, rw + (size_t)(t - t0) * kStats);
+            }
         }
-        __builtin_amdgcn_wave_barrier();  // feat[wv] is rewritten for the next tile
+        __syncthreads();
+        float M = 0.f;
+        for (int w = 0; w < nw; ++w) M = fmaxf(M, smax[w]);
+        for (int t = t0 + wv; t < t1; t += nw) {
+            float f[16];  // every lane computes them; lane k < 16 keeps feature k
+            normalise(rw + (size_t)(t - t0) * kStats, M, tile, sub, dtype, conn, log2px, f);
+            float xv = 0.f;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) xv = lane == k ? f[k] : xv;
+            float y0, y1, h1, h2;
+            mlp_sample(m, xv, y0, y1, h1, h2);
+            if (lane == 0) {
+                y[(size_t)(t - t0) * 2] = y0;
+                y[(size_t)(t - t0) * 2 + 1] = y1;
+                if (order)  // shared with the other workgroups, past the caches
+                    __hip_atomic_store(prio + t, isnan(y1) ? -INFINITY : y1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < 2 * (t1 - t0); i += blockDim.x)
+            st_host(out + (size_t)t0 * 2 + i, y[i]);
+    };
+    if (t1 - t0 <= kRawLds) {
+        __shared__ float sraw[kRawLds * kStats];
+        __shared__ float sy[kRawLds * 2];
+        tiles(sraw, sy);
+    } else {
+        tiles(raw + (size_t)t0 * kStats, raw + (size_t)ntiles * kStats + (size_t)t0 * 2);
     }
-    if (order) {
-        const int nt = t1 - t0, ntx = (dd.W + tile - 1) / tile, mpt = tile / 64;
+    // The outputs went out as stores that no device cache keeps (system scope
+    // to the host, device scope for prio): waiting until they have completed
+    // puts them before this workgroup's count, and so before the flag.  (A
+    // release fence would do it too, but writes back the whole L2 -- dirty
+    // with the row pass just before -- and cost ~10 us.)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t done = gridDim.x == 1 ? 1u
+                                             : __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        last = done == gridDim.x;
+        if (last && gridDim.x > 1) __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (!last) return;
+    if (order) {  // one image: every workgroup's priorities are in prio
+        const int nt = ntiles, ntx = (dd.W + tile - 1) / tile, mpt = tile / 64;
+        for (int t = (int)threadIdx.x; t < nt; t += blockDim.x)
+            key[t] = __hip_atomic_load(prio + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         // rank of tile t = tiles before it in the order; megapixels by rank
         const int t = (int)threadIdx.x;  // nt <= kOrderMax = blockDim.x
-        int rk = 0;
         if (t < nt) {
             const float p = key[t];
+            int rk = 0;
             for (int u = 0; u < nt; ++u) {
                 const float q = key[u];
                 rk += (q > p) || (q == p && u < t);
@@ -485,23 +534,7 @@ __global__ __launch_bounds__(1024) void k_plan_predict(Descs ds, const uint32_t*
         if (t < nt) order[nt + t] = before + v - c;
         if (t == nt - 1) order[2 * nt] = before + v;
     }
-    // The outputs went out as system-scope stores, which no cache keeps:
-    // waiting until they have completed puts them before the flag.  (A
-    // system-scope release fence would do it too, but writes back the whole
-    // L2 -- dirty with the row pass just before -- and cost ~10 us.)
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t done = gridDim.x == 1 ? 1u
-                                             : __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED,
-                                                                      __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        if (done == gridDim.x) {
-            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-    }
+    if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Backward per sample for the weighted squared loss
@@ -652,9 +685,13 @@ hipError_t launch_plan_predict(const Descs& ds, const uint32_t* tile_base, int n
                                uint32_t seq, uint32_t* order, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
     if (order && (tile_base || ntiles > kOrderMax)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_plan_predict, tile_base ? ds.n : 1, 1024, 0, s, ds, tile_base, ntiles,
+    const int grid = tile_base ? ds.n : (ntiles + kPlanTiles - 1) / kPlanTiles;
+    // raw: statistics (ntiles x 16) and outputs (ntiles x 2) of units too
+    // large for LDS, then the priorities (ntiles) shared by the workgroups
+    float* prio = raw + (size_t)ntiles * 18;
+    hipLaunchKernelGGL(k_plan_predict, grid, kPlanThreads, 0, s, ds, tile_base, ntiles,
                        features_slabs(tile, sub), tile, sub, dtype, conn, part, raw, params, out,
-                       count, flag, seq, order);
+                       count, flag, seq, prio, order);
     return hipGetLastError();
 }
 

@@ -364,7 +364,7 @@ def set_model(weights):
     _raise(lib().iwpp_sched_set_model(w.ctypes.data, w.size))
 
 
-def tile_features(marker, mask, tile, connectivity=8, sub=16, workspace=None, stream=None):
+def tile_features(marker, mask, tile, connectivity=8, sub=32, workspace=None, stream=None):
     """(raw, feat): torch float32 device tensors of shape (tiles, 16)."""
     torch = _torch()
     ntx = (marker.w + tile - 1) // tile

@@ -44,7 +44,7 @@ MODEL_PATH = os.path.join(MODEL_DIR, "sched_mlp.f32")
 META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
 
 MAX_ROUNDS = 4
-FEATURE_SUB = 16  # row subsampling of the features (the library uses the same)
+FEATURE_SUB = 32  # row subsampling of the features (the library uses the same)
 FEATURE_SET = 4   # feature definitions (2: feature 15 = log2(image pixels) / 32;
                   # 4: left/right border inflow on the sampled rows only)
 
