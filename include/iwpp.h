@@ -75,9 +75,9 @@ typedef struct {
     float ms_sweep;          /* device time of the sweep phase (HIP events) */
     float ms_seed;           /* device time of the seeding passes */
     float ms_queue;          /* device time of the queue-phase kernels */
-    float ms_plan;           /* host time of the learned schedule's planning: tile
-                                features, network, read-back (includes waiting for
-                                earlier work on the stream) */
+    float ms_plan;           /* host time the learned schedule waited for the
+                                network's outputs (the tile features, the first
+                                row pass and the network run meanwhile) */
     /* With iwpp_options.trace_phases: wave-microseconds spent in the queue
      * phase, summed over all waves (the reference's clock64 split,
      * IWPP MP/src/MorphologicalReconstruction.cu:199-231,534-540). */
@@ -93,7 +93,7 @@ typedef struct {
     int sweep_rounds;     /* rounds of (row pass, column pass down, column pass up)
                              before seeding the queue; -1 = the schedule decides */
     int col_block_rows;   /* rows per block of the column passes (0 = from the
-                             image size, >= 8192 waves, default) */
+                             image size, default) */
     int max_local_iters;  /* local sweep rounds per megapixel visit before it
                              re-queues itself (default 8) */
     int queue_capacity;   /* entries per queue; 0 = sized from the image.
