@@ -1062,7 +1062,7 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
 // and the interior goes back to LDS once, if it changed.  A lane owns its row
 // in a row sweep (its column in a column sweep): no other lane touches those
 // words.  ch[0], ch[1]: whether each direction changed a pixel.
-DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2]) {
+DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward) {
     const int base = tix(lane, -1);
     uint32_t w[66];
 #pragma unroll
@@ -1076,6 +1076,7 @@ DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2]) {
         w[c] = nw;
         p = nw;
     }
+    if (last_forward && !wave_any(ch[0])) return;  // converged: nothing to sweep or write
     p = w[65];
 #pragma unroll
     for (int c = kMP; c >= 1; --c) {
@@ -1093,8 +1094,14 @@ DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2]) {
 // Column sweeps over the interior, lane = column: top to bottom, then bottom
 // to top, the same way.  8-connectivity adds the previous row's left/right
 // neighbours (DPP), lanes 0 and 63 taking them from the halo columns.
-template <int CONN>
-DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2]) {
+// 8-connectivity reads the halo columns' two pixels of the previous row at
+// every step; they are read from LDS G steps at a time, at the start of each
+// group (scheduling barriers keep them there: read one step ahead, each step
+// waited on its LDS read; read all at once, 128 live values would halve the
+// waves per SIMD).
+template <int CONN, int G>
+DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward) {
+    static_assert(kMP % G == 0, "whole groups");
     const int base = tix(-1, lane);
     uint32_t w[66];
 #pragma unroll
@@ -1105,23 +1112,32 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2]) {
         const int DIR = d == 0 ? 1 : -1;
         uint32_t p = d == 0 ? w[0] : w[65];
 #pragma unroll
-        for (int k = 0; k < kMP; ++k) {
-            const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
-            const int rp = r - DIR;                    // ... and of the row before it
-            uint32_t q = p;
-            if constexpr (CONN == 8) {  // upd reads only q's low half
-                const uint32_t hl = s[rp * TS], hr = s[rp * TS + 65];  // halo columns of row rp
-                q = max3_lo(p, from_left(p, hl), from_right(p, hr));
+        for (int k0 = 0; k0 < kMP; k0 += G) {
+            uint32_t hl[G], hr[G];  // halo columns of the rows before this group's rows
+            if constexpr (CONN == 8) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int rp = DIR > 0 ? k0 + i : kMP + 1 - (k0 + i);
+                    hl[i] = s[rp * TS];
+                    hr[i] = s[rp * TS + 65];
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            const uint32_t nw = upd(w[r], q);
-            ch[d] |= nw != w[r];
-            w[r] = nw;
-            p = nw;
-            // keep the scheduler from hoisting every halo read of the sweep
-            // (128 live values would halve the waves per SIMD)
-            if constexpr (CONN == 8)
-                if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const int k = k0 + i;
+                const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
+                uint32_t q = p;
+                if constexpr (CONN == 8)  // upd reads only q's low half
+                    q = max3_lo(p, from_left(p, hl[i]), from_right(p, hr[i]));
+                const uint32_t nw = upd(w[r], q);
+                ch[d] |= nw != w[r];
+                w[r] = nw;
+                p = nw;
+            }
+            if constexpr (CONN == 8) __builtin_amdgcn_sched_barrier(0);
         }
+        if (d == 0 && last_forward && !wave_any(ch[0])) return;  // converged
     }
     if (ch[0] || ch[1]) {
 #pragma unroll
@@ -1129,20 +1145,27 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2]) {
     }
 }
 
+// Halo reads per group of column steps: uint8 visits have the registers for
+// 16 (two waves per SIMD either way); uint16 ones keep to one step ahead.
+template <class T>
+constexpr int kHaloGroup = sizeof(T) == 1 ? 16 : 1;
+
 // Sweeps until the four directions in a row change nothing, or max_sweeps:
 // rows forward, rows backward, columns forward, columns backward, in turn.
 // Each sweep is idempotent, so three quiet sweeps after the last change mean
 // every direction leaves the tile as it is.  Two sweeps share one pass over
-// the registers.  `changed`: some sweep raised a pixel.
-template <int CONN>
+// the registers; when two quiet sweeps precede a pair and its forward sweep
+// is quiet too, the tile has converged and the backward sweep is left out.
+// `changed`: some sweep raised a pixel.
+template <int CONN, int G>
 DEV bool tile_converge(uint32_t* s, int max_sweeps, bool& changed) {
     const int lane = lane_id();
     int quiet = -1;
     changed = false;
     for (int k = 0; k < max_sweeps; k += 2) {
         bool ch[2];
-        if ((k & 2) == 0) sweep_rows_pair(s, lane, ch);
-        else sweep_cols_pair<CONN>(s, lane, ch);
+        if ((k & 2) == 0) sweep_rows_pair(s, lane, ch, quiet == 2);
+        else sweep_cols_pair<CONN, G>(s, lane, ch, quiet == 2);
         wave_lds_sync();
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
@@ -1795,7 +1818,7 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
         const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
         const Border ob = tile_border(s);
         bool changed;
-        const bool conv = tile_converge<CONN>(s, 4 * q.max_iters, changed);
+        const bool conv = tile_converge<CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
         pt.lap(2);
         tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
         uint32_t m2 = 0;
@@ -1904,7 +1927,7 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
             const Border ob = tile_border(s);
             pt.lap(1);
             bool changed;
-            const bool conv = tile_converge<CONN>(s, 4 * q.max_iters, changed);
+            const bool conv = tile_converge<CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
             pt.lap(2);
             bool rose_any = changed;
             if constexpr (EXCL) tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
