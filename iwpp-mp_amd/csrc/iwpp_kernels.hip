@@ -980,8 +980,63 @@ DEV uint32_t ldp_coh(const uint8_t* base, size_t pitch, int x, int y) {
 
 // Loads the megapixel and its halo into the wave's LDS tile.  Rows are read as
 // 32-bit words (4 u8 / 2 u16 pixels per lane), 4 or 2 rows per instruction.
+// uint8: the 66 pixels of a row with its halo lie in the 18 words from the
+// one holding pixel x0 - 1 (pitches and bases are multiples of 4); 18 lanes
+// per row, 3 rows per load instruction, so the halo columns come with the
+// rows (one cache line or two per row) instead of one line per halo pixel.
+DEV void tile_fetch_u8(const Image& im, int mx, int my, uint32_t* s) {
+    constexpr int WPR = 18, RPI = 3, NIT = 22, NB = 1, BIT = NIT / NB;
+    const int lane = lane_id();
+    const int x0 = mx * kMP, y0 = my * kMP;
+    const int rl = lane / WPR, d = lane - rl * WPR;  // rl == 3: idle lanes
+    const int xw = x0 - 4 + 4 * d;                   // first pixel of the lane's word
+    const size_t xoff = (size_t)min(max(xw, 0), (im.W - 1) & ~3);
+    // per lane, fixed over the rows: which of its 4 pixels fall in the tile's
+    // frame (columns -1..64), and which of those hold image pixels
+    const bool hasL = x0 > 0, hasR = x0 + kMP < im.W;
+    uint32_t frame = 0, inside = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int c = 4 * d - 4 + k;
+        if (rl < RPI && c >= -1 && c <= 64) {
+            frame |= 1u << k;
+            if (x0 + c < im.W && (c >= 0 || hasL) && (c < 64 || hasR)) inside |= 1u << k;
+        }
+    }
+    const int base = rl * TS + 4 * d - 3;  // LDS index of row -1 + rl, column 4d - 4
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        uint32_t gj[BIT], gi[BIT];
+#pragma unroll
+        for (int u = 0; u < BIT; ++u) {  // clamped addresses: always valid
+            const int yc = min(max(y0 - 1 + (b * BIT + u) * RPI + rl, 0), im.H - 1);
+            gj[u] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
+            gi[u] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff));
+        }
+#pragma unroll
+        for (int u = 0; u < BIT; ++u) {
+            const int it = b * BIT + u, row = -1 + it * RPI + rl;
+            if (row > 64) continue;
+            const uint32_t in = (y0 + row >= 0 && y0 + row < im.H) ? inside : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!((frame >> k) & 1u)) continue;
+                const bool v = (in >> k) & 1u;
+                const uint32_t j = v ? (gj[u] >> (8 * k)) & 0xFFu : 0u;
+                const uint32_t m = v ? (gi[u] >> (8 * k)) & 0xFFu : 0u;
+                s[base + it * RPI * TS + k] = j | (m << 16);
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
 template <class T>
 DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
+    if constexpr (sizeof(T) == 1) {
+        tile_fetch_u8(im, mx, my, s);
+        return;
+    }
     constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
     constexpr int BITS = 8 * (int)sizeof(T);
     constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
