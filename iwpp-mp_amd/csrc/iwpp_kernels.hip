@@ -966,31 +966,34 @@ DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+
 template <class T>
 DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
     return *gp(reinterpret_cast<const T*>(base + (size_t)y * pitch + (size_t)x * sizeof(T)));
 }
-// One marker pixel through a device-coherent load of its (4-byte aligned) word.
-template <class T>
-DEV uint32_t ldp_coh(const uint8_t* base, size_t pitch, int x, int y) {
-    const size_t off = (size_t)x * sizeof(T);
-    const uint32_t w = ld_coh(base + (size_t)y * pitch + (off & ~(size_t)3));
-    return (w >> (8 * (off & 3))) & (uint32_t)((1u << (8 * sizeof(T))) - 1u);
-}
 
-// Loads the megapixel and its halo into the wave's LDS tile.  Rows are read as
-// 32-bit words (4 u8 / 2 u16 pixels per lane), 4 or 2 rows per instruction.
-// uint8: the 66 pixels of a row with its halo lie in the 18 words from the
-// one holding pixel x0 - 1 (pitches and bases are multiples of 4); 18 lanes
-// per row, 3 rows per load instruction, so the halo columns come with the
-// rows (one cache line or two per row) instead of one line per halo pixel.
-DEV void tile_fetch_u8(const Image& im, int mx, int my, uint32_t* s) {
-    constexpr int WPR = 18, RPI = 3, NIT = 22, NB = 1, BIT = NIT / NB;
+// Loads the megapixel and its halo into the wave's LDS tile (absent pixels 0).
+// The 66 pixels of a row with its halo lie among the 72 from x0 - 4 (x0 - 4
+// is a multiple of 4 pixels, and pitches and bases of 4 bytes): 18 lanes per
+// row, 4 pixels each (one 32-bit word for uint8, two for uint16), 3 rows per
+// load instruction, so the halo columns come with the rows (one cache line or
+// two per row) instead of one line per halo pixel.  All loads of a batch are
+// issued before any is used, from clamped (always valid) addresses; uint16
+// loads in two batches (the registers of one would halve the waves per SIMD).
+template <class T>
+DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
+    constexpr int WPL = (int)sizeof(T);                 // 32-bit words per lane
+    constexpr int BITS = 8 * (int)sizeof(T), PPW = 4 / (int)sizeof(T);
+    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
+    constexpr int RPI = 3, NIT = 22, NB = (int)sizeof(T), BIT = NIT / NB;
     const int lane = lane_id();
     const int x0 = mx * kMP, y0 = my * kMP;
-    const int rl = lane / WPR, d = lane - rl * WPR;  // rl == 3: idle lanes
-    const int xw = x0 - 4 + 4 * d;                   // first pixel of the lane's word
-    const size_t xoff = (size_t)min(max(xw, 0), (im.W - 1) & ~3);
+    const int rl = lane / 18, d = lane - rl * 18;  // rl == 3: idle lanes
+    const int xw = x0 - 4 + 4 * d;                  // first pixel of the lane's 4
+    const int lastw = (im.W * (int)sizeof(T) - 1) & ~3;  // the row's last word (bytes)
+    size_t off[WPL];
+#pragma unroll
+    for (int w = 0; w < WPL; ++w) off[w] = (size_t)min(max(xw * (int)sizeof(T) + 4 * w, 0), lastw);
     // per lane, fixed over the rows: which of its 4 pixels fall in the tile's
     // frame (columns -1..64), and which of those hold image pixels
     const bool hasL = x0 > 0, hasR = x0 + kMP < im.W;
@@ -1006,12 +1009,15 @@ DEV void tile_fetch_u8(const Image& im, int mx, int my, uint32_t* s) {
     const int base = rl * TS + 4 * d - 3;  // LDS index of row -1 + rl, column 4d - 4
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        uint32_t gj[BIT], gi[BIT];
+        uint32_t gj[BIT][WPL], gi[BIT][WPL];
 #pragma unroll
-        for (int u = 0; u < BIT; ++u) {  // clamped addresses: always valid
+        for (int u = 0; u < BIT; ++u) {
             const int yc = min(max(y0 - 1 + (b * BIT + u) * RPI + rl, 0), im.H - 1);
-            gj[u] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
-            gi[u] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff));
+#pragma unroll
+            for (int w = 0; w < WPL; ++w) {
+                gj[u][w] = ld_coh(im.J + (size_t)yc * im.pj + off[w]);
+                gi[u][w] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + off[w]));
+            }
         }
 #pragma unroll
         for (int u = 0; u < BIT; ++u) {
@@ -1022,91 +1028,11 @@ DEV void tile_fetch_u8(const Image& im, int mx, int my, uint32_t* s) {
             for (int k = 0; k < 4; ++k) {
                 if (!((frame >> k) & 1u)) continue;
                 const bool v = (in >> k) & 1u;
-                const uint32_t j = v ? (gj[u] >> (8 * k)) & 0xFFu : 0u;
-                const uint32_t m = v ? (gi[u] >> (8 * k)) & 0xFFu : 0u;
+                const uint32_t j = v ? (gj[u][k / PPW] >> (BITS * (k % PPW))) & PM : 0u;
+                const uint32_t m = v ? (gi[u][k / PPW] >> (BITS * (k % PPW))) & PM : 0u;
                 s[base + it * RPI * TS + k] = j | (m << 16);
             }
         }
-    }
-    wave_lds_sync();
-}
-
-template <class T>
-DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
-    if constexpr (sizeof(T) == 1) {
-        tile_fetch_u8(im, mx, my, s);
-        return;
-    }
-    constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
-    constexpr int BITS = 8 * (int)sizeof(T);
-    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
-    const int lane = lane_id();
-    const int x0 = mx * kMP, y0 = my * kMP;
-    const int wl = lane % WPR, rl = lane / WPR;
-    const int xw = x0 + wl * PPW;
-    const bool xin = xw < im.W;
-    // All loads of a batch first, unconditionally from clamped (always valid)
-    // addresses so that they issue back to back; out-of-image words are
-    // zeroed afterwards.  uint16 tiles take two batches (one would hold 66
-    // registers of loads and halve the waves per SIMD).
-    constexpr int NIT = (66 + RPI - 1) / RPI;
-    constexpr int NB = sizeof(T) == 1 ? 1 : 2, BIT = (NIT + NB - 1) / NB;
-    const size_t xoff = (size_t)(xin ? xw : x0) * sizeof(T);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        uint32_t gj[BIT], gi[BIT];
-#pragma unroll
-        for (int u = 0; u < BIT; ++u) {
-            const int it = b * BIT + u;
-            const int y = y0 - 1 + it * RPI + rl;
-            const int yc = min(max(y, 0), im.H - 1);
-            gj[u] = ld_coh(im.J + (size_t)yc * im.pj + xoff);
-            gi[u] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + xoff));
-        }
-#pragma unroll
-        for (int u = 0; u < BIT; ++u) {
-            const int it = b * BIT + u;
-            const int row = -1 + it * RPI + rl;
-            const int y = y0 + row;
-            const bool valid = it < NIT && row <= 64 && xin && y >= 0 && y < im.H;
-            gj[u] = valid ? gj[u] : 0u;
-            gi[u] = valid ? gi[u] : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < BIT; ++u) {
-            const int it = b * BIT + u;
-            const int row = -1 + it * RPI + rl;
-            if (it >= NIT || row > 64) continue;
-#pragma unroll
-            for (int k = 0; k < PPW; ++k) {
-                const bool in = xw + k < im.W;
-                const uint32_t j = in ? (gj[u] >> (BITS * k)) & PM : 0u;
-                const uint32_t m = in ? (gi[u] >> (BITS * k)) & PM : 0u;
-                s[tix(row, wl * PPW + k)] = j | (m << 16);
-            }
-        }
-    }
-    // halo columns x0-1 and x0+64, rows -1..64 (loads clamped the same way)
-    const bool hasL = x0 > 0, hasR = x0 + kMP < im.W;
-    const int xl = hasL ? x0 - 1 : x0, xr = hasR ? x0 + kMP : x0;
-    uint32_t hj[4], hi[4];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int y = y0 - 1 + b * 64 + lane;
-        const int yc = min(max(y, 0), im.H - 1);
-        hj[2 * b] = ldp_coh<T>(im.J, im.pj, xl, yc);
-        hi[2 * b] = ldp<T>(im.I, im.pi, xl, yc);
-        hj[2 * b + 1] = ldp_coh<T>(im.J, im.pj, xr, yc);
-        hi[2 * b + 1] = ldp<T>(im.I, im.pi, xr, yc);
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int row = -1 + b * 64 + lane;
-        const int y = y0 + row;
-        if (row > 64) continue;
-        const bool iny = y >= 0 && y < im.H;
-        s[tix(row, -1)] = (iny && hasL) ? hj[2 * b] | (hi[2 * b] << 16) : 0u;
-        s[tix(row, 64)] = (iny && hasR) ? hj[2 * b + 1] | (hi[2 * b + 1] << 16) : 0u;
     }
     wave_lds_sync();
 }
@@ -1200,10 +1126,10 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward
     }
 }
 
-// Halo reads per group of column steps: uint8 visits have the registers for
-// 16 (two waves per SIMD either way); uint16 ones keep to one step ahead.
+// Halo reads per group of column steps (16 fit the registers beside two
+// waves per SIMD).
 template <class T>
-constexpr int kHaloGroup = sizeof(T) == 1 ? 16 : 1;
+constexpr int kHaloGroup = 16;
 
 // Sweeps until the four directions in a row change nothing, or max_sweeps:
 // rows forward, rows backward, columns forward, columns backward, in turn.
