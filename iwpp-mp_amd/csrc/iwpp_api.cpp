@@ -246,6 +246,15 @@ struct Run {
     const uint32_t* order_tiles = nullptr, *order_start = nullptr;
     int order_nt = 0, order_ntx = 0, order_mpt = 0;
     // learned schedule, between the feature pass and the host's plan
+    // seeding flags set by the sweep phase's last pass (one image, no tile
+    // schedule): the round whose upward column pass sets them, and whether
+    // the flags in the workspace are those of the marker as it now is
+    int seed_round = -1;
+    bool flags_ready = false;
+    // learned plan still to be completed (the network's outputs read, the
+    // rounds fixed): before the first upward column pass, which then sets the
+    // seeding flags if the plan keeps to one round
+    SchedulePlan* plan_end = nullptr;
     bool plan_predict = false;  // the network is still to be launched (after the first rows)
     bool plan_order = false;    // ... and writes the seeding order to ws->order
     int plan_tiles = 0, plan_tile = 0;
@@ -254,6 +263,8 @@ struct Run {
 };
 
 int learned_plan_predict(Run& r);
+int learned_plan_end_single(Run& r, SchedulePlan& plan);
+int learned_plan_end_batch(Run& r, SchedulePlan& plan);
 
 int ensure_events(iwpp_workspace* ws) {
     for (hipEvent_t& e : ws->ev)
@@ -431,8 +442,27 @@ int sweep_phase(Run& r, int k0, int k1) {
         if (rp) return rp;
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
                                       COL_IN_PLACE, r.wide, nullptr, r.s));
+        if (r.plan_end) {  // the host plans while the downward pass runs
+            SchedulePlan& plan = *r.plan_end;
+            r.plan_end = nullptr;
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = r.ds.n == 1 ? learned_plan_end_single(r, plan)
+                                       : learned_plan_end_batch(r, plan);
+            if (rc) return rc;
+            r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - t0).count();
+            r.seed_round = plan.sweep_rounds - 1;
+        }
+        const bool seed = k == r.seed_round && r.ds.n == 1 && !r.round_masks && !r.graph;
+        if (seed) {  // the last pass sets the seeding flags (k_unstable_scan is skipped)
+            const size_t words = ((size_t)r.total_mp + 3) / 4;
+            HIP_TRY(grow(r.ws->flags, r.ws->flags_n, words * 4));
+            HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(r.ws->flags), 0u, words, r.s));
+        }
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
-                                      COL_IN_PLACE, r.wide, nullptr, r.s));
+                                      seed ? COL_IN_PLACE_SEED : COL_IN_PLACE, r.wide, nullptr,
+                                      r.s, seed ? r.ws->flags : nullptr));
+        r.flags_ready = seed;
     }
     r.st.sweep_rounds += std::max(0, k1 - k0);
     return IWPP_OK;
@@ -443,7 +473,7 @@ int setup_queue(Run& r) {
     const int kind = r.queue;
     const size_t nmp = (size_t)r.total_mp;
     HIP_TRY(grow(ws->st, ws->st_n, nmp));
-    HIP_TRY(grow(ws->flags, ws->flags_n, nmp));
+    HIP_TRY(grow(ws->flags, ws->flags_n, (nmp + 3) / 4 * 4));
     size_t cap;
     if (kind == Q_MUTEX) {
         cap = next_pow2(nmp + 16384);
@@ -491,8 +521,12 @@ int queue_launches(Run& r) {
     HIP_TRY(launch_fill_u32(ws->ctl, 0u, (size_t)C_NSLOTS * kCtlStride, r.s));
     HIP_TRY(launch_fill_u32(ws->st, 0u, (size_t)r.total_mp, r.s));
     if (kind == Q_MUTEX) HIP_TRY(launch_ring_init(ws->ring, r.cap, r.s));
-    HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
-                                  ws->flags, nullptr, r.s));
+    if (!r.flags_ready)  // (after a restart the marker has changed: scan)
+        HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
+                                      ws->flags, nullptr, r.s));
+    else  // the last column pass set the flags inside its blocks: the edges
+        HIP_TRY(launch_seed_edges(r.h[0], r.dtype, r.conn, r.wide, ws->flags, r.s));
+    r.flags_ready = false;
     HIP_TRY(launch_seed_queue(ws->flags, r.total_mp, kind, r.q, r.s));
     HIP_TRY(launch_prepare_queue(r.q, r.s));
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[2], r.s));
@@ -906,15 +940,12 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
         HIP_TRY(hipEventRecord(r.ws->ev[0], r.s));
     }
     if (planning) {
-        rc = sweep_phase(r, 0, 1);  // launched behind the network, before its answer
+        r.plan_end = &plan;  // completed inside the first round (sweep_phase)
+        rc = sweep_phase(r, 0, 1);
         if (rc) return rc;
-        const auto t0 = std::chrono::steady_clock::now();
-        rc = n == 1 ? learned_plan_end_single(r, plan) : learned_plan_end_batch(r, plan);
-        if (rc) return rc;
-        r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
-                           std::chrono::steady_clock::now() - t0).count();
         rc = sweep_phase(r, 1, plan.sweep_rounds);
     } else {
+        r.seed_round = plan.sweep_rounds - 1;
         rc = sweep_phase(r, 0, plan.sweep_rounds);
     }
     if (rc) return rc;

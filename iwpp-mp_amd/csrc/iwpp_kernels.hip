@@ -789,10 +789,20 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
 // reading src and writing dst (out of place).  Inside the block each row
 // uses the block's previous output row; neighbours outside the block come
 // from src, so the pass is deterministic.
-template <class T, int VB, int CONN, int DIR, bool INPLACE>
+//
+// SEED (the last pass of the sweep phase, in place, no tile schedule): the
+// pass also sets flags[mp_base + my * nmx + mx] = 1 for every megapixel that
+// holds a pixel p with a neighbour q such that min(J(q), I(p)) > J(p) at the
+// end of the pass (the seeding test of k_unstable_scan), for the pairs with
+// p and q both in this block: a row is tested when the rows on both sides
+// are final (one row behind the pass).  Pairs across a strip edge or a block
+// boundary are k_seed_edges' (after the pass, on final values).  The flags
+// must be zero before the pass.
+template <class T, int VB, int CONN, int DIR, bool INPLACE, bool SEED = false>
 DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, size_t ps,
                    size_t pd, size_t pi, int W, int H, int R, int strip, int seg,
-                   const TileMask& tm, uint32_t* changed) {
+                   const TileMask& tm, uint32_t* changed, uint8_t* flags = nullptr,
+                   int mp_base = 0, int nmx = 0) {
     using V = VecIO<T, VB>;
     using L = LanePx<T, VB>;
     constexpr int NW = V::NW, NP = L::NP;
@@ -825,6 +835,60 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
         L::split(w, prev);
     }
     bool any_change = false;
+    // ---- seeding state (SEED): r1 = the last row computed (final), c1 its
+    // mask, d1 / d2 = what r1 and the row before it offer a vertical
+    // neighbour inside the strip (8-connectivity: the pixel and its left and
+    // right neighbours; 4: the pixel).
+    const int mx_lane = (o / (int)sizeof(T)) >> 6;
+    const bool lane_in = o < rowbytes;
+    uint32_t r1[NP], c1[NP], d1[NP], d2[NP];
+    uint32_t acc = 0;  // this lane's pixels of band `band`: a flag
+    int band = -1;
+    // neighbour maxima inside the strip (pixels outside count as 0); with
+    // `self` the pixel itself too
+    auto nbmax = [&](const uint32_t (&Rw)[NP], uint32_t (&Dst)[NP], bool self) {
+        uint32_t left, right;
+        if constexpr (L::PAIRS) {
+            left = from_left(Rw[NP - 1] >> 16, 0u) | (Rw[NP - 1] << 16);
+            right = (Rw[0] >> 16) | (from_right(Rw[0] & 0xFFFFu, 0u) << 16);
+        } else {
+            left = from_left(Rw[NP - 1], 0u);
+            right = from_right(Rw[0], 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const uint32_t a = k > 0 ? Rw[k - 1] : left, b = k < NP - 1 ? Rw[k + 1] : right;
+            Dst[k] = self ? L::vmax(Rw[k], L::vmax(a, b)) : L::vmax(a, b);
+        }
+    };
+    auto vert = [&](const uint32_t (&Rw)[NP], uint32_t (&Dst)[NP]) {
+        if constexpr (CONN == 8) nbmax(Rw, Dst, true);
+        else {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) Dst[k] = Rw[k];
+        }
+    };
+    auto flag_band = [&](int y, bool u) {  // accumulated per 64-row band
+        const int b = y >> 6;
+        if (b != band) {
+            if (acc && lane_in) flags[mp_base + band * nmx + mx_lane] = 1;
+            band = b;
+            acc = 0;
+        }
+        acc |= u ? 1u : 0u;
+    };
+    // test of row yc (r1), its vertical neighbours offering da / db
+    auto test_row = [&](int yc, const uint32_t (&da)[NP], const uint32_t (&db)[NP]) {
+        uint32_t h[NP];
+        nbmax(r1, h, false);
+        bool u = false;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const uint32_t q = L::vmax(h[k], L::vmax(da[k], db[k]));
+            u |= L::vmax(L::vmin(q, c1[k]), r1[k]) != r1[k];
+        }
+        flag_band(yc, u);
+    };
     // Rows go in groups of CU: the group's loads (source, mask and, for 8-
     // connectivity, the strip's outside neighbours in the previous source row)
     // are issued together before any of its rows is computed, so a wave keeps
@@ -897,6 +961,33 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
                 L::join(prev, ow);
                 V::store(D + (size_t)y * pd, o, rowbytes, ow);
             }
+            if constexpr (SEED) {
+                uint32_t dn[NP];
+                vert(prev, dn);
+                if (i == 0) {  // no row before r1 inside the block
+#pragma unroll
+                    for (int k = 0; k < NP; ++k) d1[k] = 0;
+                } else {
+                    test_row(y - DIR, d2, dn);  // r1 = row y - DIR: both neighbours final
+                }
+#pragma unroll
+                for (int k = 0; k < NP; ++k) {  // shift: r1 <- this row
+                    d2[k] = d1[k];
+                    d1[k] = dn[k];
+                    r1[k] = prev[k];
+                    c1[k] = c[k];
+                }
+            }
+        }
+    }
+    if constexpr (SEED) {
+        if (n > 0) {
+            const int ylast = DIR > 0 ? y1 - 1 : y0;
+            uint32_t zero[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) zero[k] = 0;
+            test_row(ylast, d2, zero);  // the last row: nothing after it inside the block
+            flag_band(band >= 0 ? -64 : 0, false);  // flush
         }
     }
     if (any_change && changed) *changed = 1u;
@@ -911,7 +1002,7 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
 // reconstruction is unchanged (only the order of propagation is), and every
 // unchanged vector saves its write.
 template <class T, int VB, int CONN, int DIR, int MODE>
-__global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
+__global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, uint8_t* flags) {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ds.total) return;
     const ImgDesc d = desc(ds, find_cw(ds, w));
@@ -922,8 +1013,11 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed) {
         col_block<T, VB, CONN, DIR, false>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
     else if constexpr (MODE == 1)
         col_block<T, VB, CONN, DIR, false>(d.P, d.J, d.I, d.pp, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
-    else
+    else if constexpr (MODE == 2)
         col_block<T, VB, CONN, DIR, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+    else  // in place, with the seeding flags
+        col_block<T, VB, CONN, DIR, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm,
+                                               changed, flags, d.mp_base, d.nmx);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
@@ -1507,6 +1601,62 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
 }
 
 // -------------------------------------------------------------- utilities ---
+// The seeding test for the pixel pairs the seeding column pass leaves out:
+// those across an edge between two strips (spx pixels wide) or a boundary
+// between two blocks (R rows), on final values (after the pass).  A wave
+// takes 64 rows of one strip edge (lane = row) or 64 columns of one block
+// boundary (lane = column) and tests both sides.
+template <class T, int CONN>
+__global__ __launch_bounds__(kWG) void k_seed_edges(Image im, int spx, int R, int nedges,
+                                                     int nbounds, uint8_t* flags, int mp_base,
+                                                     int nmx) {
+    const int w = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)), lane = lane_id();
+    const int nbands = (im.H + 63) / 64, nchunks = (im.W + 63) / 64;
+    auto J = [&](int y, int x) -> uint32_t {
+        return *gp(reinterpret_cast<const T*>(im.J + (size_t)y * im.pj + (size_t)x * sizeof(T)));
+    };
+    auto I = [&](int y, int x) -> uint32_t {
+        return *gp(reinterpret_cast<const T*>(im.I + (size_t)y * im.pi + (size_t)x * sizeof(T)));
+    };
+    auto flag = [&](int y, int x) { flags[mp_base + (y >> 6) * nmx + (x >> 6)] = 1; };
+    if (w < nedges * nbands) {  // strip edge between columns xe - 1 and xe
+        const int xe = (w / nbands + 1) * spx, yb0 = (w % nbands) * 64, y = yb0 + lane;
+        if (xe >= im.W) return;  // the whole wave
+        // each lane loads its row; the rows above and below come from the
+        // neighbouring lanes (DPP), at the band's ends from one more load
+        const bool in = y < im.H;
+        const int yc = in ? y : im.H - 1;
+        const uint32_t jl = in ? J(yc, xe - 1) : 0u, jr = in ? J(yc, xe) : 0u;
+        uint32_t qL = jl, qR = jr;
+        if constexpr (CONN == 8) {
+            uint32_t al = 0, ar = 0, bl = 0, br = 0;  // the rows just above / below the band
+            if (lane == 0 && yb0 > 0) { al = J(yb0 - 1, xe - 1); ar = J(yb0 - 1, xe); }
+            if (lane == 63 && yb0 + 64 < im.H) { bl = J(yb0 + 64, xe - 1); br = J(yb0 + 64, xe); }
+            qL = max(qL, max(from_left(jl, al), from_right(jl, bl)));
+            qR = max(qR, max(from_left(jr, ar), from_right(jr, br)));
+        }
+        if (!in) return;
+        if (min(qL, I(y, xe)) > jr) flag(y, xe);
+        if (min(qR, I(y, xe - 1)) > jl) flag(y, xe - 1);
+        return;
+    }
+    const int wb = w - nedges * nbands;
+    if (wb >= nbounds * nchunks) return;
+    const int yb = (wb / nchunks + 1) * R, x = (wb % nchunks) * 64 + lane;  // rows yb - 1 | yb
+    if (yb >= im.H || x >= im.W) return;
+    uint32_t qT = 0, qB = 0;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx) {
+        if (CONN == 4 && dx != 0) continue;
+        const int xx = x + dx;
+        if (xx < 0 || xx >= im.W) continue;
+        qT = max(qT, J(yb - 1, xx));
+        qB = max(qB, J(yb, xx));
+    }
+    if (min(qT, I(yb, x)) > J(yb, x)) flag(yb, x);
+    if (min(qB, I(yb - 1, x)) > J(yb - 1, x)) flag(yb - 1, x);
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -2143,34 +2293,55 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
 }
 
 template <class T, int VB, int CONN, int DIR>
-static void col_pass_dir(const Descs& ds, int mode, uint32_t* changed, hipStream_t s) {
+static void col_pass_dir(const Descs& ds, int mode, uint32_t* changed, uint8_t* flags,
+                         hipStream_t s) {
     const int grid = (ds.total + 3) / 4;
-    if (mode == 0) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 0>), grid, kWG, 0, s, ds, changed);
-    else if (mode == 1) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 1>), grid, kWG, 0, s, ds, changed);
-    else hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 2>), grid, kWG, 0, s, ds, changed);
+    if (mode == 0) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 0>), grid, kWG, 0, s, ds, changed, flags);
+    else if (mode == 1) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 1>), grid, kWG, 0, s, ds, changed, flags);
+    else if (mode == 2) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 2>), grid, kWG, 0, s, ds, changed, flags);
+    else hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 3>), grid, kWG, 0, s, ds, changed, flags);
 }
 
 template <class T, int VB>
 static void col_pass_t(const Descs& ds, int conn, int dir, int mode, uint32_t* changed,
-                       hipStream_t s) {
+                       uint8_t* flags, hipStream_t s) {
     if (conn == 8) {
-        if (dir > 0) col_pass_dir<T, VB, 8, 1>(ds, mode, changed, s);
-        else col_pass_dir<T, VB, 8, -1>(ds, mode, changed, s);
+        if (dir > 0) col_pass_dir<T, VB, 8, 1>(ds, mode, changed, flags, s);
+        else col_pass_dir<T, VB, 8, -1>(ds, mode, changed, flags, s);
     } else {
-        if (dir > 0) col_pass_dir<T, VB, 4, 1>(ds, mode, changed, s);
-        else col_pass_dir<T, VB, 4, -1>(ds, mode, changed, s);
+        if (dir > 0) col_pass_dir<T, VB, 4, 1>(ds, mode, changed, flags, s);
+        else col_pass_dir<T, VB, 4, -1>(ds, mode, changed, flags, s);
     }
 }
 
 hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, int mode,
-                                 bool wide, uint32_t* changed, hipStream_t s) {
+                                 bool wide, uint32_t* changed, hipStream_t s, uint8_t* flags) {
     if (ds.total <= 0) return hipSuccess;
+    if (mode == COL_IN_PLACE_SEED && !flags) return hipErrorInvalidValue;
     if (dtype == 1) {
-        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, mode, changed, s);
-        else col_pass_t<uint8_t, 4>(ds, conn, dir, mode, changed, s);
+        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, mode, changed, flags, s);
+        else col_pass_t<uint8_t, 4>(ds, conn, dir, mode, changed, flags, s);
     } else {
-        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, mode, changed, s);
-        else col_pass_t<uint16_t, 4>(ds, conn, dir, mode, changed, s);
+        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, mode, changed, flags, s);
+        else col_pass_t<uint16_t, 4>(ds, conn, dir, mode, changed, flags, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide, uint8_t* flags,
+                             hipStream_t s) {
+    const int spx = 64 * (wide ? 16 : 4) / dtype;
+    const int nedges = (d.W + spx - 1) / spx - 1, nbounds = (d.H + d.R - 1) / d.R - 1;
+    const long waves = (long)nedges * ((d.H + 63) / 64) + (long)nbounds * ((d.W + 63) / 64);
+    if (waves <= 0) return hipSuccess;
+    const Image im{d.J, d.I, d.pj, d.pi, d.W, d.H};
+    const int grid = (int)((waves + 3) / 4);
+    if (dtype == 1) {
+        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint8_t, 8>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        else hipLaunchKernelGGL((k_seed_edges<uint8_t, 4>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+    } else {
+        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint16_t, 8>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        else hipLaunchKernelGGL((k_seed_edges<uint16_t, 4>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
     }
     return hipGetLastError();
 }
