@@ -811,7 +811,9 @@ int learned_plan_end_batch(Run& r, SchedulePlan& plan) {
     int rmax = 0;
     long long skipped = 0;
     for (int i = 0; i < n; ++i) {
-        r.h[i].rounds = std::min(cap, image_rounds(r.h[i], pred, L.base[i], L.base[i + 1], L.tile));
+        // at least the round already under way (its upward pass is still to come)
+        r.h[i].rounds = std::max(1, std::min(cap, image_rounds(r.h[i], pred, L.base[i],
+                                                               L.base[i + 1], L.tile)));
         rmax = std::max(rmax, r.h[i].rounds);
     }
     for (int i = 0; i < n; ++i) skipped += (long long)(rmax - r.h[i].rounds) * (L.base[i + 1] - L.base[i]);
