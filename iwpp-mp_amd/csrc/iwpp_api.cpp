@@ -77,6 +77,8 @@ struct iwpp_workspace {
     size_t st_n = 0;
     uint8_t* flags = nullptr;   // per-megapixel unstable flags
     size_t flags_n = 0;
+    uint32_t* edges = nullptr;  // strip border columns of the seeding column pass
+    size_t edges_n = 0;
     uint32_t* qa = nullptr;
     size_t qa_n = 0;
     uint32_t* qb = nullptr;
@@ -146,6 +148,7 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     if (!ws) return;
     (void)hipFree(ws->st);
     (void)hipFree(ws->flags);
+    (void)hipFree(ws->edges);
     (void)hipFree(ws->qa);
     (void)hipFree(ws->qb);
     (void)hipFree(ws->ring);
@@ -458,10 +461,12 @@ int sweep_phase(Run& r, int k0, int k1) {
             const size_t words = ((size_t)r.total_mp + 3) / 4;
             HIP_TRY(grow(r.ws->flags, r.ws->flags_n, words * 4));
             HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(r.ws->flags), 0u, words, r.s));
+            HIP_TRY(grow(r.ws->edges, r.ws->edges_n, (size_t)r.h[0].strips * 2 * r.h[0].H));
         }
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
                                       seed ? COL_IN_PLACE_SEED : COL_IN_PLACE, r.wide, nullptr,
-                                      r.s, seed ? r.ws->flags : nullptr));
+                                      r.s, seed ? r.ws->flags : nullptr,
+                                      seed ? r.ws->edges : nullptr));
         r.flags_ready = seed;
     }
     r.st.sweep_rounds += std::max(0, k1 - k0);
@@ -525,7 +530,7 @@ int queue_launches(Run& r) {
         HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
                                       ws->flags, nullptr, r.s));
     else  // the last column pass set the flags inside its blocks: the edges
-        HIP_TRY(launch_seed_edges(r.h[0], r.dtype, r.conn, r.wide, ws->flags, r.s));
+        HIP_TRY(launch_seed_edges(r.h[0], r.dtype, r.conn, r.wide, ws->edges, ws->flags, r.s));
     r.flags_ready = false;
     HIP_TRY(launch_seed_queue(ws->flags, r.total_mp, kind, r.q, r.s));
     HIP_TRY(launch_prepare_queue(r.q, r.s));

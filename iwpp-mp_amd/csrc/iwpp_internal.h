@@ -132,15 +132,18 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
 // else 4-byte lanes.
 // COL_IN_PLACE_SEED: in place, and sets the megapixel seeding flags (zeroed
 // before) as k_unstable_scan would after the pass; one image without a tile
-// schedule, the last pass of the sweep phase.
+// schedule, the last pass of the sweep phase.  It also writes the strips'
+// border columns as they leave the pass: edges[(strip * 2 + side) * H + y] =
+// marker | mask << 16 of the strip's first (side 0) or last (side 1) pixel of
+// row y (strips * 2 * H words).
 enum ColMode : int { COL_J_TO_P = 0, COL_P_TO_J = 1, COL_IN_PLACE = 2, COL_IN_PLACE_SEED = 3 };
-// The seeding test across the strip edges and block boundaries of the
-// column pass (one image; with COL_IN_PLACE_SEED it completes the flags).
-hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide, uint8_t* flags,
-                             hipStream_t s);
+// The seeding test across the strip edges (from `edges`) and block
+// boundaries of a COL_IN_PLACE_SEED pass (one image), completing its flags.
+hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide,
+                             const uint32_t* edges, uint8_t* flags, hipStream_t s);
 hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, int mode,
                                  bool wide, uint32_t* changed, hipStream_t s,
-                                 uint8_t* flags = nullptr);
+                                 uint8_t* flags = nullptr, uint32_t* edges = nullptr);
 bool col_wide(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,
               size_t pi, int W, int dtype);
 int col_block_width(const void* src, const void* dst, const void* mask, size_t ps, size_t pd,

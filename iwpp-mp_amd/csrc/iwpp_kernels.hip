@@ -797,12 +797,14 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
 // p and q both in this block: a row is tested when the rows on both sides
 // are final (one row behind the pass).  Pairs across a strip edge or a block
 // boundary are k_seed_edges' (after the pass, on final values).  The flags
-// must be zero before the pass.
+// must be zero before the pass.  For k_seed_edges the pass writes the strip's
+// border columns as it leaves them: edges[(strip * 2 + side) * H + y] =
+// marker | mask << 16 of the strip's first (side 0) or last (side 1) pixel.
 template <class T, int VB, int CONN, int DIR, bool INPLACE, bool SEED = false>
 DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, size_t ps,
                    size_t pd, size_t pi, int W, int H, int R, int strip, int seg,
                    const TileMask& tm, uint32_t* changed, uint8_t* flags = nullptr,
-                   int mp_base = 0, int nmx = 0) {
+                   int mp_base = 0, int nmx = 0, uint32_t* edges = nullptr) {
     using V = VecIO<T, VB>;
     using L = LanePx<T, VB>;
     constexpr int NW = V::NW, NP = L::NP;
@@ -876,6 +878,17 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             acc = 0;
         }
         acc |= u ? 1u : 0u;
+    };
+    // the strip's border columns of one 64-row band (lane = row), stored
+    // for the rows of this block when the pass leaves the band
+    uint32_t eL = 0, eR = 0;
+    int eband = -1;
+    auto edge_flush = [&]() {
+        const int yr = eband * 64 + lane;
+        if (eband >= 0 && yr >= y0 && yr < y1) {
+            *gp(edges + (size_t)strip * 2 * H + yr) = eL;
+            *gp(edges + ((size_t)strip * 2 + 1) * H + yr) = eR;
+        }
     };
     // test of row yc (r1), its vertical neighbours offering da / db
     auto test_row = [&](int yc, const uint32_t (&da)[NP], const uint32_t (&db)[NP]) {
@@ -962,6 +975,17 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
                 V::store(D + (size_t)y * pd, o, rowbytes, ow);
             }
             if constexpr (SEED) {
+                {  // the strip's border pixels (final) into lane y % 64
+                    const uint32_t f = rdlane(prev[0], 0), cf = rdlane(c[0], 0);
+                    const uint32_t l = rdlane(prev[NP - 1], 63), cl = rdlane(c[NP - 1], 63);
+                    if ((y >> 6) != eband) {
+                        edge_flush();
+                        eband = y >> 6;
+                    }
+                    const bool here = lane == (y & 63);
+                    eL = here ? (f & 0xFFFFu) | (cf << 16) : eL;
+                    eR = here ? (L::PAIRS ? (l >> 16) | (cl & 0xFFFF0000u) : l | (cl << 16)) : eR;
+                }
                 uint32_t dn[NP];
                 vert(prev, dn);
                 if (i == 0) {  // no row before r1 inside the block
@@ -988,6 +1012,7 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             for (int k = 0; k < NP; ++k) zero[k] = 0;
             test_row(ylast, d2, zero);  // the last row: nothing after it inside the block
             flag_band(band >= 0 ? -64 : 0, false);  // flush
+            edge_flush();
         }
     }
     if (any_change && changed) *changed = 1u;
@@ -1002,7 +1027,8 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
 // reconstruction is unchanged (only the order of propagation is), and every
 // unchanged vector saves its write.
 template <class T, int VB, int CONN, int DIR, int MODE>
-__global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, uint8_t* flags) {
+__global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, uint8_t* flags,
+                                                  uint32_t* edges) {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= ds.total) return;
     const ImgDesc d = desc(ds, find_cw(ds, w));
@@ -1017,7 +1043,7 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, u
         col_block<T, VB, CONN, DIR, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
     else  // in place, with the seeding flags
         col_block<T, VB, CONN, DIR, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm,
-                                               changed, flags, d.mp_base, d.nmx);
+                                               changed, flags, d.mp_base, d.nmx, edges);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
@@ -1605,11 +1631,13 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
 // those across an edge between two strips (spx pixels wide) or a boundary
 // between two blocks (R rows), on final values (after the pass).  A wave
 // takes 64 rows of one strip edge (lane = row) or 64 columns of one block
-// boundary (lane = column) and tests both sides.
+// boundary (lane = column) and tests both sides.  The strip edges read the
+// border columns the pass wrote to `edges` (one word per row and side,
+// marker | mask << 16): in the image a pixel per row would cost a cache line.
 template <class T, int CONN>
-__global__ __launch_bounds__(kWG) void k_seed_edges(Image im, int spx, int R, int nedges,
-                                                     int nbounds, uint8_t* flags, int mp_base,
-                                                     int nmx) {
+__global__ __launch_bounds__(kWG) void k_seed_edges(Image im, const uint32_t* edges, int spx,
+                                                     int R, int nedges, int nbounds,
+                                                     uint8_t* flags, int mp_base, int nmx) {
     const int w = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)), lane = lane_id();
     const int nbands = (im.H + 63) / 64, nchunks = (im.W + 63) / 64;
     auto J = [&](int y, int x) -> uint32_t {
@@ -1620,24 +1648,27 @@ __global__ __launch_bounds__(kWG) void k_seed_edges(Image im, int spx, int R, in
     };
     auto flag = [&](int y, int x) { flags[mp_base + (y >> 6) * nmx + (x >> 6)] = 1; };
     if (w < nedges * nbands) {  // strip edge between columns xe - 1 and xe
-        const int xe = (w / nbands + 1) * spx, yb0 = (w % nbands) * 64, y = yb0 + lane;
+        const int se = w / nbands, xe = (se + 1) * spx, yb0 = (w % nbands) * 64, y = yb0 + lane;
         if (xe >= im.W) return;  // the whole wave
+        const uint32_t* EL = edges + ((size_t)se * 2 + 1) * im.H;  // column xe - 1
+        const uint32_t* ER = edges + ((size_t)se * 2 + 2) * im.H;  // column xe
         // each lane loads its row; the rows above and below come from the
         // neighbouring lanes (DPP), at the band's ends from one more load
         const bool in = y < im.H;
         const int yc = in ? y : im.H - 1;
-        const uint32_t jl = in ? J(yc, xe - 1) : 0u, jr = in ? J(yc, xe) : 0u;
+        const uint32_t el = in ? *gp(EL + yc) : 0u, er = in ? *gp(ER + yc) : 0u;
+        const uint32_t jl = el & 0xFFFFu, jr = er & 0xFFFFu;
         uint32_t qL = jl, qR = jr;
         if constexpr (CONN == 8) {
             uint32_t al = 0, ar = 0, bl = 0, br = 0;  // the rows just above / below the band
-            if (lane == 0 && yb0 > 0) { al = J(yb0 - 1, xe - 1); ar = J(yb0 - 1, xe); }
-            if (lane == 63 && yb0 + 64 < im.H) { bl = J(yb0 + 64, xe - 1); br = J(yb0 + 64, xe); }
+            if (lane == 0 && yb0 > 0) { al = *gp(EL + yb0 - 1) & 0xFFFFu; ar = *gp(ER + yb0 - 1) & 0xFFFFu; }
+            if (lane == 63 && yb0 + 64 < im.H) { bl = *gp(EL + yb0 + 64) & 0xFFFFu; br = *gp(ER + yb0 + 64) & 0xFFFFu; }
             qL = max(qL, max(from_left(jl, al), from_right(jl, bl)));
             qR = max(qR, max(from_left(jr, ar), from_right(jr, br)));
         }
         if (!in) return;
-        if (min(qL, I(y, xe)) > jr) flag(y, xe);
-        if (min(qR, I(y, xe - 1)) > jl) flag(y, xe - 1);
+        if (min(qL, er >> 16) > jr) flag(y, xe);
+        if (min(qR, el >> 16) > jl) flag(y, xe - 1);
         return;
     }
     const int wb = w - nedges * nbands;
@@ -2294,42 +2325,43 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
 
 template <class T, int VB, int CONN, int DIR>
 static void col_pass_dir(const Descs& ds, int mode, uint32_t* changed, uint8_t* flags,
-                         hipStream_t s) {
+                         uint32_t* edges, hipStream_t s) {
     const int grid = (ds.total + 3) / 4;
-    if (mode == 0) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 0>), grid, kWG, 0, s, ds, changed, flags);
-    else if (mode == 1) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 1>), grid, kWG, 0, s, ds, changed, flags);
-    else if (mode == 2) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 2>), grid, kWG, 0, s, ds, changed, flags);
-    else hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 3>), grid, kWG, 0, s, ds, changed, flags);
+    if (mode == 0) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 0>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else if (mode == 1) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 1>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else if (mode == 2) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 2>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 3>), grid, kWG, 0, s, ds, changed, flags, edges);
 }
 
 template <class T, int VB>
 static void col_pass_t(const Descs& ds, int conn, int dir, int mode, uint32_t* changed,
-                       uint8_t* flags, hipStream_t s) {
+                       uint8_t* flags, uint32_t* edges, hipStream_t s) {
     if (conn == 8) {
-        if (dir > 0) col_pass_dir<T, VB, 8, 1>(ds, mode, changed, flags, s);
-        else col_pass_dir<T, VB, 8, -1>(ds, mode, changed, flags, s);
+        if (dir > 0) col_pass_dir<T, VB, 8, 1>(ds, mode, changed, flags, edges, s);
+        else col_pass_dir<T, VB, 8, -1>(ds, mode, changed, flags, edges, s);
     } else {
-        if (dir > 0) col_pass_dir<T, VB, 4, 1>(ds, mode, changed, flags, s);
-        else col_pass_dir<T, VB, 4, -1>(ds, mode, changed, flags, s);
+        if (dir > 0) col_pass_dir<T, VB, 4, 1>(ds, mode, changed, flags, edges, s);
+        else col_pass_dir<T, VB, 4, -1>(ds, mode, changed, flags, edges, s);
     }
 }
 
 hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, int mode,
-                                 bool wide, uint32_t* changed, hipStream_t s, uint8_t* flags) {
+                                 bool wide, uint32_t* changed, hipStream_t s, uint8_t* flags,
+                                 uint32_t* edges) {
     if (ds.total <= 0) return hipSuccess;
-    if (mode == COL_IN_PLACE_SEED && !flags) return hipErrorInvalidValue;
+    if (mode == COL_IN_PLACE_SEED && (!flags || !edges)) return hipErrorInvalidValue;
     if (dtype == 1) {
-        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, mode, changed, flags, s);
-        else col_pass_t<uint8_t, 4>(ds, conn, dir, mode, changed, flags, s);
+        if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, mode, changed, flags, edges, s);
+        else col_pass_t<uint8_t, 4>(ds, conn, dir, mode, changed, flags, edges, s);
     } else {
-        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, mode, changed, flags, s);
-        else col_pass_t<uint16_t, 4>(ds, conn, dir, mode, changed, flags, s);
+        if (wide) col_pass_t<uint16_t, 16>(ds, conn, dir, mode, changed, flags, edges, s);
+        else col_pass_t<uint16_t, 4>(ds, conn, dir, mode, changed, flags, edges, s);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide, uint8_t* flags,
-                             hipStream_t s) {
+hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide,
+                             const uint32_t* edges, uint8_t* flags, hipStream_t s) {
     const int spx = 64 * (wide ? 16 : 4) / dtype;
     const int nedges = (d.W + spx - 1) / spx - 1, nbounds = (d.H + d.R - 1) / d.R - 1;
     const long waves = (long)nedges * ((d.H + 63) / 64) + (long)nbounds * ((d.W + 63) / 64);
@@ -2337,11 +2369,11 @@ hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide, u
     const Image im{d.J, d.I, d.pj, d.pi, d.W, d.H};
     const int grid = (int)((waves + 3) / 4);
     if (dtype == 1) {
-        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint8_t, 8>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
-        else hipLaunchKernelGGL((k_seed_edges<uint8_t, 4>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint8_t, 8>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        else hipLaunchKernelGGL((k_seed_edges<uint8_t, 4>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
     } else {
-        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint16_t, 8>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
-        else hipLaunchKernelGGL((k_seed_edges<uint16_t, 4>), grid, kWG, 0, s, im, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint16_t, 8>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        else hipLaunchKernelGGL((k_seed_edges<uint16_t, 4>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
     }
     return hipGetLastError();
 }
