@@ -794,8 +794,9 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
 // pass also sets flags[mp_base + my * nmx + mx] = 1 for every megapixel that
 // holds a pixel p with a neighbour q such that min(J(q), I(p)) > J(p) at the
 // end of the pass (the seeding test of k_unstable_scan), for the pairs with
-// p and q both in this block: a row is tested when the rows on both sides
-// are final (one row behind the pass).  Pairs across a strip edge or a block
+// p and q both in this block: when a row is computed it is final, and so is
+// the row before it, so the pairs inside the row and between the two rows
+// are tested (both ways) at once.  Pairs across a strip edge or a block
 // boundary are k_seed_edges' (after the pass, on final values).  The flags
 // must be zero before the pass.  For k_seed_edges the pass writes the strip's
 // border columns as it leaves them: edges[(strip * 2 + side) * H + y] =
@@ -837,13 +838,13 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
         L::split(w, prev);
     }
     bool any_change = false;
-    // ---- seeding state (SEED): r1 = the last row computed (final), c1 its
-    // mask, d1 / d2 = what r1 and the row before it offer a vertical
-    // neighbour inside the strip (8-connectivity: the pixel and its left and
-    // right neighbours; 4: the pixel).
+    // ---- seeding state (SEED): r1 = the row computed before (final), c1 its
+    // mask, v1 what r1 offers a vertical neighbour inside the strip
+    // (8-connectivity: the pixel and its left and right neighbours; 4: the
+    // pixel).
     const int mx_lane = (o / (int)sizeof(T)) >> 6;
     const bool lane_in = o < rowbytes;
-    uint32_t r1[NP], c1[NP], d1[NP], d2[NP];
+    uint32_t r1[NP], c1[NP], v1[NP];
     uint32_t acc = 0;  // this lane's pixels of band `band`: a flag
     int band = -1;
     // neighbour maxima inside the strip (pixels outside count as 0); with
@@ -861,13 +862,6 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
         for (int k = 0; k < NP; ++k) {
             const uint32_t a = k > 0 ? Rw[k - 1] : left, b = k < NP - 1 ? Rw[k + 1] : right;
             Dst[k] = self ? L::vmax(Rw[k], L::vmax(a, b)) : L::vmax(a, b);
-        }
-    };
-    auto vert = [&](const uint32_t (&Rw)[NP], uint32_t (&Dst)[NP]) {
-        if constexpr (CONN == 8) nbmax(Rw, Dst, true);
-        else {
-#pragma unroll
-            for (int k = 0; k < NP; ++k) Dst[k] = Rw[k];
         }
     };
     auto flag_band = [&](int y, bool u) {  // accumulated per 64-row band
@@ -889,18 +883,6 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             *gp(edges + (size_t)strip * 2 * H + yr) = eL;
             *gp(edges + ((size_t)strip * 2 + 1) * H + yr) = eR;
         }
-    };
-    // test of row yc (r1), its vertical neighbours offering da / db
-    auto test_row = [&](int yc, const uint32_t (&da)[NP], const uint32_t (&db)[NP]) {
-        uint32_t h[NP];
-        nbmax(r1, h, false);
-        bool u = false;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            const uint32_t q = L::vmax(h[k], L::vmax(da[k], db[k]));
-            u |= L::vmax(L::vmin(q, c1[k]), r1[k]) != r1[k];
-        }
-        flag_band(yc, u);
     };
     // Rows go in groups of CU: the group's loads (source, mask and, for 8-
     // connectivity, the strip's outside neighbours in the previous source row)
@@ -986,31 +968,28 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
                     eL = here ? (f & 0xFFFFu) | (cf << 16) : eL;
                     eR = here ? (L::PAIRS ? (l >> 16) | (cl & 0xFFFF0000u) : l | (cl << 16)) : eR;
                 }
-                uint32_t dn[NP];
-                vert(prev, dn);
-                if (i == 0) {  // no row before r1 inside the block
+                // row y against its own neighbours and row r1 = y - DIR, and
+                // r1 against row y (the row before y in the block: i > 0)
+                uint32_t h[NP];
+                nbmax(prev, h, false);
+                bool uy = false, ur = false;
 #pragma unroll
-                    for (int k = 0; k < NP; ++k) d1[k] = 0;
-                } else {
-                    test_row(y - DIR, d2, dn);  // r1 = row y - DIR: both neighbours final
-                }
-#pragma unroll
-                for (int k = 0; k < NP; ++k) {  // shift: r1 <- this row
-                    d2[k] = d1[k];
-                    d1[k] = dn[k];
+                for (int k = 0; k < NP; ++k) {
+                    const uint32_t dn = CONN == 8 ? L::vmax(h[k], prev[k]) : prev[k];
+                    const uint32_t q = i > 0 ? L::vmax(h[k], v1[k]) : h[k];
+                    uy |= L::vmax(L::vmin(q, c[k]), prev[k]) != prev[k];
+                    if (i > 0) ur |= L::vmax(L::vmin(dn, c1[k]), r1[k]) != r1[k];
+                    v1[k] = dn;
                     r1[k] = prev[k];
                     c1[k] = c[k];
                 }
+                if (i > 0) flag_band(y - DIR, ur);
+                flag_band(y, uy);
             }
         }
     }
     if constexpr (SEED) {
         if (n > 0) {
-            const int ylast = DIR > 0 ? y1 - 1 : y0;
-            uint32_t zero[NP];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) zero[k] = 0;
-            test_row(ylast, d2, zero);  // the last row: nothing after it inside the block
             flag_band(band >= 0 ? -64 : 0, false);  // flush
             edge_flush();
         }
@@ -1699,49 +1678,64 @@ __global__ void k_ring_init(unsigned long long* ring, size_t n) {
         ring[i] = (unsigned long long)i << 32;  // slot i expects push ticket i
 }
 
-// Compacts the unstable flags into queue A (ballot + popcount inside a wave,
-// one atomicAdd per wave) and marks each queued megapixel in st:
+// Compacts the unstable flags into queue A and marks each queued megapixel
+// in st:
 //   mutex: st = PENDING, ring slot sequence = index + 1, pending += count
 //   UQ:    st = 1 (already in the next read queue)
+// A wave takes K consecutive 64-flag chunks (ballot + popcount per chunk) and
+// reserves their queue entries with one atomicAdd on the tail: atomics on one
+// address serialise, one per chunk cost 40 us at 262144 megapixels.  Entries
+// keep the chunks' order inside a wave.
+template <int K>
 __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nmp, int kind,
                                                      QueueArgs q) {
     const int lane = lane_id();
-    for (int base = (blockIdx.x * kWG + threadIdx.x) & ~63; base < nmp;
-         base += gridDim.x * kWG) {
-        const int pos_in = base + lane;
-        int m = pos_in < nmp ? pos_in : nmp;
-        if (q.order_tiles && pos_in < nmp) {  // tile-ordered seeding
-            int lo = 0, hi = q.order_nt - 1;  // last rank with start <= pos
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((int)q.order_start[mid] <= pos_in) lo = mid;
-                else hi = mid - 1;
+    for (int base = ((blockIdx.x * kWG + (int)threadIdx.x) >> 6) * 64 * K; base < nmp;
+         base += gridDim.x * kWG * K) {
+        int m[K];
+        uint64_t b[K];
+        uint32_t total = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int pos_in = base + k * 64 + lane;
+            m[k] = pos_in < nmp ? pos_in : nmp;
+            if (q.order_tiles && pos_in < nmp) {  // tile-ordered seeding
+                int lo = 0, hi = q.order_nt - 1;  // last rank with start <= pos
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((int)q.order_start[mid] <= pos_in) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int t = (int)q.order_tiles[lo];
+                const int local = pos_in - (int)q.order_start[lo];
+                const int tx = t % q.order_ntx, ty = t / q.order_ntx;
+                const int wmp = min(q.order_mpt, q.order_nmx - tx * q.order_mpt);
+                m[k] = (ty * q.order_mpt + local / wmp) * q.order_nmx + tx * q.order_mpt +
+                       local % wmp;
             }
-            const int t = (int)q.order_tiles[lo];
-            const int local = pos_in - (int)q.order_start[lo];
-            const int tx = t % q.order_ntx, ty = t / q.order_ntx;
-            const int wmp = min(q.order_mpt, q.order_nmx - tx * q.order_mpt);
-            m = (ty * q.order_mpt + local / wmp) * q.order_nmx + tx * q.order_mpt + local % wmp;
+            b[k] = ballot(m[k] < nmp && flags[m[k]] != 0);
+            total += (uint32_t)__popcll(b[k]);
         }
-        const bool f = m < nmp && flags[m] != 0;
-        const uint64_t b = ballot(f);
-        const uint32_t n = (uint32_t)__popcll(b);
-        if (n == 0) continue;
+        if (total == 0) continue;
         uint32_t start = 0;
-        if (lane == 0) start = a_add(ctl_at(q.ctl, C_TAIL), n);
+        if (lane == 0) start = a_add(ctl_at(q.ctl, C_TAIL), total);
         start = first_lane(start);
-        const uint32_t off = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-        if (f) {
-            const uint32_t pos = start + off;
-            if (pos < q.cap) {
-                if (kind != Q_MUTEX) q.qa[pos] = (uint32_t)m;
-                if (kind == Q_MUTEX) {
-                    q.st[m] = ST_PENDING;
-                    q.ring[pos] = ((unsigned long long)(pos + 1) << 32) | (uint32_t)m;
-                } else if (kind == Q_UQ) {
-                    q.st[m] = 1u;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t off = (uint32_t)__popcll(b[k] & ((1ull << lane) - 1ull));
+            if ((b[k] >> lane) & 1ull) {
+                const uint32_t pos = start + off;
+                if (pos < q.cap) {
+                    if (kind != Q_MUTEX) q.qa[pos] = (uint32_t)m[k];
+                    if (kind == Q_MUTEX) {
+                        q.st[m[k]] = ST_PENDING;
+                        q.ring[pos] = ((unsigned long long)(pos + 1) << 32) | (uint32_t)m[k];
+                    } else if (kind == Q_UQ) {
+                        q.st[m[k]] = 1u;
+                    }
                 }
             }
+            start += (uint32_t)__popcll(b[k]);
         }
     }
 }
@@ -2416,8 +2410,11 @@ hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s) {
 
 hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const QueueArgs& q,
                              hipStream_t s) {
-    const int grid = std::min((nmp + kWG - 1) / kWG, cu_count() * 8);
-    hipLaunchKernelGGL(k_seed_queue, std::max(grid, 1), kWG, 0, s, flags, nmp, kind, q);
+    // small images: a chunk per wave, for more waves
+    const int k = nmp >= 16384 ? 8 : 1;
+    const int grid = std::max(1, std::min((nmp + kWG * k - 1) / (kWG * k), cu_count() * 8));
+    if (k == 8) hipLaunchKernelGGL((k_seed_queue<8>), grid, kWG, 0, s, flags, nmp, kind, q);
+    else hipLaunchKernelGGL((k_seed_queue<1>), grid, kWG, 0, s, flags, nmp, kind, q);
     return hipGetLastError();
 }
 
