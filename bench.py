@@ -185,14 +185,17 @@ def main():
     # the algorithmic bytes one launch must move: 3*sizeof(T) per pixel a
     # sweep pass owns (read marker, read mask, write marker; row and column
     # passes alike), 3*sizeof(T) per pixel of each megapixel visit (64*64
-    # pixels) in the queue phase, 2*sizeof(T) per pixel for the seeding scan.
+    # pixels) in the queue phase, 2*sizeof(T) per pixel for the seeding scan
+    # (an upper bound for the seeding phase: on a first queue attempt the last
+    # column pass sets the flags and k_seed_edges only reads the strip edges).
     queue_name = "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>")
     kernels = {
         "sweep passes (k_row_pass*, k_col_pass)":
             (phases["ms_sweep"], 3 * rounds, bytes_sweep,
              ("k_row_pass", "k_col_pass")),
         queue_name: (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize, ("k_queue_",)),
-        "k_unstable_scan": (phases["ms_seed"], 1, 2 * tsize * px, ("k_unstable_scan",)),
+        "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
+            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_", "k_unstable_scan")),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
     tot_ms, launches, alg_bytes, prefixes = kernels[dom]
