@@ -241,6 +241,8 @@ struct Run {
                              // bytes (every plane 16-byte aligned), else 0
     int single_row_pass = 0; // one image: use the register row kernels
     int total_rows = 0, total_cw = 0, total_sw = 0, total_mp = 0;
+    int total_se = 0;        // seeding edge-test waves
+    size_t total_edges = 0;  // words of strip border columns (seeding pass)
     QueueArgs q{};
     size_t cap = 0;
     // learned schedule (single image): per-round tile masks, priority order
@@ -347,7 +349,8 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         rr = (rr + 16) / 32 * 32;
         R = (int)std::min<long>(512, std::max<long>(32, rr));
     }
-    long rows = 0, cw = 0, sw = 0, mp = 0;
+    long rows = 0, cw = 0, sw = 0, mp = 0, se = 0;
+    long long edges = 0;
     for (int i = 0; i < n; ++i) {
         ImgDesc& d = r.h[i];
         d.R = R;
@@ -355,13 +358,20 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         d.cw_base = (int)cw;
         d.sw_base = (int)sw;
         d.mp_base = (int)mp;
+        d.se_base = (int)se;
+        d.edge_base = edges;
         rows += d.H;
         cw += (long)d.strips * ((d.H + R - 1) / R);
         sw += (long)d.strips * d.nmy;
         mp += (long)d.nmx * d.nmy;
+        se += seed_edge_waves(d);
+        edges += (long long)d.strips * 2 * d.H;
     }
-    if (rows > 0x7FFFFFFF || cw > 0x7FFFFFFF || sw > 0x7FFFFFFF || mp > 0x7FFFFFF0)
+    if (rows > 0x7FFFFFFF || cw > 0x7FFFFFFF || sw > 0x7FFFFFFF || mp > 0x7FFFFFF0 ||
+        se > 0x7FFFFFFF)
         return fail(IWPP_ERR_ARG, "batch too large");
+    r.total_se = (int)se;
+    r.total_edges = (size_t)edges;
     r.total_rows = (int)rows;
     r.total_cw = (int)cw;
     r.total_sw = (int)sw;
@@ -456,12 +466,15 @@ int sweep_phase(Run& r, int k0, int k1) {
                                std::chrono::steady_clock::now() - t0).count();
             r.seed_round = plan.sweep_rounds - 1;
         }
-        const bool seed = k == r.seed_round && r.ds.n == 1 && !r.round_masks && !r.graph;
-        if (seed) {  // the last pass sets the seeding flags (k_unstable_scan is skipped)
+        // the last pass sets the seeding flags (k_unstable_scan is skipped)
+        // when it is the last of every image (no tile schedule)
+        bool seed = k == r.seed_round && !r.round_masks;
+        for (int i = 0; seed && i < r.ds.n; ++i) seed = r.h[i].rounds > k;
+        if (seed) {
             const size_t words = ((size_t)r.total_mp + 3) / 4;
             HIP_TRY(grow(r.ws->flags, r.ws->flags_n, words * 4));
             HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(r.ws->flags), 0u, words, r.s));
-            HIP_TRY(grow(r.ws->edges, r.ws->edges_n, (size_t)r.h[0].strips * 2 * r.h[0].H));
+            HIP_TRY(grow(r.ws->edges, r.ws->edges_n, std::max<size_t>(1, r.total_edges)));
         }
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
                                       seed ? COL_IN_PLACE_SEED : COL_IN_PLACE, r.wide, nullptr,
@@ -479,6 +492,8 @@ int setup_queue(Run& r) {
     const size_t nmp = (size_t)r.total_mp;
     HIP_TRY(grow(ws->st, ws->st_n, nmp));
     HIP_TRY(grow(ws->flags, ws->flags_n, (nmp + 3) / 4 * 4));
+    // (sized here too, so that a graph's dry run allocates it before capture)
+    HIP_TRY(grow(ws->edges, ws->edges_n, std::max<size_t>(1, r.total_edges)));
     size_t cap;
     if (kind == Q_MUTEX) {
         cap = next_pow2(nmp + 16384);
@@ -530,7 +545,8 @@ int queue_launches(Run& r) {
         HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
                                       ws->flags, nullptr, r.s));
     else  // the last column pass set the flags inside its blocks: the edges
-        HIP_TRY(launch_seed_edges(r.h[0], r.dtype, r.conn, r.wide, ws->edges, ws->flags, r.s));
+        HIP_TRY(launch_seed_edges(with_total(r.ds, r.total_se), r.dtype, r.conn, r.wide,
+                                  ws->edges, ws->flags, r.s));
     r.flags_ready = false;
     HIP_TRY(launch_seed_queue(ws->flags, r.total_mp, kind, r.q, r.s));
     HIP_TRY(launch_prepare_queue(r.q, r.s));

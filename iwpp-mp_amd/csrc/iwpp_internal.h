@@ -75,6 +75,9 @@ struct ImgDesc {
     TileMask tm;         // walls for the sweep passes
     int rounds = 0x7FFFFFFF;  // sweep rounds this image takes (a batch under the
                               // learned schedule); later rounds skip it
+    long long edge_base = 0;  // first word of the image's strip border columns
+                              // (COL_IN_PLACE_SEED): strips * 2 * H words
+    int se_base = 0;          // first wave of the seeding edge test
 };
 
 // Images of a launch: a device array, or (d == nullptr) the single image `one`
@@ -131,16 +134,19 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
 // J; dir +1 down, -1 up; wide = 16-byte lanes (every image 16-byte aligned),
 // else 4-byte lanes.
 // COL_IN_PLACE_SEED: in place, and sets the megapixel seeding flags (zeroed
-// before) as k_unstable_scan would after the pass; one image without a tile
-// schedule, the last pass of the sweep phase.  It also writes the strips'
+// before) as k_unstable_scan would after the pass; without tile schedules,
+// the last pass of every image of the launch.  It also writes the strips'
 // border columns as they leave the pass: edges[(strip * 2 + side) * H + y] =
 // marker | mask << 16 of the strip's first (side 0) or last (side 1) pixel of
-// row y (strips * 2 * H words).
+// row y (strips * 2 * H words, from edges + edge_base for each image).
 enum ColMode : int { COL_J_TO_P = 0, COL_P_TO_J = 1, COL_IN_PLACE = 2, COL_IN_PLACE_SEED = 3 };
 // The seeding test across the strip edges (from `edges`) and block
-// boundaries of a COL_IN_PLACE_SEED pass (one image), completing its flags.
-hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide,
+// boundaries of a COL_IN_PLACE_SEED pass, completing its flags (ds.total =
+// all edge-test waves, seed_edge_waves() per image from se_base).
+hipError_t launch_seed_edges(const Descs& ds, int dtype, int conn, bool wide,
                              const uint32_t* edges, uint8_t* flags, hipStream_t s);
+// Edge-test waves of one image (strips and R set).
+long seed_edge_waves(const ImgDesc& d);
 hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, int mode,
                                  bool wide, uint32_t* changed, hipStream_t s,
                                  uint8_t* flags = nullptr, uint32_t* edges = nullptr);

@@ -420,6 +420,7 @@ DEV ImgDesc desc(const Descs& a, int i) {
 IWPP_FIND(search_row, row_base)
 IWPP_FIND(search_cw, cw_base)
 IWPP_FIND(search_sw, sw_base)
+IWPP_FIND(search_se, se_base)
 #undef IWPP_FIND
 // One load from the work-item table when there is one (a dependent chain of
 // loads per work item otherwise: measured as the main cost of batched passes).
@@ -1022,7 +1023,7 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, u
         col_block<T, VB, CONN, DIR, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
     else  // in place, with the seeding flags
         col_block<T, VB, CONN, DIR, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm,
-                                               changed, flags, d.mp_base, d.nmx, edges);
+                                               changed, flags, d.mp_base, d.nmx, edges + d.edge_base);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
@@ -1613,11 +1614,17 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
 // boundary (lane = column) and tests both sides.  The strip edges read the
 // border columns the pass wrote to `edges` (one word per row and side,
 // marker | mask << 16): in the image a pixel per row would cost a cache line.
+// Waves of several images: ds.total in all, se_base the first of each image.
 template <class T, int CONN>
-__global__ __launch_bounds__(kWG) void k_seed_edges(Image im, const uint32_t* edges, int spx,
-                                                     int R, int nedges, int nbounds,
-                                                     uint8_t* flags, int mp_base, int nmx) {
-    const int w = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)), lane = lane_id();
+__global__ __launch_bounds__(kWG) void k_seed_edges(Descs ds, const uint32_t* edges_all, int spx,
+                                                     uint8_t* flags) {
+    const int gw = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)), lane = lane_id();
+    if (gw >= ds.total) return;  // the whole wave
+    const ImgDesc d = desc(ds, search_se(ds, gw));
+    const Image im = image_of(d);
+    const int w = gw - d.se_base, R = d.R, mp_base = d.mp_base, nmx = d.nmx;
+    const int nedges = d.strips - 1, nbounds = (im.H + R - 1) / R - 1;
+    const uint32_t* edges = edges_all + d.edge_base;
     const int nbands = (im.H + 63) / 64, nchunks = (im.W + 63) / 64;
     auto J = [&](int y, int x) -> uint32_t {
         return *gp(reinterpret_cast<const T*>(im.J + (size_t)y * im.pj + (size_t)x * sizeof(T)));
@@ -2354,20 +2361,22 @@ hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, 
     return hipGetLastError();
 }
 
-hipError_t launch_seed_edges(const ImgDesc& d, int dtype, int conn, bool wide,
+long seed_edge_waves(const ImgDesc& d) {
+    const long nedges = std::max(0, d.strips - 1), nbounds = (d.H + d.R - 1) / d.R - 1;
+    return nedges * ((d.H + 63) / 64) + std::max(0L, nbounds) * ((d.W + 63) / 64);
+}
+
+hipError_t launch_seed_edges(const Descs& ds, int dtype, int conn, bool wide,
                              const uint32_t* edges, uint8_t* flags, hipStream_t s) {
+    if (ds.total <= 0) return hipSuccess;
     const int spx = 64 * (wide ? 16 : 4) / dtype;
-    const int nedges = (d.W + spx - 1) / spx - 1, nbounds = (d.H + d.R - 1) / d.R - 1;
-    const long waves = (long)nedges * ((d.H + 63) / 64) + (long)nbounds * ((d.W + 63) / 64);
-    if (waves <= 0) return hipSuccess;
-    const Image im{d.J, d.I, d.pj, d.pi, d.W, d.H};
-    const int grid = (int)((waves + 3) / 4);
+    const int grid = (ds.total + 3) / 4;
     if (dtype == 1) {
-        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint8_t, 8>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
-        else hipLaunchKernelGGL((k_seed_edges<uint8_t, 4>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint8_t, 8>), grid, kWG, 0, s, ds, edges, spx, flags);
+        else hipLaunchKernelGGL((k_seed_edges<uint8_t, 4>), grid, kWG, 0, s, ds, edges, spx, flags);
     } else {
-        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint16_t, 8>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
-        else hipLaunchKernelGGL((k_seed_edges<uint16_t, 4>), grid, kWG, 0, s, im, edges, spx, d.R, nedges, nbounds, flags, d.mp_base, d.nmx);
+        if (conn == 8) hipLaunchKernelGGL((k_seed_edges<uint16_t, 8>), grid, kWG, 0, s, ds, edges, spx, flags);
+        else hipLaunchKernelGGL((k_seed_edges<uint16_t, 4>), grid, kWG, 0, s, ds, edges, spx, flags);
     }
     return hipGetLastError();
 }

@@ -1,6 +1,7 @@
 """GPU parity: every kernel and the full reconstruction, through the C ABI,
 against the CPU oracle on the same inputs.  Integer work: bit-exact."""
 import sys
+import zlib
 
 import numpy as np
 import pytest
@@ -127,6 +128,27 @@ def test_reconstruct_random(torch_cuda, queue, conn, dtype):
                 want = oracle.reconstruct(J, I, conn)
                 got = dj.numpy()
                 assert np.array_equal(got, want), (h, w, kind, sweeps, st)
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_reconstruct_wide_rows(torch_cuda, queue, conn, dtype):
+    # rows of at least 16 KiB take the column pass with 16-byte lanes, whose
+    # last pass also sets the seeding flags and the strips' border columns:
+    # a ragged last strip, a full-width row and heights of several blocks
+    rng = np.random.default_rng(zlib.crc32(f"wide/{queue}/{conn}/{dtype.__name__}".encode()))
+    size = np.dtype(dtype).itemsize
+    for h, w in ((150, 16397 // size), (97, 16384 // size)):
+        for kind in ("seeds", "hdome"):
+            for sweeps in (1, 2):
+                say("wide rows", queue, conn, dtype.__name__, h, w, kind, sweeps)
+                J, I = rand_pair(rng, h, w, dtype, kind)
+                dj, di = up(J), up(I)
+                st = iwpp.reconstruct(dj, di, conn, 0, queue, "plain",
+                                      iwpp.default_options(sweep_rounds=sweeps))
+                got = dj.numpy()
+                assert np.array_equal(got, oracle.reconstruct(J, I, conn)), (h, w, kind, sweeps, st)
 
 
 @pytest.mark.parametrize("queue", VARIANTS)
