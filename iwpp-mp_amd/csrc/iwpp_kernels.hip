@@ -257,8 +257,8 @@ struct VecIO {
 // lanes they are held as 16-bit pairs, pixel k in the low half and pixel
 // k + NP in the high half, so that one packed min/max (v_pk_min_u16 /
 // v_pk_max_u16) advances both halves of the lane's segment; the halves are
-// joined through their (P, Q) summaries.  Narrow lanes hold one pixel per
-// register.  The scan step is out = max(v, min(x, c)); a segment's summary
+// joined through their (P, Q) summaries.  4-byte u8 lanes pair bytes 0 | 2 and
+// 1 | 3 the same way; 4-byte u16 lanes hold one pixel per register.  The scan step is out = max(v, min(x, c)); a segment's summary
 // (P, Q) maps an incoming x to max(P, min(x, Q)).
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 DEV uint32_t pk_max(uint32_t a, uint32_t b) {
@@ -275,7 +275,7 @@ DEV uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amd
 template <class T, int VB>
 struct LanePx {
     using V = VecIO<T, VB>;
-    static constexpr bool PAIRS = VB == 16;
+    static constexpr bool PAIRS = VB == 16 || (VB == 4 && sizeof(T) == 1);
     static constexpr int NW = V::NW;
     static constexpr int NP = PAIRS ? V::NPX / 2 : V::NPX;  // registers per plane
     uint32_t v[NP], c[NP];
@@ -284,6 +284,9 @@ struct LanePx {
     static __device__ __forceinline__ void split(const uint32_t (&w)[NW], uint32_t (&p)[NP]) {
         if constexpr (!PAIRS) {
             V::unpack(w, p);
+        } else if constexpr (VB == 4) {  // u8: p[k] = byte k | byte k+2 << 16
+            p[0] = perm(0u, w[0], 0x0C020C00u);
+            p[1] = perm(0u, w[0], 0x0C030C01u);
         } else if constexpr (sizeof(T) == 1) {  // p[k] = byte k | byte k+8 << 16
 #pragma unroll
             for (int k = 0; k < NP; ++k)
@@ -300,6 +303,8 @@ struct LanePx {
     static __device__ __forceinline__ void join(const uint32_t (&p)[NP], uint32_t (&w)[NW]) {
         if constexpr (!PAIRS) {
             V::pack(p, w);
+        } else if constexpr (VB == 4) {
+            w[0] = perm(p[1], p[0], 0x06020400u);
         } else if constexpr (sizeof(T) == 1) {
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
