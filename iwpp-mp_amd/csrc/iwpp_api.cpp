@@ -136,7 +136,12 @@ extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtyp
         // per-megapixel state, flags and the ring for a single image of
         // max_pixels pixels (edge megapixels of any shape included)
         const size_t nmp = max_pixels / ((size_t)kMP * kMP) + 2 * (size_t)std::sqrt((double)max_pixels) / kMP + 4;
-        if (grow(ws->st, ws->st_n, nmp) != hipSuccess || grow(ws->flags, ws->flags_n, nmp) != hipSuccess) {
+        // strip border columns of the seeding pass for a square image (4-byte
+        // lanes, the narrowest strips: 256 bytes)
+        const size_t side = (size_t)std::ceil(std::sqrt((double)max_pixels));
+        const size_t edge_words = 2 * side * ((side * (size_t)max_dtype + 255) / 256);
+        if (grow(ws->st, ws->st_n, nmp) != hipSuccess || grow(ws->flags, ws->flags_n, nmp) != hipSuccess ||
+            grow(ws->edges, ws->edges_n, edge_words) != hipSuccess) {
             iwpp_workspace_destroy(ws);
             return nullptr;
         }
