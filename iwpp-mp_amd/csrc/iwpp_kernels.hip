@@ -258,8 +258,9 @@ struct VecIO {
 // k + NP in the high half, so that one packed min/max (v_pk_min_u16 /
 // v_pk_max_u16) advances both halves of the lane's segment; the halves are
 // joined through their (P, Q) summaries.  4-byte u8 lanes pair bytes 0 | 2 and
-// 1 | 3 the same way; 4-byte u16 lanes hold one pixel per register.  The scan step is out = max(v, min(x, c)); a segment's summary
-// (P, Q) maps an incoming x to max(P, min(x, Q)).
+// 1 | 3 the same way; 4-byte u16 lanes hold one pixel per register.  The scan
+// step is out = max(v, min(x, c)); a segment's summary (P, Q) maps an incoming
+// x to max(P, min(x, Q)).
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 DEV uint32_t pk_max(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
