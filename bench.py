@@ -66,6 +66,52 @@ def profile_traffic(workload, prefixes, largest=False):
                        f"{what}, weighted by dispatches over the {'/'.join(prefixes)} kernels")
 
 
+def cpu_model():
+    """The host CPU's model name (as lscpu reports it), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def single_kernel_roofline(candidates, workload, largest=False):
+    """Roofline of the single kernel with the largest average launch time.
+    candidates: name -> (total ms per reconstruction, launches, algorithmic
+    bytes per launch, profile prefixes).  Returns (roofline dict, table)."""
+    table = {}
+    for name, (tot, launches, alg, prefixes) in candidates.items():
+        avg = tot / launches if launches else 0.0
+        ach = alg / (avg * 1e-3) / 1e9 if avg > 0 else None
+        table[name] = {"ms_per_reconstruction": tot, "launches": launches, "avg_launch_ms": avg,
+                       "algorithmic_bytes_per_launch": alg, "achieved_GBps": ach,
+                       "frac": ach / HBM_PEAK_GBS if ach else None, "prefixes": prefixes}
+    dom = max(table, key=lambda k: table[k]["avg_launch_ms"])
+    t = table[dom]
+    traffic, traffic_src = profile_traffic(workload, t["prefixes"], largest)
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": t["achieved_GBps"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": t["frac"],
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": t["algorithmic_bytes_per_launch"],
+        "avg_launch_ms": t["avg_launch_ms"],
+        "basis": "the single kernel with the largest average launch time inside the timed steps; "
+                 "algorithmic bytes per launch / that average (the library's HIP events on its "
+                 "stream, around each pass)",
+        "traffic_basis": traffic_src,
+    }
+    for v in table.values():
+        v.pop("prefixes")
+    return roofline, table
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +125,9 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
+    ap.add_argument("--uniform-steps", type=int, default=5,
+                    help="timed steps of the plain schedule at each uniform sweep-round count "
+                         "0..3 (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the reconstruction "
@@ -111,7 +160,8 @@ def main():
 
     from iwpp import learned
 
-    def run_schedule(schedule):
+    def run_schedule(schedule, o=opt, steps=args.steps, warmup=args.warmup, label=None):
+        label = label or schedule
         if schedule == "learned":
             learned.install()
         else:
@@ -119,34 +169,47 @@ def main():
 
         def step():
             return iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue,
-                                    schedule, opt, ws)
+                                    schedule, o, ws)
 
-        for i in range(args.warmup):
+        for i in range(warmup):
             work.buf.copy_(marker0.buf)
             st = step()
-            log(f"{schedule} warmup {i}: {st}")
+            log(f"{label} warmup {i}: {st}")
         # Each step reconstructs a fresh copy of the marker (the reconstruction
         # is in place); the copy is made before the step's timed bracket.
         stats = []
         tot = 0.0
-        for _ in range(args.steps):
+        for _ in range(steps):
             work.buf.copy_(marker0.buf)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             stats.append(step())
             torch.cuda.synchronize()
             tot += time.perf_counter() - t0
-        ms = tot * 1e3 / args.steps
-        log(f"{schedule}: timed {args.steps} steps: {ms:.3f} ms/step, "
+        ms = tot * 1e3 / steps
+        log(f"{label}: timed {steps} steps: {ms:.3f} ms/step, "
             f"{px / (ms * 1e-3) / 1e6:.1f} Mpixel/s")
         return ms, stats, work.numpy()
 
     runs = {}
+    runs_out_uniform = {}
     for sched in (("plain", "learned") if args.schedule == "both" else (args.schedule,)):
         runs[sched] = run_schedule(sched)
     main_sched = "learned" if "learned" in runs else "plain"
     ms_step, stats, result = runs[main_sched]
     value = px / (ms_step * 1e-3) / 1e6
+    # The plain schedule with every uniform number of sweep rounds: the best
+    # of these in hindsight is what a schedule has to beat.
+    uniform = {}
+    if args.uniform_steps > 0 and args.sweeps < 0:
+        for k in range(0, 4):
+            o = iwpp.default_options(sweep_rounds=k)
+            ms_k, st_k, out_k = run_schedule("plain", o, args.uniform_steps, 1,
+                                             f"plain, {k} rounds")
+            uniform[k] = {"ms_per_step": ms_k, "tile_visits": st_k[-1]["tile_visits"],
+                          "ms_sweep": statistics.mean(x["ms_sweep"] for x in st_k),
+                          "ms_queue": statistics.mean(x["ms_queue"] for x in st_k)}
+            runs_out_uniform[k] = out_k
     iwpp.set_model(None)
 
     # --- per-kernel device times (HIP events on the stream the kernels run on)
@@ -189,32 +252,30 @@ def main():
     # (an upper bound for the seeding phase: on a first queue attempt the last
     # column pass sets the flags and k_seed_edges only reads the strip edges).
     queue_name = "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>")
-    kernels = {
-        "sweep passes (k_row_pass*, k_col_pass)":
-            (phases["ms_sweep"], 3 * rounds, bytes_sweep,
-             ("k_row_pass", "k_col_pass")),
+    ms_rows = statistics.mean(s_["ms_rows"] for s_ in stats)
+    ms_cols = statistics.mean(s_["ms_cols"] for s_ in stats)
+    candidates = {
+        "k_row_pass (forward + backward, one launch per round)":
+            (ms_rows, rounds, bytes_sweep, ("k_row_pass",)),
+        "k_col_pass (in place, down and up: two launches per round)":
+            (ms_cols, 2 * rounds, bytes_sweep, ("k_col_pass",)),
         queue_name: (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize, ("k_queue_",)),
         "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
             (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_", "k_unstable_scan")),
     }
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    tot_ms, launches, alg_bytes, prefixes = kernels[dom]
-    avg_ms = tot_ms / max(1, launches)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
-    traffic, traffic_src = profile_traffic(case.name, prefixes)
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": alg_bytes,
-        "avg_launch_ms": avg_ms,
-        "basis": "algorithmic bytes per launch / average launch time inside the timed steps "
-                 "(the library's HIP events on its stream)",
-        "traffic_basis": traffic_src,
+    roofline, per_kernel = single_kernel_roofline(candidates, case.name)
+    sweep_avg = phases["ms_sweep"] / max(1, 3 * rounds)
+    roofline["sweep_family"] = {
+        "kernels": "k_row_pass*, k_col_pass (every pass of the sweep phase)",
+        "avg_launch_ms": sweep_avg,
+        "achieved": bytes_sweep / (sweep_avg * 1e-3) / 1e9 if sweep_avg > 0 else None,
+        "frac": bytes_sweep / (sweep_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if sweep_avg > 0 else None,
+    }
+    roofline["end_to_end"] = {
+        "basis": "3*sizeof(T)*pixels (every pixel's marker and mask read once and marker "
+                 "written once) / step time",
+        "achieved": 3 * tsize * px / (ms_step * 1e-3) / 1e9,
+        "frac": 3 * tsize * px / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
     sweep_kernels = {
         "row_pass_ms": ms_row,
@@ -223,7 +284,7 @@ def main():
         "col_pass_ms": ms_col,
         "col_pass_GBps": bytes_sweep / (ms_col * 1e-3) / 1e9,
         "col_pass_frac": bytes_sweep / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "per_reconstruction_ms": {k: v[0] for k, v in kernels.items()},
+        "per_kernel": per_kernel,
         "basis": "row_pass / col_pass: one isolated launch each over the clipped marker, HIP events",
     }
 
@@ -244,8 +305,9 @@ def main():
         cpu = {"value": px / med / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
                "sample": f"full {case.name} image, median of {len(times)} runs "
                          f"({med * 1e3:.1f} ms each), oracle/iwpp_oracle.c -O3",
-               "host_cpus": os.cpu_count()}
-        exact = all(bool(np.array_equal(r[2], want)) for r in runs.values())
+               "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
+        exact = all(bool(np.array_equal(r[2], want)) for r in runs.values()) and \
+            all(bool(np.array_equal(o, want)) for o in runs_out_uniform.values())
         log(f"cpu oracle {med * 1e3:.1f} ms; GPU outputs bit-exact: {exact}")
 
     out = {
@@ -279,6 +341,12 @@ def main():
     }
     if len(runs) == 2:
         out["learned_vs_plain"] = runs["plain"][0] / runs["learned"][0]
+    if uniform:
+        best_k = min(uniform, key=lambda k: uniform[k]["ms_per_step"])
+        out["uniform_rounds"] = uniform
+        out["best_uniform_rounds"] = best_k
+        out[f"{main_sched}_vs_best_uniform"] = uniform[best_k]["ms_per_step"] / ms_step
+    ws.close()
     print(json.dumps(out), flush=True)
 
 
@@ -394,24 +462,30 @@ def bench_batch(args):
                 break
         cpu = {"value": px_cpu / t_cpu / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
                "sample": f"first {px_cpu} px of the c4 images in order, "
-                         f"{t_cpu:.1f} s, oracle/iwpp_oracle.c -O3", "host_cpus": os.cpu_count()}
+                         f"{t_cpu:.1f} s, oracle/iwpp_oracle.c -O3", "host_cpus": os.cpu_count(),
+               "cpu_model": cpu_model()}
         exact = bool(ok and batch_same and graph_same and learned_same)
         log(f"oracle on the sample: {ok}")
     phases = {k: batch_st[k] for k in ("ms_sweep", "ms_seed", "ms_queue")}
     rounds = max(1, batch_st["sweep_rounds"])
-    kernels = {
+    tsize = cases[0].mask.itemsize
+    candidates = {
+        "k_row_pass_wave (forward + backward, one launch per round)":
+            (batch_st["ms_rows"], rounds, 3 * tsize * px, ("k_row_pass",)),
+        "k_col_pass (in place, down and up: two launches per round)":
+            (batch_st["ms_cols"], 2 * rounds, 3 * tsize * px, ("k_col_pass",)),
         "k_queue_" + ("mutex" if batch_st["queue"] == 0 else "bsp"):
-            (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * cases[0].mask.itemsize,
+            (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * tsize,
              ("k_queue_",)),
-        "sweep passes (k_row_pass_wave, k_col_pass)":
-            (phases["ms_sweep"], 3 * rounds, 3 * cases[0].mask.itemsize * px,
-             ("k_row_pass", "k_col_pass")),
+        "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
+            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_", "k_unstable_scan")),
     }
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    tot, launches, alg, prefixes = kernels[dom]
-    avg = tot / max(1, launches)
-    achieved = alg / (avg * 1e-3) / 1e9 if avg > 0 else None
-    traffic, traffic_src = profile_traffic("c4_batch_256_u8", prefixes, largest=True)
+    roofline, per_kernel = single_kernel_roofline(candidates, "c4_batch_256_u8", largest=True)
+    roofline["end_to_end"] = {
+        "basis": "3*sizeof(T)*pixels / time of the fastest batched form",
+        "achieved": 3 * tsize * px / (best_ms * 1e-3) / 1e9,
+        "frac": 3 * tsize * px / (best_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
     out = {
         "metric": METRIC,
         "value": px / best_ms / 1e3,
@@ -427,11 +501,8 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {"workload": "c4_batch_256_u8", "images": len(cases), "pixels": px,
                    "connectivity": 8, "form": form, "tuning": tuning},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": traffic, "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg,
-                     "basis": "batched call's phase times (the library's HIP events), per launch",
-                     "traffic_basis": traffic_src},
+        "roofline": roofline,
+        "per_kernel": per_kernel,
         "cpu_baseline": cpu,
         "exact": exact,
         "per_image_loop": {"ms": ms_loop, "mpixel_s": px / ms_loop / 1e3},
@@ -444,7 +515,10 @@ def bench_batch(args):
         "learned_batch_vs_loop": ms_loop / ms_learned,
         "learned_batch_vs_plain_batch": ms_batch / ms_learned,
     }
+    # release the graph before its workspace, both before the interpreter exits
     g.close()
+    gws.close()
+    ws.close()
     print(json.dumps(out), flush=True)
 
 

@@ -86,6 +86,11 @@ typedef struct {
     double us_converge;      /* local sweeps */
     double us_store;         /* write-back */
     double us_notify;        /* neighbour tests and queue pushes */
+    /* Split of ms_sweep (HIP events around each pass): the row passes, and
+     * the column passes (downward and upward, the learned schedule's network
+     * kernel excluded). */
+    float ms_rows;
+    float ms_cols;
 } iwpp_stats;
 
 /* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
@@ -251,6 +256,21 @@ int iwpp_sweep_cols_block_width(const void* src, const void* dst, const void* ma
 int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pitch_marker,
                              size_t pitch_mask, int width, int height, int dtype,
                              int connectivity, uint8_t* flags, int* count, void* stream);
+
+/* One sweep round (row pass, column pass down, column pass up, all in place,
+ * clipping first) over n images of one dtype, whose upward column pass sets
+ * the megapixel seeding flags as a reconstruction's last sweep round does
+ * before its queue phase (the test of pairs inside each column block and
+ * strip during the pass, then the test across strip edges and block
+ * boundaries).  flags (device uint8): one per megapixel, the images' grids
+ * (ceil(h/64) x ceil(w/64), row-major) one after another in batch order;
+ * 1 = a pixel of that megapixel can still be raised by a neighbour.
+ * block_rows: column block height (0 = as a reconstruction chooses it,
+ * otherwise a multiple of 32).  No reference counterpart: the reference
+ * seeds every megapixel (IWPP MP/src/globalQueue.cu:83-97); this exposes the
+ * pass that replaces it, so that its flags can be checked on their own. */
+int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int connectivity,
+                          int block_rows, uint8_t* flags, iwpp_workspace* ws, void* stream);
 
 /* Message for the last error on the calling thread ("" if none). */
 const char* iwpp_last_error(void);

@@ -92,6 +92,8 @@ struct iwpp_workspace {
     size_t tiles_n = 0;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // phase timing,
                                                                         // [4]: learned plan ready
+    std::vector<hipEvent_t> ev_round;  // per sweep round: row pass start, row pass end,
+                                       // column passes start
     ImgDesc* descs = nullptr;   // batch descriptors (device)
     size_t descs_n = 0;
     uint32_t* img_of_mp = nullptr;  // global megapixel -> image (device)
@@ -171,6 +173,8 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->plan_count);
     if (ws->own) (void)hipStreamDestroy(ws->own);
     for (hipEvent_t e : ws->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ws->ev_round)
         if (e) (void)hipEventDestroy(e);
     delete ws;
 }
@@ -270,6 +274,7 @@ struct Run {
     int plan_tiles = 0, plan_tile = 0;
     const uint32_t* plan_tile_base = nullptr;
     uint32_t plan_seq = 0;
+    int timed_rounds = 0;  // sweep rounds with per-pass events (ws->ev_round)
 };
 
 int learned_plan_predict(Run& r);
@@ -446,6 +451,18 @@ int sweep_phase(Run& r, int k0, int k1) {
             r.h[0].tm = tm;
             r.ds.one.tm = tm;
         }
+        hipEvent_t* ev = nullptr;  // this round's pass events (timed runs)
+        if (r.timed) {
+            std::vector<hipEvent_t>& er = r.ws->ev_round;
+            while (er.size() < 3 * (size_t)(k + 1)) {
+                hipEvent_t e = nullptr;
+                HIP_TRY(hipEventCreate(&e));
+                er.push_back(e);
+            }
+            ev = er.data() + 3 * k;
+            r.timed_rounds = k + 1;
+            HIP_TRY(hipEventRecord(ev[0], r.s));
+        }
         if (r.ds.n == 1) {
             const ImgDesc& d = r.h[0];
             HIP_TRY(launch_row_pass(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, k == 0, d.tm,
@@ -454,10 +471,12 @@ int sweep_phase(Run& r, int k0, int k1) {
             HIP_TRY(launch_row_pass_batch(with_total(r.ds, r.total_rows, k), r.dtype, k == 0, r.row16,
                                           r.row_wave, nullptr, r.s));
         }
+        if (ev) HIP_TRY(hipEventRecord(ev[1], r.s));
         // learned plan: the network goes between the first row pass and the
         // column passes, so the host plans while those run
         const int rp = learned_plan_predict(r);
         if (rp) return rp;
+        if (ev) HIP_TRY(hipEventRecord(ev[2], r.s));
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
                                       COL_IN_PLACE, r.wide, nullptr, r.s));
         if (r.plan_end) {  // the host plans while the downward pass runs
@@ -566,7 +585,7 @@ int queue_launches(Run& r) {
 int read_counters(Run& r) {
     const uint32_t* h = r.ws->host_ctl;
     const double us_per_tick = 1e6 / ((double)timeout_ticks(1.0));
-    r.st.seeded += h[C_TAIL * kCtlStride] + 0ll;
+    r.st.seeded += h[C_SEEDED * kCtlStride] + 0ll;  // (C_TAIL also counts the mutex ring's pushes)
     r.st.tile_visits += (long long)(*(const unsigned long long*)(h + C_VISITS_LO * kCtlStride));
     r.st.queue_rounds += (int)h[C_ROUND * kCtlStride];
     if (r.q.trace) {
@@ -982,7 +1001,18 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
     if (rc) return rc;
     if (r.timed) {
         float a = 0;
-        if (r.st.restarts == 0) HIP_TRY(hipEventElapsedTime(&a, r.ws->ev[0], r.ws->ev[1]));
+        if (r.st.restarts == 0) {
+            HIP_TRY(hipEventElapsedTime(&a, r.ws->ev[0], r.ws->ev[1]));
+            const hipEvent_t* er = r.ws->ev_round.data();
+            for (int k = 0; k < r.timed_rounds; ++k) {
+                float rows = 0, cols = 0;
+                HIP_TRY(hipEventElapsedTime(&rows, er[3 * k], er[3 * k + 1]));
+                HIP_TRY(hipEventElapsedTime(&cols, er[3 * k + 2],
+                                            k + 1 < r.timed_rounds ? er[3 * k + 3] : r.ws->ev[1]));
+                r.st.ms_rows += rows;
+                r.st.ms_cols += cols;
+            }
+        }
         r.st.ms_sweep = a;
     }
     return IWPP_OK;
@@ -1412,6 +1442,43 @@ extern "C" int iwpp_unstable_megapixels(const void* marker, const void* mask, si
     ds.total = d.strips * d.nmy;
     HIP_TRY(launch_unstable_batch(ds, dtype, conn, wide, flags, (uint32_t*)count,
                                   (hipStream_t)stream));
+    return IWPP_OK;
+}
+
+extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int conn,
+                                     int block_rows, uint8_t* flags, iwpp_workspace* ws,
+                                     void* stream) {
+    g_err.clear();
+    if (!imgs || n <= 0) return fail(IWPP_ERR_ARG, "empty batch");
+    if (!flags) return fail(IWPP_ERR_ARG, "null flags");
+    for (int i = 0; i < n; ++i) {
+        const int rc = check_image(imgs[i].marker, imgs[i].mask, imgs[i].pitch_marker,
+                                   imgs[i].pitch_mask, imgs[i].width, imgs[i].height, dtype);
+        if (rc) return rc;
+    }
+    int rc = check_common(dtype, conn, 0, IWPP_QUEUE_MUTEX, IWPP_SCHED_PLAIN);
+    if (rc) return rc;
+    if (block_rows < 0 || block_rows % 32) return fail(IWPP_ERR_ARG, "block_rows must be 0 or a multiple of 32");
+    if (!ws) ws = default_ws();
+    if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
+    Run r{};
+    r.dtype = dtype;
+    r.conn = conn;
+    r.ws = ws;
+    r.queue = Q_MUTEX;
+    iwpp_default_options(&r.opt);
+    r.opt.col_block_rows = block_rows;
+    HIP_TRY(call_stream(ws, stream, &r.s));
+    rc = setup_images(r, imgs, n);
+    if (rc) return rc;
+    r.seed_round = 0;  // the upward pass of this one round sets the flags
+    rc = sweep_phase(r, 0, 1);
+    if (rc) return rc;
+    if (!r.flags_ready) return fail(IWPP_ERR_UNSUPPORTED, "the seeding pass did not run");
+    HIP_TRY(launch_seed_edges(with_total(r.ds, r.total_se), r.dtype, r.conn, r.wide, ws->edges,
+                              ws->flags, r.s));
+    HIP_TRY(hipMemcpyAsync(flags, ws->flags, (size_t)r.total_mp, hipMemcpyDeviceToDevice, r.s));
+    if (!stream) HIP_TRY(hipStreamSynchronize(r.s));
     return IWPP_OK;
 }
 

@@ -10,8 +10,11 @@ Reference interface mirrored here (IWPP MP/include/MorphologicalReconstruction.c
     MorphologicalReconstruction(marker, mask, pitchMarker, pitchMask, size)
 with device pointers, row pitches in bytes, a square uint8 image, in place.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -53,7 +56,8 @@ class Stats(ctypes.Structure):
                 ("ms_plan", ctypes.c_float),
                 ("us_wait", ctypes.c_double), ("us_fetch", ctypes.c_double),
                 ("us_converge", ctypes.c_double), ("us_store", ctypes.c_double),
-                ("us_notify", ctypes.c_double)]
+                ("us_notify", ctypes.c_double), ("ms_rows", ctypes.c_float),
+                ("ms_cols", ctypes.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -88,6 +92,7 @@ SIGNATURES = {
     "iwpp_mlp_backward": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, ctypes.c_float, ctypes.c_float,
                                _VP, _VP, _VP]),
     "iwpp_mlp_adam": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.c_float, _I, _VP]),
+    "iwpp_sweep_round_seed": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
     "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
 }
@@ -216,18 +221,59 @@ def _stream(stream):
     return int(stream)
 
 
+# Library handles that own device memory, streams, events or graphs.  They
+# are released explicitly (close()), when collected, or by the exit hook
+# below, which runs while the HIP runtime is still up: graphs first, then
+# workspaces.  Finalizers that run during interpreter shutdown do nothing,
+# since by then the runtime (torch's, which this library shares) may already
+# be unloading.
+_LIVE = {}
+
+
+def _track(obj):
+    _LIVE[id(obj)] = weakref.ref(obj)
+
+
+def _untrack(obj):
+    _LIVE.pop(id(obj), None)
+
+
+def _release_all():
+    objs = [r() for r in list(_LIVE.values())]
+    objs = [o for o in objs if o is not None]
+    for o in sorted(objs, key=lambda o: 0 if isinstance(o, BatchGraph) else 1):
+        try:
+            o.close()
+        except Exception:
+            pass
+    _LIVE.clear()
+
+
+atexit.register(_release_all)
+
+
 class Workspace:
     def __init__(self, max_pixels=0, max_dtype=U8):
         self.handle = lib().iwpp_workspace_create(max_pixels, max_dtype)
         if not self.handle:
             raise IWPPError(-5, "iwpp_workspace_create failed")
+        _track(self)
 
     def close(self):
         if self.handle:
             lib().iwpp_workspace_destroy(self.handle)
             self.handle = None
+        _untrack(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -319,6 +365,7 @@ class BatchGraph:
         if not self.handle:
             msg = lib().iwpp_last_error()
             raise IWPPError(-2, msg.decode() if msg else "graph capture failed")
+        _track(self)
 
     def _sid(self):
         st = self.stream
@@ -341,8 +388,11 @@ class BatchGraph:
         if self.handle:
             lib().iwpp_batch_graph_destroy(self.handle)
             self.handle = None
+        _untrack(self)
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -453,6 +503,27 @@ def unstable_megapixels(marker, mask, connectivity, stream=None):
                                           flags.data_ptr(), count.data_ptr(), _stream(stream)))
     torch.cuda.synchronize()
     return flags.cpu().numpy(), int(count.item())
+
+
+def sweep_round_seed(pairs, connectivity=8, block_rows=0, workspace=None, stream=None):
+    """One in-place sweep round over a batch of (marker, mask) pairs whose
+    last column pass sets the megapixel seeding flags; returns one uint8
+    numpy flag grid (ceil(h/64), ceil(w/64)) per image."""
+    torch = _torch()
+    arr, dtype = _batch_array(pairs)
+    grids = [((mk.h + MP_SIZE - 1) // MP_SIZE, (mk.w + MP_SIZE - 1) // MP_SIZE) for mk, _ in pairs]
+    total = sum(a * b for a, b in grids)
+    flags = torch.full((total,), 0xEE, dtype=torch.uint8, device=pairs[0][0].buf.device)
+    _raise(lib().iwpp_sweep_round_seed(arr, len(pairs), dtype, connectivity, block_rows,
+                                       flags.data_ptr(), workspace.handle if workspace else None,
+                                       _stream(stream)))
+    torch.cuda.synchronize()
+    host = flags.cpu().numpy()
+    out, off = [], 0
+    for a, b in grids:
+        out.append(host[off:off + a * b].reshape(a, b).copy())
+        off += a * b
+    return out
 
 
 def MorphologicalReconstruction(marker, mask, pitchMarker, pitchMask, size):

@@ -242,11 +242,84 @@ def _full_case_check(case, queues):
 
 
 def test_config3_full(torch_cuda):
-    _full_case_check(W.c3(), ("mutex", "dq", "uq"))
+    case = W.c3()
+    _full_case_check(case, ("mutex", "dq", "uq"))
+    from iwpp import learned
+    learned.install()
+    try:
+        say(case.name, "learned")
+        dj = up(case.marker)
+        st = iwpp.reconstruct(dj, up(case.mask), case.connectivity, case.tile, "uq", "learned")
+        assert st["schedule"] == iwpp.SCHEDULES["learned"], st
+        assert np.array_equal(dj.numpy(), oracle.reconstruct(case.marker, case.mask,
+                                                             case.connectivity)), st
+    finally:
+        iwpp.set_model(None)
 
 
-def test_config5_full(torch_cuda):
-    _full_case_check(W.c5(), ("mutex", "uq"))
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("pitch_kind", ["aligned", "odd"])
+def test_clip(torch_cuda, dtype, pitch_kind):
+    # the stand-alone clip entry point (IWPP MP/src/MorphologicalReconstruction.cu:827-836)
+    rng = np.random.default_rng(21)
+    top = 255 if dtype == np.uint8 else 4095
+    for h, w in [(1, 1), (3, 17), (64, 1023), (130, 4099), (17, 16400)]:
+        say("clip", dtype.__name__, pitch_kind, h, w)
+        J = rng.integers(0, top + 1, (h, w)).astype(dtype)
+        I = rng.integers(0, top + 1, (h, w)).astype(dtype)
+        pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
+        dj, di = up(J, pitch), up(I, pitch)
+        raw_before = dj.raw_numpy()
+        iwpp.clip(dj, di)
+        torch_cuda.cuda.synchronize()
+        assert np.array_equal(dj.numpy(), oracle.clip(J, I)), (h, w)
+        rb = w * np.dtype(dtype).itemsize
+        assert np.array_equal(dj.raw_numpy()[:, rb:], raw_before[:, rb:])
+
+
+# Batches for the fused seeding test: single images with 16-byte lanes (rows
+# of at least 16 KiB, a ragged last strip), with 4-byte lanes (odd pitch, or
+# rows under 16 KiB), and mixed-size batches whose column strips are busy
+# enough for 16-byte lanes or not.
+def _seed_batches(dtype):
+    size = np.dtype(dtype).itemsize
+    return [
+        ("wide ragged", [(150, 16397 // size)], None),
+        ("wide full", [(300, 16384 // size)], None),
+        ("odd pitch", [(333, 1000)], "odd"),
+        ("narrow", [(200, 3000)], None),
+        ("batch busy", [(100, 4096 // size), (64, 2048 // size), (130, 3072 // size)], None),
+        ("batch sparse", [(300, 200), (64, 700), (513, 65), (1, 50), (129, 129)], None),
+    ]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("conn", [4, 8])
+def test_fused_seeding_flags(torch_cuda, dtype, conn):
+    # The last column pass of a sweep phase sets the seeding flags itself
+    # (tests inside its blocks and strips, then across strip edges and block
+    # boundaries).  Its flags must be exactly those of the stability test on
+    # the marker it leaves: a missing flag loses work, an extra one wastes a
+    # visit.
+    rng = np.random.default_rng(zlib.crc32(f"seedflags/{dtype.__name__}/{conn}".encode()))
+    for name, shapes, pitch_kind in _seed_batches(dtype):
+        for block_rows in (0, 32, 96):
+            for kind in ("seeds", "hdome", "random"):
+                say("fused seeding", dtype.__name__, conn, name, block_rows, kind)
+                cases = [rand_pair(rng, h, w, dtype, kind) for h, w in shapes]
+                pairs = []
+                for (J, I), (h, w) in zip(cases, shapes):
+                    pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
+                    pairs.append((up(J, pitch), up(I, pitch)))
+                flags = iwpp.sweep_round_seed(pairs, conn, block_rows)
+                for (J, I), (dj, _), f in zip(cases, pairs, flags):
+                    after = dj.numpy()
+                    want = oracle.unstable_blocks(after, I, conn)
+                    assert np.array_equal(f, want), (name, J.shape, block_rows, kind,
+                                                     int(f.sum()), int(want.sum()))
+                    # the round only raised pixels towards the reconstruction
+                    assert (after >= oracle.clip(J, I)).all()
+                    assert (after <= oracle.reconstruct(J, I, conn)).all()
 
 
 def _batch_inputs(rng, shapes, dtype):
