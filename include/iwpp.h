@@ -272,6 +272,26 @@ int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pitch_
 int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int connectivity,
                           int block_rows, uint8_t* flags, iwpp_workspace* ws, void* stream);
 
+/* Allocation-free form of iwpp_reconstruct_ex.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  One image, plain schedule, mutex queue variant, default tile;
+ * opt (NULL = defaults) must leave queue_capacity at 0.  The image may hold
+ * at most IWPP_NOALLOC_MAX_MP megapixels of 64 x 64 pixels (one 32768 x 32768
+ * image); larger ones are rejected with IWPP_ERR_ARG before anything runs.
+ * Its scratch (control block, megapixel states and flags, queue ring:
+ * 5.25 MiB) is a static device array that the library holds on each GPU it
+ * runs on until the process ends; the call allocates and frees nothing.
+ * Every call on one GPU shares that array, which keeps the last call's
+ * contents (each call clears what must start at zero): calls that overlap on
+ * one GPU (several host threads or streams) race, and the caller must
+ * serialise them.  stream NULL = the NULL stream.  Synchronises the stream
+ * before returning (to read the queue phase's status). */
+#define IWPP_NOALLOC_MAX_MP 262144
+int iwpp_reconstruct_noalloc(void* marker, const void* mask, size_t pitch_marker,
+                             size_t pitch_mask, int width, int height, int dtype,
+                             int connectivity, const iwpp_options* opt, void* stream);
+
 /* Message for the last error on the calling thread ("" if none). */
 const char* iwpp_last_error(void);
 

@@ -55,9 +55,14 @@ size_t next_pow2(size_t v) {
     return p;
 }
 
+// Set while an allocation-free call runs on this thread: grow() then refuses
+// to allocate (its buffers are carved out of the static scratch beforehand).
+thread_local bool t_no_alloc = false;
+
 template <class P>
 hipError_t grow(P*& p, size_t& have, size_t need_elems) {
     if (p && have >= need_elems) return hipSuccess;
+    if (t_no_alloc) return hipErrorOutOfMemory;
     if (p) {
         hipError_t e = hipFree(p);
         if (e != hipSuccess) return e;
@@ -275,6 +280,7 @@ struct Run {
     const uint32_t* plan_tile_base = nullptr;
     uint32_t plan_seq = 0;
     int timed_rounds = 0;  // sweep rounds with per-pass events (ws->ev_round)
+    bool scan_seed = false;  // seed from k_unstable_scan only (no strip-edge buffer)
 };
 
 int learned_plan_predict(Run& r);
@@ -492,7 +498,7 @@ int sweep_phase(Run& r, int k0, int k1) {
         }
         // the last pass sets the seeding flags (k_unstable_scan is skipped)
         // when it is the last of every image (no tile schedule)
-        bool seed = k == r.seed_round && !r.round_masks;
+        bool seed = !r.scan_seed && k == r.seed_round && !r.round_masks;
         for (int i = 0; seed && i < r.ds.n; ++i) seed = r.h[i].rounds > k;
         if (seed) {
             const size_t words = ((size_t)r.total_mp + 3) / 4;
@@ -517,7 +523,7 @@ int setup_queue(Run& r) {
     HIP_TRY(grow(ws->st, ws->st_n, nmp));
     HIP_TRY(grow(ws->flags, ws->flags_n, (nmp + 3) / 4 * 4));
     // (sized here too, so that a graph's dry run allocates it before capture)
-    HIP_TRY(grow(ws->edges, ws->edges_n, std::max<size_t>(1, r.total_edges)));
+    if (!r.scan_seed) HIP_TRY(grow(ws->edges, ws->edges_n, std::max<size_t>(1, r.total_edges)));
     size_t cap;
     if (kind == Q_MUTEX) {
         cap = next_pow2(nmp + 16384);
@@ -1480,6 +1486,115 @@ extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, i
     HIP_TRY(hipMemcpyAsync(flags, ws->flags, (size_t)r.total_mp, hipMemcpyDeviceToDevice, r.s));
     if (!stream) HIP_TRY(hipStreamSynchronize(r.s));
     return IWPP_OK;
+}
+
+// ---------------------------------------------- allocation-free entry point --
+// iwpp_reconstruct_noalloc runs the single-image, plain-schedule, mutex-queue
+// reconstruction of iwpp_reconstruct_ex with every device buffer it needs
+// beyond the caller's images carved out of one static device array, so that
+// it never allocates.  Layout (bytes from the array's start):
+//   control block | megapixel states (u32) | seeding flags (u8) | queue ring (u64)
+// Seeding takes k_unstable_scan's flags (the fused form of the last column
+// pass needs a strip-edge buffer that grows with the image's height).
+namespace {
+constexpr size_t kNaMaxMP = IWPP_NOALLOC_MAX_MP;
+constexpr size_t kNaRing = 524288;  // ring entries: next_pow2(kNaMaxMP + 16384)
+constexpr size_t kNaCtlBytes = 4096;
+constexpr size_t kNaStOff = kNaCtlBytes;
+constexpr size_t kNaFlagsOff = kNaStOff + kNaMaxMP * 4;
+constexpr size_t kNaRingOff = kNaFlagsOff + kNaMaxMP;
+constexpr size_t kNaBytes = kNaRingOff + kNaRing * 8;
+static_assert((size_t)C_NSLOTS * kCtlStride * 4 <= kNaCtlBytes, "control block fits");
+static_assert(kNaRing >= kNaMaxMP + 16384 && (kNaRing & (kNaRing - 1)) == 0, "ring size");
+static_assert(kNaRingOff % 256 == 0, "ring alignment");
+}  // namespace
+
+// One copy per GPU, held until the process ends.  In host code this name
+// only stands for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};
+
+// The current device's address of `scratch` (looked up on the first call
+// with that device current).
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) return fail(IWPP_ERR_UNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(IWPP_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        g_na_ptr[dev] = p;
+    }
+    *out = g_na_ptr[dev];
+    return IWPP_OK;
+}
+
+struct NoAllocScope {
+    NoAllocScope() { t_no_alloc = true; }
+    ~NoAllocScope() { t_no_alloc = false; }
+};
+}  // namespace
+
+extern "C" int iwpp_reconstruct_noalloc(void* marker, const void* mask, size_t pm, size_t pk,
+                                        int w, int h, int dtype, int conn,
+                                        const iwpp_options* opt, void* stream) {
+    g_err.clear();
+    int rc = check_image(marker, mask, pm, pk, w, h, dtype);
+    if (rc) return rc;
+    rc = check_common(dtype, conn, 0, IWPP_QUEUE_MUTEX, IWPP_SCHED_PLAIN);
+    if (rc) return rc;
+    iwpp_options o;
+    iwpp_default_options(&o);
+    if (opt) o = *opt;
+    if (o.queue_capacity != 0)
+        return fail(IWPP_ERR_ARG, "queue_capacity must be 0: the allocation-free entry point "
+                                  "sizes its queue itself");
+    const size_t nmp = (size_t)((w + kMP - 1) / kMP) * (size_t)((h + kMP - 1) / kMP);
+    if (nmp > kNaMaxMP)
+        return fail(IWPP_ERR_ARG, "image of %zu megapixels (64 x 64 pixels each): the "
+                                  "allocation-free entry point takes at most %zu",
+                    nmp, kNaMaxMP);
+    void* base = nullptr;
+    rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    // A workspace whose buffers are views of the scratch, each at its full
+    // size, so that nothing grows; never passed to iwpp_workspace_destroy.
+    iwpp_workspace fx;
+    fx.device = -1;
+    fx.ctl = static_cast<uint32_t*>(base);
+    fx.ctl_n = kNaCtlBytes / 4;
+    fx.st = reinterpret_cast<uint32_t*>(static_cast<char*>(base) + kNaStOff);
+    fx.st_n = kNaMaxMP;
+    fx.flags = reinterpret_cast<uint8_t*>(static_cast<char*>(base) + kNaFlagsOff);
+    fx.flags_n = kNaMaxMP;
+    fx.ring = reinterpret_cast<unsigned long long*>(static_cast<char*>(base) + kNaRingOff);
+    fx.ring_n = kNaRing;
+    alignas(16) uint32_t host_ctl[(size_t)C_NSLOTS * kCtlStride];  // counters read back here
+    fx.host_ctl = host_ctl;
+    Run r{};
+    r.dtype = dtype;
+    r.conn = conn;
+    r.ws = &fx;
+    r.s = (hipStream_t)stream;  // NULL: the NULL stream itself (the mutex kernel is an
+                                // ordinary launch)
+    r.timed = false;
+    r.scan_seed = true;
+    const iwpp_image img{marker, mask, pm, pk, w, h};
+    NoAllocScope guard;
+    rc = reconstruct_run(r, &img, 1, 0, IWPP_QUEUE_MUTEX, IWPP_SCHED_PLAIN, &o);
+    if (rc == IWPP_ERR_HIP && std::strstr(g_err.c_str(), "grow("))
+        return fail(IWPP_ERR_UNSUPPORTED, "the allocation-free call needed device memory "
+                                          "beyond its static scratch");
+    return rc;
 }
 
 extern "C" const char* iwpp_last_error(void) { return g_err.c_str(); }

@@ -93,6 +93,8 @@ SIGNATURES = {
                                _VP, _VP, _VP]),
     "iwpp_mlp_adam": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.c_float, _I, _VP]),
     "iwpp_sweep_round_seed": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP]),
+    "iwpp_reconstruct_noalloc": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I,
+                                      ctypes.POINTER(Options), _VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
     "MorphologicalReconstruction": (None, [_VP, _VP, _SZ, _SZ, _I]),
 }
@@ -295,6 +297,20 @@ def reconstruct(marker, mask, connectivity=8, tile=0, queue="uq", schedule="plai
                                    workspace.handle if workspace else None, _stream(stream))
     _raise(rc)
     return st.as_dict()
+
+
+NOALLOC_MAX_MP = 262144
+
+
+def reconstruct_noalloc(marker, mask, connectivity=8, options=None, stream=None):
+    """In-place reconstruction through the allocation-free entry point (mutex
+    queue, plain schedule; scratch in the library's static device array).
+    Synchronises the stream."""
+    assert marker.h == mask.h and marker.w == mask.w and marker.dtype == mask.dtype
+    opt = options if options is not None else default_options()
+    _raise(lib().iwpp_reconstruct_noalloc(marker.ptr, mask.ptr, marker.pitch, mask.pitch,
+                                          marker.w, marker.h, marker.dtype, connectivity,
+                                          ctypes.byref(opt), _stream(stream)))
 
 
 class ImageDesc(ctypes.Structure):
