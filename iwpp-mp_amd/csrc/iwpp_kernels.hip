@@ -1033,18 +1033,64 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, u
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
-// A wave owns one megapixel at a time.  Its LDS tile holds rows -1..64 and
-// columns -1..64 (halo included, absent pixels 0) as packed words
-// (marker | mask << 16).  Row stride 67 words: lane-per-row and
-// lane-per-column accesses are both bank-conflict free (67 is odd, 67 mod 64
-// = 3 is a unit).  Every sweep is a per-lane sequential scan: lane = row for
-// the row sweeps, lane = column for the column sweeps.
-constexpr int TS = 67;             // LDS row stride, words
-constexpr int TILE_WORDS = 66 * TS;
-constexpr int ROSE_BYTES = 4 * 64; // per-border "rose in this visit" flags
-constexpr int WAVE_LDS = TILE_WORDS * 4 + ROSE_BYTES;
+// A wave owns one megapixel at a time.  Its LDS tile holds rows -1..64 of the
+// 72 columns -4..67 (the megapixel, its one-pixel halo, and three columns on
+// each side that are loaded with the halo columns and never read); absent
+// pixels are 0.  In registers every pixel is a word (marker | mask << 16); in
+// LDS:
+//  * uint8 images: 16-bit pixels (marker | mask << 8), row stride 148 bytes
+//    = 37 words (odd: a lane-per-row word access is bank-conflict free; a
+//    lane-per-column 16-bit access reads one contiguous row per step).
+//    9.8 KiB per wave: four waves per SIMD fit the LDS.
+//  * uint16 images: 32-bit pixels (marker | mask << 16), row stride 73 words.
+// A fetching lane's 4 pixels land on word boundaries (8 or 16 bytes) and are
+// written with no per-pixel test.  Every sweep is a per-lane sequential scan:
+// lane = row for the row sweeps, lane = column for the column sweeps.
+typedef __attribute__((address_space(3))) uint8_t lds8;
+typedef __attribute__((address_space(3))) uint16_t lds16;
+typedef __attribute__((address_space(3))) uint32_t lds32;
 
-DEV int tix(int r, int c) { return (r + 1) * TS + (c + 1); }  // r, c in [-1, 64]
+template <class T>
+struct TileFmt;
+template <>
+struct TileFmt<uint8_t> {
+    static constexpr int PXB = 2;    // bytes per pixel
+    static constexpr int RB = 148;   // row stride, bytes
+};
+template <>
+struct TileFmt<uint16_t> {
+    static constexpr int PXB = 4;
+    static constexpr int RB = 73 * 4;
+};
+template <class T>
+constexpr int TILE_BYTES = 66 * TileFmt<T>::RB;
+constexpr int ROSE_BYTES = 4 * 64;  // per-border "rose in this visit" flags
+template <class T>
+constexpr int WAVE_LDS = (TILE_BYTES<T> + ROSE_BYTES + 15) / 16 * 16;
+
+template <class T>
+constexpr int toff(int r, int c) {  // r in [-1, 64], c in [-4, 67]
+    return (r + 1) * TileFmt<T>::RB + (c + 4) * TileFmt<T>::PXB;
+}
+// 16-bit tile pixel (marker | mask << 8) <-> register word (marker | mask << 16)
+DEV uint32_t widen8(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c010c00u); }
+DEV uint32_t narrow8(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0c0c0200u); }
+// two register words -> one tile word of two 16-bit pixels (a low, b high)
+DEV uint32_t narrow8x2(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x06040200u); }
+
+// Pixel at byte offset o of the tile as a word (marker | mask << 16).
+template <class T>
+DEV uint32_t tld_at(const lds8* p) {
+    if constexpr (sizeof(T) == 1) return widen8(*reinterpret_cast<const lds16*>(p));
+    else return *reinterpret_cast<const lds32*>(p);
+}
+template <class T>
+DEV void tst_at(lds8* p, uint32_t w) {
+    if constexpr (sizeof(T) == 1) *reinterpret_cast<lds16*>(p) = (uint16_t)narrow8(w);
+    else *reinterpret_cast<lds32*>(p) = w;
+}
+template <class T>
+DEV uint32_t tld(const lds8* s, int r, int c) { return tld_at<T>(s + toff<T>(r, c)); }
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 DEV u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
@@ -1055,9 +1101,6 @@ DEV uint32_t upd(uint32_t w, uint32_t p) {
     const u16x2 W = as2(w), P = as2(p);
     const u16x2 t = __builtin_elementwise_min(W.yx, P.xx);  // (min(m,p), min(j,p) <= m)
     return as1(__builtin_elementwise_max(W, t));
-}
-DEV uint32_t pmax(uint32_t a, uint32_t b) {
-    return as1(__builtin_elementwise_max(as2(a), as2(b)));
 }
 // Maximum of the low halves (one v_max3_u16); the high half is left zero.
 DEV uint32_t max3_lo(uint32_t a, uint32_t b, uint32_t c) {
@@ -1072,7 +1115,6 @@ DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-
 template <class T>
 DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
     return *gp(reinterpret_cast<const T*>(base + (size_t)y * pitch + (size_t)x * sizeof(T)));
@@ -1086,12 +1128,15 @@ DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
 // two per row) instead of one line per halo pixel.  All loads of a batch are
 // issued before any is used, from clamped (always valid) addresses; uint16
 // loads in two batches (the registers of one would halve the waves per SIMD).
+// A lane's 4 pixels go to LDS as 2 (uint8) or 4 (uint16) words built with
+// byte permutes, masked to 0 where they lie outside the image; rows outside
+// the image are written as 0 (a separate path: megapixels on the image's top
+// or bottom edge).
 template <class T>
-DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
-    constexpr int WPL = (int)sizeof(T);                 // 32-bit words per lane
-    constexpr int BITS = 8 * (int)sizeof(T), PPW = 4 / (int)sizeof(T);
-    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
+DEV void tile_fetch(const Image& im, int mx, int my, lds8* s) {
+    constexpr int WPL = (int)sizeof(T);                 // 32-bit words per lane and plane
     constexpr int RPI = 3, NIT = 22, NB = (int)sizeof(T), BIT = NIT / NB;
+    constexpr int RB = TileFmt<T>::RB;
     const int lane = lane_id();
     const int x0 = mx * kMP, y0 = my * kMP;
     const int rl = lane / 18, d = lane - rl * 18;  // rl == 3: idle lanes
@@ -1100,19 +1145,20 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
     size_t off[WPL];
 #pragma unroll
     for (int w = 0; w < WPL; ++w) off[w] = (size_t)min(max(xw * (int)sizeof(T) + 4 * w, 0), lastw);
-    // per lane, fixed over the rows: which of its 4 pixels fall in the tile's
-    // frame (columns -1..64), and which of those hold image pixels
-    const bool hasL = x0 > 0, hasR = x0 + kMP < im.W;
-    uint32_t frame = 0, inside = 0;
+    // byte masks of the lane's tile words: pixels outside the image are 0
+    uint32_t keep[2 * WPL];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int c = 4 * d - 4 + k;
-        if (rl < RPI && c >= -1 && c <= 64) {
-            frame |= 1u << k;
-            if (x0 + c < im.W && (c >= 0 || hasL) && (c < 64 || hasR)) inside |= 1u << k;
+        const bool in = xw + k >= 0 && xw + k < im.W;
+        if constexpr (sizeof(T) == 1) {
+            if (k % 2 == 0) keep[k / 2] = in ? 0xFFFFu : 0u;
+            else keep[k / 2] |= in ? 0xFFFF0000u : 0u;
+        } else {
+            keep[k] = in ? 0xFFFFFFFFu : 0u;
         }
     }
-    const int base = rl * TS + 4 * d - 3;  // LDS index of row -1 + rl, column 4d - 4
+    const bool rows_inside = y0 > 0 && y0 + kMP < im.H;  // rows -1..64 all in the image
+    lds8* a = s + rl * RB + 4 * d * TileFmt<T>::PXB;      // the lane's 4 pixels of row rl - 1
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         uint32_t gj[BIT][WPL], gi[BIT][WPL];
@@ -1125,18 +1171,32 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
                 gi[u][w] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + off[w]));
             }
         }
+        if (rl < RPI) {
 #pragma unroll
-        for (int u = 0; u < BIT; ++u) {
-            const int it = b * BIT + u, row = -1 + it * RPI + rl;
-            if (row > 64) continue;
-            const uint32_t in = (y0 + row >= 0 && y0 + row < im.H) ? inside : 0u;
+            for (int u = 0; u < BIT; ++u) {
+                const int it = b * BIT + u;
+                uint32_t m[2 * WPL];
+                if (rows_inside) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (!((frame >> k) & 1u)) continue;
-                const bool v = (in >> k) & 1u;
-                const uint32_t j = v ? (gj[u][k / PPW] >> (BITS * (k % PPW))) & PM : 0u;
-                const uint32_t m = v ? (gi[u][k / PPW] >> (BITS * (k % PPW))) & PM : 0u;
-                s[base + it * RPI * TS + k] = j | (m << 16);
+                    for (int k = 0; k < 2 * WPL; ++k) m[k] = keep[k];
+                } else {
+                    const int y = y0 - 1 + it * RPI + rl;
+                    const bool rv = y >= 0 && y < im.H;
+#pragma unroll
+                    for (int k = 0; k < 2 * WPL; ++k) m[k] = rv ? keep[k] : 0u;
+                }
+                lds32* o = reinterpret_cast<lds32*>(a + it * RPI * RB);
+                if constexpr (sizeof(T) == 1) {
+                    // [j0 m0 j1 m1], [j2 m2 j3 m3]
+                    o[0] = __builtin_amdgcn_perm(gi[u][0], gj[u][0], 0x05010400u) & m[0];
+                    o[1] = __builtin_amdgcn_perm(gi[u][0], gj[u][0], 0x07030602u) & m[1];
+                } else {
+                    // (j | m << 16) per pixel
+                    o[0] = __builtin_amdgcn_perm(gi[u][0], gj[u][0], 0x05040100u) & m[0];
+                    o[1] = __builtin_amdgcn_perm(gi[u][0], gj[u][0], 0x07060302u) & m[1];
+                    o[2] = __builtin_amdgcn_perm(gi[u][1], gj[u][1], 0x05040100u) & m[2];
+                    o[3] = __builtin_amdgcn_perm(gi[u][1], gj[u][1], 0x07060302u) & m[3];
+                }
             }
         }
     }
@@ -1144,16 +1204,27 @@ DEV void tile_fetch(const Image& im, int mx, int my, uint32_t* s) {
 }
 
 // Row sweeps over the interior, lane = row: left to right, then right to
-// left.  The lane's 66 words (halo included) are read into registers first,
+// left.  The lane's 66 pixels (halo included) are read into registers first,
 // so the reads issue back to back, both dependent chains run on registers,
 // and the interior goes back to LDS once, if it changed.  A lane owns its row
 // in a row sweep (its column in a column sweep): no other lane touches those
-// words.  ch[0], ch[1]: whether each direction changed a pixel.
-DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward) {
-    const int base = tix(lane, -1);
-    uint32_t w[66];
+// pixels.  ch[0], ch[1]: whether each direction changed a pixel.
+template <class T>
+DEV void sweep_rows_pair(lds8* s, int lane, bool (&ch)[2], bool last_forward) {
+    lds32* row = reinterpret_cast<lds32*>(s + (lane + 1) * TileFmt<T>::RB);
+    uint32_t w[66];  // columns -1..64
+    if constexpr (sizeof(T) == 1) {
+        // word k holds columns 2k - 4 (low) and 2k - 3 (high)
 #pragma unroll
-    for (int c = 0; c < 66; ++c) w[c] = s[base + c];
+        for (int k = 1; k <= 34; ++k) {
+            const uint32_t v = row[k];
+            if (k > 1) w[2 * k - 3] = widen8(v);
+            if (k < 34) w[2 * k - 2] = widen8(v >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int c = -1; c <= kMP; ++c) w[c + 1] = row[c + 4];
+    }
     ch[0] = ch[1] = false;
     uint32_t p = w[0];
 #pragma unroll
@@ -1173,8 +1244,13 @@ DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward
         p = nw;
     }
     if (ch[0] || ch[1]) {
+        if constexpr (sizeof(T) == 1) {
 #pragma unroll
-        for (int c = 1; c <= kMP; ++c) s[base + c] = w[c];
+            for (int k = 2; k <= 33; ++k) row[k] = narrow8x2(w[2 * k - 3], w[2 * k - 2]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < kMP; ++c) row[c + 4] = w[c + 1];
+        }
     }
 }
 
@@ -1186,13 +1262,14 @@ DEV void sweep_rows_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward
 // group (scheduling barriers keep them there: read one step ahead, each step
 // waited on its LDS read; read all at once, 128 live values would halve the
 // waves per SIMD).
-template <int CONN, int G>
-DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward) {
+template <class T, int CONN, int G>
+DEV void sweep_cols_pair(lds8* s, int lane, bool (&ch)[2], bool last_forward) {
     static_assert(kMP % G == 0, "whole groups");
-    const int base = tix(-1, lane);
+    constexpr int RB = TileFmt<T>::RB;
+    lds8* col = s + (lane + 4) * TileFmt<T>::PXB;  // row -1 of the lane's column
     uint32_t w[66];
 #pragma unroll
-    for (int r = 0; r < 66; ++r) w[r] = s[base + r * TS];
+    for (int r = 0; r < 66; ++r) w[r] = tld_at<T>(col + r * RB);
     ch[0] = ch[1] = false;
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
@@ -1204,9 +1281,9 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward
             if constexpr (CONN == 8) {
 #pragma unroll
                 for (int i = 0; i < G; ++i) {
-                    const int rp = DIR > 0 ? k0 + i : kMP + 1 - (k0 + i);
-                    hl[i] = s[rp * TS];
-                    hr[i] = s[rp * TS + 65];
+                    const int rp = DIR > 0 ? k0 + i : kMP + 1 - (k0 + i);  // row rp - 1
+                    hl[i] = tld_at<T>(s + toff<T>(rp - 1, -1));
+                    hr[i] = tld_at<T>(s + toff<T>(rp - 1, kMP));
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -1228,14 +1305,17 @@ DEV void sweep_cols_pair(uint32_t* s, int lane, bool (&ch)[2], bool last_forward
     }
     if (ch[0] || ch[1]) {
 #pragma unroll
-        for (int r = 1; r <= kMP; ++r) s[base + r * TS] = w[r];
+        for (int r = 1; r <= kMP; ++r) tst_at<T>(col + r * RB, w[r]);
     }
 }
 
-// Halo reads per group of column steps (16 fit the registers beside two
-// waves per SIMD).
+// Halo reads per group of column steps.
 template <class T>
-constexpr int kHaloGroup = 16;
+constexpr int kHaloGroup = sizeof(T) == 1 ? 8 : 16;
+// Waves per SIMD the queue kernels are compiled for: what the LDS tile
+// allows (uint8 tiles: four; uint16 tiles: two).
+template <class T>
+constexpr int kQueueWaves = sizeof(T) == 1 ? 4 : 2;
 
 // Sweeps until the four directions in a row change nothing, or max_sweeps:
 // rows forward, rows backward, columns forward, columns backward, in turn.
@@ -1244,15 +1324,15 @@ constexpr int kHaloGroup = 16;
 // the registers; when two quiet sweeps precede a pair and its forward sweep
 // is quiet too, the tile has converged and the backward sweep is left out.
 // `changed`: some sweep raised a pixel.
-template <int CONN, int G>
-DEV bool tile_converge(uint32_t* s, int max_sweeps, bool& changed) {
+template <class T, int CONN, int G>
+DEV bool tile_converge(lds8* s, int max_sweeps, bool& changed) {
     const int lane = lane_id();
     int quiet = -1;
     changed = false;
     for (int k = 0; k < max_sweeps; k += 2) {
         bool ch[2];
-        if ((k & 2) == 0) sweep_rows_pair(s, lane, ch, quiet == 2);
-        else sweep_cols_pair<CONN, G>(s, lane, ch, quiet == 2);
+        if ((k & 2) == 0) sweep_rows_pair<T>(s, lane, ch, quiet == 2);
+        else sweep_cols_pair<T, CONN, G>(s, lane, ch, quiet == 2);
         wave_lds_sync();
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
@@ -1273,15 +1353,26 @@ DEV bool tile_converge(uint32_t* s, int max_sweeps, bool& changed) {
 struct Border {
     uint32_t tb, lr;
 };
-DEV Border tile_border(const uint32_t* s) {
+template <class T>
+DEV Border tile_border(const lds8* s) {
     const int lane = lane_id();
-    return Border{(s[tix(0, lane)] & 0xFFFFu) | (s[tix(kMP - 1, lane)] << 16),
-                  (s[tix(lane, 0)] & 0xFFFFu) | (s[tix(lane, kMP - 1)] << 16)};
+    return Border{(tld<T>(s, 0, lane) & 0xFFFFu) | (tld<T>(s, kMP - 1, lane) << 16),
+                  (tld<T>(s, lane, 0) & 0xFFFFu) | (tld<T>(s, lane, kMP - 1) << 16)};
 }
 
 template <class T>
 DEV void st_px_coh(uint8_t* p, uint32_t v) {
     __hip_atomic_store(gp(reinterpret_cast<T*>(p)), (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The marker values of a store lane's PPW = 4 / sizeof(T) consecutive pixels
+// of one tile row, packed as in memory (one 32-bit word); p points at the
+// first pixel's tile entry (4-byte aligned).
+template <class T>
+DEV uint32_t tile_marker_word(const lds8* p) {
+    const lds32* q = reinterpret_cast<const lds32*>(p);
+    if constexpr (sizeof(T) == 1) return __builtin_amdgcn_perm(q[1], q[0], 0x06040200u);
+    else return __builtin_amdgcn_perm(q[1], q[0], 0x05040100u);
 }
 
 // Write-back for a wave that is the megapixel's only writer (mutex queue, UQ):
@@ -1292,39 +1383,40 @@ DEV void st_px_coh(uint8_t* p, uint32_t v) {
 // layout as tile_store) come from the border values taken after the fetch.
 // Returns without waiting for the stores.
 template <class T>
-DEV void tile_store_excl(const uint32_t* s, uint8_t* rose, Border ob, bool changed,
+DEV void tile_store_excl(const lds8* s, lds8* rose, Border ob, bool changed,
                          const Image& im, int mx, int my) {
     constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
     constexpr int BITS = 8 * (int)sizeof(T);
+    constexpr int RB = TileFmt<T>::RB;
     const int lane = lane_id();
     if (!changed) {
-        reinterpret_cast<uint32_t*>(rose)[lane] = 0u;
+        reinterpret_cast<lds32*>(rose)[lane] = 0u;
         wave_lds_sync();
         return;
     }
-    rose[lane] = (s[tix(0, lane)] & 0xFFFFu) != (ob.tb & 0xFFFFu);
-    rose[64 + lane] = (s[tix(kMP - 1, lane)] & 0xFFFFu) != (ob.tb >> 16);
-    rose[128 + lane] = (s[tix(lane, 0)] & 0xFFFFu) != (ob.lr & 0xFFFFu);
-    rose[192 + lane] = (s[tix(lane, kMP - 1)] & 0xFFFFu) != (ob.lr >> 16);
+    rose[lane] = (tld<T>(s, 0, lane) & 0xFFFFu) != (ob.tb & 0xFFFFu);
+    rose[64 + lane] = (tld<T>(s, kMP - 1, lane) & 0xFFFFu) != (ob.tb >> 16);
+    rose[128 + lane] = (tld<T>(s, lane, 0) & 0xFFFFu) != (ob.lr & 0xFFFFu);
+    rose[192 + lane] = (tld<T>(s, lane, kMP - 1) & 0xFFFFu) != (ob.lr >> 16);
     const int x0 = mx * kMP, y0 = my * kMP;
     const int wl = lane % WPR, rl = lane / WPR;
     const int xw = x0 + wl * PPW;
     if (xw < im.W) {
         const bool whole = xw + PPW <= im.W;
+        const lds8* src = s + toff<T>(rl, wl * PPW);
+        uint8_t* dst = im.J + (size_t)(y0 + rl) * im.pj + (size_t)xw * sizeof(T);
+        const size_t step = (size_t)RPI * im.pj;
 #pragma unroll
         for (int it = 0; it < kMP / RPI; ++it) {
-            const int row = it * RPI + rl;
-            const int y = y0 + row;
+            const int y = y0 + it * RPI + rl;
             if (y >= im.H) continue;
-            uint8_t* wp = im.J + (size_t)y * im.pj + (size_t)xw * sizeof(T);
+            const uint32_t f = tile_marker_word<T>(src + it * RPI * RB);
+            uint8_t* wp = dst + it * step;
             if (whole) {
-                uint32_t f = 0;
-#pragma unroll
-                for (int k = 0; k < PPW; ++k) f |= (s[tix(row, wl * PPW + k)] & 0xFFFFu) << (BITS * k);
                 st_coh(wp, f);
             } else {
                 for (int k = 0; k < PPW && xw + k < im.W; ++k)
-                    st_px_coh<T>(wp + k * sizeof(T), s[tix(row, wl * PPW + k)] & 0xFFFFu);
+                    st_px_coh<T>(wp + k * sizeof(T), f >> (BITS * k));
             }
         }
     }
@@ -1358,7 +1450,7 @@ DEV int neighbour_index(int m, int nmx, uint32_t bit) {
 // column, [192..255] right column).  Returns true when some pixel in memory
 // rose.
 template <class T>
-DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, int my) {
+DEV bool tile_store(const lds8* s, lds8* rose, const Image& im, int mx, int my) {
     constexpr int PPW = 4 / (int)sizeof(T), WPR = 64 / PPW, RPI = 64 / WPR;
     constexpr int BITS = 8 * (int)sizeof(T);
     constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
@@ -1366,7 +1458,7 @@ DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, i
     const int x0 = mx * kMP, y0 = my * kMP;
     const int wl = lane % WPR, rl = lane / WPR;
     const int xw = x0 + wl * PPW;
-    reinterpret_cast<uint32_t*>(rose)[lane] = 0u;
+    reinterpret_cast<lds32*>(rose)[lane] = 0u;
     wave_lds_sync();
     bool any = false;
     if (xw < im.W) {
@@ -1384,10 +1476,11 @@ DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, i
             if (y >= im.H) continue;
             uint32_t* wp = reinterpret_cast<uint32_t*>(im.J + (size_t)y * im.pj + (size_t)xw * sizeof(T));
             const uint32_t g = gv[it];
+            const uint32_t tw = tile_marker_word<T>(s + toff<T>(row, wl * PPW));
             uint32_t f = g, up = 0;
 #pragma unroll
             for (int k = 0; k < PPW; ++k) {
-                const uint32_t v = s[tix(row, wl * PPW + k)] & 0xFFFFu;
+                const uint32_t v = (tw >> (BITS * k)) & PM;
                 const uint32_t old = (g >> (BITS * k)) & PM;
                 if (xw + k < im.W && v > old) {
                     f = (f & ~(PM << (BITS * k))) | (v << (BITS * k));
@@ -1427,8 +1520,8 @@ DEV bool tile_store(const uint32_t* s, uint8_t* rose, const Image& im, int mx, i
 
 // Which neighbours hold a pixel q next to a pixel p of this megapixel that
 // rose in this visit, with min(J(p), I(q)) > J(q) on the fetched halo.
-template <int CONN>
-DEV uint32_t tile_notify_mask(const uint32_t* s, const uint8_t* rose, const Image& im, int mx,
+template <class T, int CONN>
+DEV uint32_t tile_notify_mask(const lds8* s, const lds8* rose, const Image& im, int mx,
                               int my) {
     const int lane = lane_id();
     const int x0 = mx * kMP, y0 = my * kMP;
@@ -1437,36 +1530,36 @@ DEV uint32_t tile_notify_mask(const uint32_t* s, const uint8_t* rose, const Imag
     // top / bottom rows: lane = column
     bool t = false, b = false, l = false, r = false;
     if (hasT && rose[lane]) {
-        const uint32_t p = s[tix(0, lane)] & 0xFFFFu;
-        t = raises(p, s[tix(-1, lane)]);
+        const uint32_t p = tld<T>(s, 0, lane) & 0xFFFFu;
+        t = raises(p, tld<T>(s, -1, lane));
         if (CONN == 8) {
-            if (lane > 0) t |= raises(p, s[tix(-1, lane - 1)]);
-            if (lane < 63) t |= raises(p, s[tix(-1, lane + 1)]);
+            if (lane > 0) t |= raises(p, tld<T>(s, -1, lane - 1));
+            if (lane < 63) t |= raises(p, tld<T>(s, -1, lane + 1));
         }
     }
     if (hasB && rose[64 + lane]) {
-        const uint32_t p = s[tix(63, lane)] & 0xFFFFu;
-        b = raises(p, s[tix(64, lane)]);
+        const uint32_t p = tld<T>(s, 63, lane) & 0xFFFFu;
+        b = raises(p, tld<T>(s, 64, lane));
         if (CONN == 8) {
-            if (lane > 0) b |= raises(p, s[tix(64, lane - 1)]);
-            if (lane < 63) b |= raises(p, s[tix(64, lane + 1)]);
+            if (lane > 0) b |= raises(p, tld<T>(s, 64, lane - 1));
+            if (lane < 63) b |= raises(p, tld<T>(s, 64, lane + 1));
         }
     }
     // left / right columns: lane = row
     if (hasL && rose[128 + lane]) {
-        const uint32_t p = s[tix(lane, 0)] & 0xFFFFu;
-        l = raises(p, s[tix(lane, -1)]);
+        const uint32_t p = tld<T>(s, lane, 0) & 0xFFFFu;
+        l = raises(p, tld<T>(s, lane, -1));
         if (CONN == 8) {
-            if (lane > 0) l |= raises(p, s[tix(lane - 1, -1)]);
-            if (lane < 63) l |= raises(p, s[tix(lane + 1, -1)]);
+            if (lane > 0) l |= raises(p, tld<T>(s, lane - 1, -1));
+            if (lane < 63) l |= raises(p, tld<T>(s, lane + 1, -1));
         }
     }
     if (hasR && rose[192 + lane]) {
-        const uint32_t p = s[tix(lane, 63)] & 0xFFFFu;
-        r = raises(p, s[tix(lane, 64)]);
+        const uint32_t p = tld<T>(s, lane, 63) & 0xFFFFu;
+        r = raises(p, tld<T>(s, lane, 64));
         if (CONN == 8) {
-            if (lane > 0) r |= raises(p, s[tix(lane - 1, 64)]);
-            if (lane < 63) r |= raises(p, s[tix(lane + 1, 64)]);
+            if (lane > 0) r |= raises(p, tld<T>(s, lane - 1, 64));
+            if (lane < 63) r |= raises(p, tld<T>(s, lane + 1, 64));
         }
     }
     if (wave_any(t)) nb |= NB_T;
@@ -1474,10 +1567,10 @@ DEV uint32_t tile_notify_mask(const uint32_t* s, const uint8_t* rose, const Imag
     if (wave_any(l)) nb |= NB_L;
     if (wave_any(r)) nb |= NB_R;
     if constexpr (CONN == 8) {
-        if (hasT && hasL && rose[0] && raises(s[tix(0, 0)] & 0xFFFFu, s[tix(-1, -1)])) nb |= NB_TL;
-        if (hasT && hasR && rose[63] && raises(s[tix(0, 63)] & 0xFFFFu, s[tix(-1, 64)])) nb |= NB_TR;
-        if (hasB && hasL && rose[64] && raises(s[tix(63, 0)] & 0xFFFFu, s[tix(64, -1)])) nb |= NB_BL;
-        if (hasB && hasR && rose[127] && raises(s[tix(63, 63)] & 0xFFFFu, s[tix(64, 64)]))
+        if (hasT && hasL && rose[0] && raises(tld<T>(s, 0, 0) & 0xFFFFu, tld<T>(s, -1, -1))) nb |= NB_TL;
+        if (hasT && hasR && rose[63] && raises(tld<T>(s, 0, 63) & 0xFFFFu, tld<T>(s, -1, 64))) nb |= NB_TR;
+        if (hasB && hasL && rose[64] && raises(tld<T>(s, 63, 0) & 0xFFFFu, tld<T>(s, 64, -1))) nb |= NB_BL;
+        if (hasB && hasR && rose[127] && raises(tld<T>(s, 63, 63) & 0xFFFFu, tld<T>(s, 64, 64)))
             nb |= NB_BR;
     }
     return nb;
@@ -1797,9 +1890,126 @@ struct Waiter {
     }
 };
 
+// ------------------------------------------- workgroup-local queue (LDS) ----
+// A ring of LQ_CAP megapixel entries in LDS, shared by the waves of one
+// workgroup of the mutex kernel.  A wave puts the neighbours its visit made
+// pending here (while fewer than LQ_SOFT are queued) instead of in the global
+// ring, and takes work from here before the global ring: the hand-off costs
+// LDS atomics instead of two round trips to the device's shared counters and
+// ring, and the next visit lands next to the last one.  The megapixel state
+// protocol is unchanged (an entry here is a PENDING megapixel and counts in
+// `pending`), so where an entry waits does not matter for correctness.
+// Tickets with sequence numbers as in the global ring: slot t % LQ_CAP holds
+// ((t + 1) << 32 | m) once ticket t is filled and (t + LQ_CAP) << 32 when it is
+// free for ticket t + LQ_CAP.  `count` = entries reserved by pushes and not yet
+// claimed by pops (at most LQ_CAP), so a push never waits on a full ring, only
+// (briefly) on the pop that is freeing its slot.
+constexpr int LQ_CAP = 16;
+constexpr int LQ_SOFT = 8;
+struct LocalQueue {
+    uint32_t count, head, tail, pad;
+    unsigned long long slot[LQ_CAP];
+};
+typedef __attribute__((address_space(3))) LocalQueue lds_lq;
+typedef __attribute__((address_space(3))) unsigned long long lds64;
+
+DEV uint32_t l_load(const lds32* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+DEV uint32_t l_add(lds32* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+DEV unsigned long long l_load64(const lds64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+DEV void l_store64(lds64* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Called by every wave of the workgroup before any uses the queue.
+DEV void lq_init(lds_lq* L) {
+    const int t = (int)threadIdx.x;
+    if (t < LQ_CAP) l_store64(&L->slot[t], (unsigned long long)t << 32);
+    if (t == 0) {
+        __hip_atomic_store(&L->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&L->head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&L->tail, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+}
+
+// Takes one entry; false when none is reserved for taking.  The wait for the
+// slot ends when its pusher, which has reserved it and runs meanwhile, writes
+// it; it is bounded like every other wait.
+DEV bool lq_pop(lds_lq* L, const QueueArgs& q, uint32_t& out) {
+    uint32_t got = 0, ok = 0;
+    if (lane_id() == 0 && l_load(&L->count) != 0u) {
+        const uint32_t c = l_add(&L->count, 0u - 1u);  // value before
+        if ((int)c <= 0) {
+            l_add(&L->count, 1u);
+        } else {
+            const uint32_t h = l_add(&L->head, 1u);
+            lds64* sl = &L->slot[h % LQ_CAP];
+            Waiter w(q.timeout_ticks);
+            unsigned long long e;
+            while ((uint32_t)((e = l_load64(sl)) >> 32) != h + 1u) {
+                if (!w.step(q.ctl)) break;
+            }
+            if ((uint32_t)(e >> 32) == h + 1u) {
+                got = (uint32_t)e;
+                ok = 1;
+                l_store64(sl, (unsigned long long)(h + LQ_CAP) << 32);
+            }
+        }
+    }
+    ok = first_lane(ok);
+    if (!ok) return false;
+    out = first_lane(got);
+    return true;
+}
+
+// Lanes with mine = true each put `value` into the local queue, as many as
+// it takes (up to LQ_SOFT queued); returns which lanes it took.
+DEV bool lq_push_some(lds_lq* L, const QueueArgs& q, bool mine, uint32_t value) {
+    const int lane = lane_id();
+    const uint64_t b = ballot(mine);
+    const uint32_t n = (uint32_t)__popcll(b);
+    if (n == 0) return false;
+    uint32_t k = 0, t0 = 0;
+    if (lane == 0) {
+        const uint32_t c = l_load(&L->count);
+        k = (int)c >= LQ_SOFT ? 0u : min(n, (uint32_t)LQ_SOFT - c);
+        if (k) {
+            const uint32_t r = l_add(&L->count, k);
+            if (r + k > (uint32_t)LQ_CAP) {
+                l_add(&L->count, 0u - k);
+                k = 0;
+            } else {
+                t0 = l_add(&L->tail, k);
+            }
+        }
+    }
+    k = first_lane(k);
+    t0 = first_lane(t0);
+    const uint32_t rank = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+    const bool local = mine && rank < k;
+    if (local) {
+        const uint32_t t = t0 + rank;
+        lds64* sl = &L->slot[t % LQ_CAP];
+        Waiter w(q.timeout_ticks);
+        bool free = true;
+        while ((uint32_t)(l_load64(sl) >> 32) != t) {
+            if (!w.step(q.ctl)) { free = false; break; }
+        }
+        if (free) l_store64(sl, ((unsigned long long)(t + 1u) << 32) | value);
+    }
+    return local;
+}
+
 // ------------------------------------------------- mutex variant (async) ----
 // One global ring queue consumed by a persistent kernel.  Every megapixel is
-// in the ring at most once and is never visited by two waves at once:
+// in a queue (the ring or one workgroup's local queue) at most once and is
+// never visited by two waves at once:
 //   notify(n):  old = st[n] |= PENDING; if old == 0, pending++ and push n
 //   pop(m):     st[m] = RUNNING                        (PENDING cleared)
 //   finish(m):  old = st[m] &= ~RUNNING; if old has PENDING, push m again,
@@ -1859,31 +2069,35 @@ DEV void pending_settle(const QueueArgs& q, uint32_t& owed, uint32_t nwaves) {
     owed = 0;
 }
 
-// Taking work.  The seeding pass fills tickets 0..S-1 before the kernel
-// starts; they are claimed in batches of B through their own counter (every
-// one is filled, nothing waits), read and their slots freed at once, and
-// visited one after another -- one atomic on a shared counter per B visits
-// instead of one per visit, which dominated the queue phase when thousands of
-// waves start on a full ring.  Tickets from S on are popped one at a time from
-// C_HEAD as before.  A wave's decrements of `pending` for finished megapixels
-// are held back while it still has claimed work and flushed before it waits
-// for a single pop or leaves: the count is never below the true number of
-// queued and running megapixels, so no wave leaves early.
+// Taking work, in this order: (1) the seeded tickets 0..S-1, filled before
+// the kernel starts and claimed in batches of B through their own counter
+// (every one is filled, nothing waits; slots freed at once) -- one atomic on a
+// shared counter per B visits instead of one per visit; (2) the workgroup's
+// local queue; (3) one ticket of the global ring from C_HEAD.  A wave holds at
+// most one ring ticket: while it waits for that ticket's entry it also
+// watches its local queue, takes local work when some arrives, and goes back
+// to the ticket afterwards (the ticket's entry, once pushed, waits for it).
+// A wave's decrements of `pending` for finished megapixels are held back
+// while it still has work in hand and flushed before it waits or leaves: the
+// count is never below the true number of queued and running megapixels, so
+// no wave leaves early.
 //
 // A visit's memory round trips overlap where the protocol allows: the next
-// claimed megapixel m2 is taken (st[m2] = RUNNING) while this visit's stores
-// drain; after one wait, the notifications of this visit and the release of
-// m go out as one round of atomics, and m2's tile loads are issued behind
-// them.  m2 is not notified by this visit: its fetch comes after this visit's
-// stores are complete.  Taking m2 before m is released only holds two
-// megapixels at once; a notification that reaches m2 while it is taken
+// megapixel m2 (claimed or local) is taken (st[m2] = RUNNING) while this
+// visit's stores drain; after one wait, the notifications of this visit and
+// the release of m go out as one round of atomics, and m2's tile loads are
+// issued behind them.  m2 is not notified by this visit: its fetch comes after
+// this visit's stores are complete.  Taking m2 before m is released only holds
+// two megapixels at once; a notification that reaches m2 while it is taken
 // queues it again, as for any running megapixel.
 template <class T, int CONN>
-__global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
-    extern __shared__ uint32_t lds[];
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(kQueueWaves<T>)))
+void k_queue_mutex(Descs ds, QueueArgs q) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
     const int lane = lane_id();
-    uint32_t* s = lds + (threadIdx.x >> 6) * (WAVE_LDS / 4);
-    uint8_t* rose = reinterpret_cast<uint8_t*>(s + TILE_WORDS);
+    lds_lq* L = (lds_lq*)(lds_raw + (kWG / 64) * WAVE_LDS<T>);
+    lds8* s = (lds8*)lds_raw + (threadIdx.x >> 6) * WAVE_LDS<T>;
+    lds8* rose = s + TILE_BYTES<T>;
     unsigned long long visits = 0;
     PhaseTimer pt(q.trace != 0);
     const uint32_t seeded = a_load(ctl_at(q.ctl, C_SEEDED));
@@ -1896,6 +2110,7 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
     // optimum: 8-16 at c5, 4 at c3, 1-2 at c2).
     const uint32_t B = max(1u, min(16u, seeded / (3u * nwaves)));
     if (seeded == 0) return;  // `pending` starts at zero: nothing to do
+    lq_init(L);
     uint32_t claimed = 0;  // lane k < nclaimed holds the k-th claimed megapixel
     int nclaimed = 0, next = 0;
     bool seeds_left = true;
@@ -1932,46 +2147,64 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
         out = rdlane(claimed, next++);
         return true;
     };
+    bool held = false;  // a ring ticket h taken and not yet served
+    uint32_t h = 0;
     uint32_t m = 0;
     bool have = false;  // m is taken and its tile is in LDS
     while (true) {
         if (!have) {
             int state = 0;  // 1 = got an entry, 2 = done, 3 = timeout
-            if (take_claimed(m)) {
+            if (take_claimed(m) || lq_pop(L, q, m)) {
                 state = 1;
             } else {
                 pending_settle(q, owed, nwaves);
-                uint32_t h = 0;
-                if (lane == 0) h = a_add(ctl_at(q.ctl, C_HEAD), 1u);
-                h = first_lane(h);
-                const uint32_t slot = h & (q.cap - 1u);
-                uint32_t got = 0;
-                if (lane == 0) {
-                    // The wave polls its own ring slot, which gets an entry or
-                    // RING_DONE.  Reads of the shared counters, which every
-                    // working wave updates, queue up with those updates and
-                    // slow the waves that still work: the timeout flag (and
-                    // `pending`, a second way out) are read on every 256th
-                    // poll only.
-                    Waiter w(q.timeout_ticks);
-                    for (uint32_t k = 0;; ++k) {
-                        const unsigned long long e = a_load64(&q.ring[slot]);
-                        if ((uint32_t)(e >> 32) == h + 1u) {
-                            got = (uint32_t)e;
-                            state = got == RING_DONE ? 2 : 1;
-                            break;
-                        }
-                        if ((k & 255u) == 255u) {
-                            if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u) { state = 3; break; }
-                            if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
-                        }
-                        if (!w.step(q.ctl)) { state = 3; break; }
+                while (true) {
+                    if (!held) {
+                        uint32_t hh = 0;
+                        if (lane == 0) hh = a_add(ctl_at(q.ctl, C_HEAD), 1u);
+                        h = first_lane(hh);
+                        held = true;
                     }
-                    if (state == 1) a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
+                    const uint32_t slot = h & (q.cap - 1u);
+                    uint32_t got = 0;
+                    if (lane == 0) {
+                        // The wave polls its own ring slot, which gets an
+                        // entry or RING_DONE, and its workgroup's local queue
+                        // (4: local work arrived).  Reads of the shared
+                        // counters, which every working wave updates, queue up
+                        // with those updates and slow the waves that still
+                        // work: the timeout flag (and `pending`, a second way
+                        // out) are read on every 256th poll only.
+                        Waiter w(q.timeout_ticks);
+                        for (uint32_t k = 0;; ++k) {
+                            const unsigned long long e = a_load64(&q.ring[slot]);
+                            if ((uint32_t)(e >> 32) == h + 1u) {
+                                got = (uint32_t)e;
+                                state = got == RING_DONE ? 2 : 1;
+                                break;
+                            }
+                            if (l_load(&L->count) != 0u) { state = 4; break; }
+                            if ((k & 255u) == 255u) {
+                                if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u) { state = 3; break; }
+                                if (a_load(ctl_at(q.ctl, C_PENDING)) == 0u) { state = 2; break; }
+                            }
+                            if (!w.step(q.ctl)) { state = 3; break; }
+                        }
+                        if (state == 1) a_store64(&q.ring[slot], (unsigned long long)(h + q.cap) << 32);
+                    }
+                    state = (int)first_lane((uint32_t)state);
+                    if (state == 1) {
+                        held = false;
+                        m = first_lane(got);
+                        break;
+                    }
+                    if (state == 4) {
+                        if (lq_pop(L, q, m)) { state = 1; break; }
+                        continue;  // taken by a sibling: back to the ticket
+                    }
+                    break;
                 }
-                state = (int)first_lane((uint32_t)state);
                 if (state != 1) break;
-                m = first_lane(got);
             }
             if (lane == 0) a_xchg(&q.st[m], ST_RUNNING);
             mem_wait();
@@ -1985,15 +2218,15 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
         const Image im = image_of(dsc);
         const int ml = (int)m - dsc.mp_base;
         const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
-        const Border ob = tile_border(s);
+        const Border ob = tile_border<T>(s);
         bool changed;
-        const bool conv = tile_converge<CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
+        const bool conv = tile_converge<T, CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
         pt.lap(2);
         tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
         uint32_t m2 = 0;
-        const bool have2 = take_claimed(m2);
+        const bool have2 = take_claimed(m2) || lq_pop(L, q, m2);
         if (have2 && lane == 0) a_xchg(&q.st[m2], ST_RUNNING);
-        uint32_t nb = tile_notify_mask<CONN>(s, rose, im, mx, my);
+        uint32_t nb = tile_notify_mask<T, CONN>(s, rose, im, mx, my);
         if (!conv) nb |= NB_SELF;
         mem_wait();  // this visit's stores are visible and m2 is taken
         pt.lap(3);
@@ -2012,13 +2245,16 @@ __global__ __launch_bounds__(kWG) void k_queue_mutex(Descs ds, QueueArgs q) {
             const int ml2 = (int)m2 - d2.mp_base;
             tile_fetch<T>(image_of(d2), ml2 % d2.nmx, ml2 / d2.nmx, s);
         }
-        const bool push = tried && old == 0u;
         const bool again = (rdlane(old, 9) & ST_PENDING) != 0u;
-        const uint32_t npush = (uint32_t)__popcll(ballot(push));
+        // newly pending neighbours, and m itself when it was notified while
+        // it ran: to the local queue first, the rest to the global ring
+        const bool push = (tried && old == 0u) || (lane == 9 && again);
+        if (lane == 9) target = m;
+        const uint32_t npush = (uint32_t)__popcll(ballot(tried && old == 0u));
         if (npush && lane == 0) a_add(ctl_at(q.ctl, C_PENDING), npush);
-        bool ok = ring_push_many(q, push, target);
-        if (again) ok = ring_push_many(q, lane == 0, m) && ok;
-        else ++owed;
+        const bool local = lq_push_some(L, q, push, target);
+        const bool ok = ring_push_many(q, push && !local, target);
+        if (!again) ++owed;
         pt.lap(4);
         if (!ok) break;
         m = m2;
@@ -2062,30 +2298,63 @@ DEV bool grid_barrier(uint32_t* ctl, uint32_t nblocks, unsigned long long limit)
 }
 
 template <class T, int CONN, int KIND>
-__global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(kQueueWaves<T>)))
+void k_queue_bsp(Descs ds, QueueArgs q) {
     constexpr bool EXCL = KIND == Q_UQ;
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint8_t lds_raw[];
     const int lane = lane_id();
-    uint32_t* s = lds + (threadIdx.x >> 6) * (WAVE_LDS / 4);
-    uint8_t* rose = reinterpret_cast<uint8_t*>(s + TILE_WORDS);
+    lds8* s = (lds8*)lds_raw + (threadIdx.x >> 6) * WAVE_LDS<T>;
+    lds8* rose = s + TILE_BYTES<T>;
     unsigned long long visits = 0;
     PhaseTimer pt(q.trace != 0);
+    const uint32_t nwaves = gridDim.x * (kWG / 64);
     while (true) {
         const uint32_t round = a_load(ctl_at(q.ctl, C_ROUND));
         const uint32_t nr = a_load(ctl_at(q.ctl, C_RSIZE));
         uint32_t* rq = (round & 1u) ? q.qb : q.qa;
         uint32_t* wq = (round & 1u) ? q.qa : q.qb;
-        while (true) {
-            uint32_t idx = 0;
-            if (lane == 0) idx = a_add(ctl_at(q.ctl, C_CURSOR), 1u);
-            idx = first_lane(idx);
-            if (idx >= nr) break;
-            uint32_t m = 0;
-            if (lane == 0) {
-                m = ld_coh(&rq[idx]);
-                if (EXCL) a_xchg(&q.st[m], 0u);  // may be queued again for the next round
+        // Read-queue entries are claimed B at a time (about three batches per
+        // wave, 1..16): wave w's first batch is [wB, wB + B), later ones come
+        // from C_CURSOR after the first nwaves * B -- one atomic on the shared
+        // cursor per B visits.  Appends to the write queue collect in a
+        // register buffer (lane i holds entry i) and go out 64 at a time with
+        // one atomic on C_WSIZE and one coalesced store (the reference's block
+        // write buffer, IWPP MP DQ/src/MorphologicalReconstruction.cu:555-589),
+        // and before the round's barrier.
+        const uint32_t B = max(1u, min(16u, nr / (3u * nwaves)));
+        uint32_t claimed = 0, wbuf = 0;
+        int nclaimed = 0, next = 0, wcount = 0;
+        bool first_claim = true;
+        auto flush = [&]() {
+            if (wcount == 0) return;
+            uint32_t start = 0;
+            if (lane == 0) start = a_add(ctl_at(q.ctl, C_WSIZE), (uint32_t)wcount);
+            start = first_lane(start);
+            if (lane < wcount) {
+                const uint32_t pos = start + (uint32_t)lane;
+                if (pos < q.cap) st_coh(&wq[pos], wbuf);
+                else a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
             }
-            m = first_lane(m);
+            wcount = 0;
+        };
+        while (true) {
+            if (next >= nclaimed) {
+                uint32_t c;
+                if (first_claim) {
+                    c = (blockIdx.x * (kWG / 64) + first_lane(threadIdx.x >> 6)) * B;
+                    first_claim = false;
+                } else {
+                    uint32_t cc = 0;
+                    if (lane == 0) cc = a_add(ctl_at(q.ctl, C_CURSOR), B);
+                    c = nwaves * B + first_lane(cc);
+                }
+                if (c >= nr) break;
+                nclaimed = (int)min(B, nr - c);
+                next = 0;
+                claimed = lane < nclaimed ? ld_coh(&rq[c + (uint32_t)lane]) : 0u;
+            }
+            const uint32_t m = rdlane(claimed, next++);
+            if (EXCL && lane == 0) a_xchg(&q.st[m], 0u);  // may be queued again for the next round
             mem_wait();
             pt.lap(0);
             const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
@@ -2093,17 +2362,17 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
             const int ml = (int)m - dsc.mp_base;
             const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
             tile_fetch<T>(im, mx, my, s);
-            const Border ob = tile_border(s);
+            const Border ob = tile_border<T>(s);
             pt.lap(1);
             bool changed;
-            const bool conv = tile_converge<CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
+            const bool conv = tile_converge<T, CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
             pt.lap(2);
             bool rose_any = changed;
             if constexpr (EXCL) tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
             else rose_any = tile_store<T>(s, rose, im, mx, my);
             mem_wait();
             pt.lap(3);
-            uint32_t nb = tile_notify_mask<CONN>(s, rose, im, mx, my);
+            uint32_t nb = tile_notify_mask<T, CONN>(s, rose, im, mx, my);
             wave_lds_sync();
             // DQ: a duplicate visitor of the same megapixel may have used stale
             // pixels, so any rise re-queues the megapixel (as the reference does).
@@ -2116,19 +2385,20 @@ __global__ __launch_bounds__(kWG) void k_queue_bsp(Descs ds, QueueArgs q) {
                 push = EXCL ? (a_or(&q.st[target], 1u) == 0u) : true;
             }
             const uint64_t b = ballot(push);
-            const uint32_t n = (uint32_t)__popcll(b);
+            const int n = __popcll(b);
             if (n) {
-                uint32_t start = 0;
-                if (lane == 0) start = a_add(ctl_at(q.ctl, C_WSIZE), n);
-                start = first_lane(start);
-                if (push) {
-                    const uint32_t pos = start + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-                    if (pos < q.cap) st_coh(&wq[pos], target);
-                    else a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
-                }
+                if (wcount + n > 64) flush();
+                // into the buffer through the (now free) rose area
+                lds32* xs = reinterpret_cast<lds32*>(rose);
+                if (push) xs[__popcll(b & ((1ull << lane) - 1ull))] = target;
+                wave_lds_sync();
+                if (lane >= wcount && lane < wcount + n) wbuf = xs[lane - wcount];
+                wave_lds_sync();
+                wcount += n;
             }
             pt.lap(4);
         }
+        flush();
         if (!grid_barrier(q.ctl, gridDim.x, q.timeout_ticks)) break;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             uint32_t ws = a_load(ctl_at(q.ctl, C_WSIZE));
@@ -2441,7 +2711,9 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
     else if (kind == Q_DQ) fn = (const void*)k_queue_bsp<T, CONN, Q_DQ>;
     else fn = (const void*)k_queue_bsp<T, CONN, Q_UQ>;
     int per_cu = 0;
-    const size_t lds_bytes = 4 * WAVE_LDS;  // 70 KiB: beyond the 64 KiB default, so say so
+    // tiles, and the mutex kernel's local queue (u16: 75 KiB, beyond the 64 KiB
+    // default: say so)
+    const size_t lds_bytes = (kWG / 64) * WAVE_LDS<T> + (kind == Q_MUTEX ? sizeof(LocalQueue) : 0);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds_bytes);
     if (e != hipSuccess) return e;
