@@ -91,6 +91,16 @@ typedef struct {
      * kernel excluded). */
     float ms_rows;
     float ms_cols;
+    /* With iwpp_options.check_queue (the reference's queue self-checks,
+     * IWPP MP DQ/src/globalQueue.cu:218-253 and IWPP MP UQ/src/
+     * MorphologicalReconstruction.cu:1312-1322, as counters):
+     * repeat_visits: mutex -- visits that overlapped another visit of the
+     * same megapixel; DQ / UQ -- visits of a megapixel already visited in the
+     * same round (UQ: must be 0; DQ allows them).  state_left: megapixels
+     * whose state word (mutex: PENDING/RUNNING bits; UQ: queued flag) is not
+     * clear when the queue phase ends (must be 0). */
+    long long repeat_visits;
+    long long state_left;
 } iwpp_stats;
 
 /* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
@@ -106,6 +116,9 @@ typedef struct {
     int workers_per_cu;   /* persistent workgroups per CU; 0 = from occupancy */
     double timeout_s;     /* device-side wait limit (default 60 s) */
     int trace_phases;     /* 1 = fill the us_* phase timers of iwpp_stats */
+    int check_queue;      /* 1 = check the queue variant's discipline and fill
+                             repeat_visits / state_left of iwpp_stats (costs
+                             two atomics per visit and one extra launch) */
 } iwpp_options;
 
 void iwpp_default_options(iwpp_options* opt);

@@ -88,6 +88,8 @@ struct iwpp_workspace {
     size_t qa_n = 0;
     uint32_t* qb = nullptr;
     size_t qb_n = 0;
+    uint32_t* chk = nullptr;    // queue-discipline checks (per megapixel)
+    size_t chk_n = 0;
     unsigned long long* ring = nullptr;  // mutex ring
     size_t ring_n = 0;
     uint32_t* ctl = nullptr;    // control block
@@ -130,6 +132,7 @@ extern "C" void iwpp_default_options(iwpp_options* o) {
     o->workers_per_cu = 0;
     o->timeout_s = 60.0;
     o->trace_phases = 0;
+    o->check_queue = 0;
 }
 
 extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtype) {
@@ -163,6 +166,7 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->edges);
     (void)hipFree(ws->qa);
     (void)hipFree(ws->qb);
+    (void)hipFree(ws->chk);
     (void)hipFree(ws->ring);
     (void)hipFree(ws->ctl);
     (void)hipFree(ws->tiles);
@@ -548,8 +552,14 @@ int setup_queue(Run& r) {
     q.nmp = r.total_mp;
     q.max_iters = std::max(1, r.opt.max_local_iters);
     q.max_rounds = 1 << 30;
+    q.grid_div = 1;
+    q.start_ticks = timeout_ticks(0.02);  // 20 ms for the grid to start together
     q.timeout_ticks = timeout_ticks(r.opt.timeout_s > 0 ? r.opt.timeout_s : 60.0);
     q.trace = r.opt.trace_phases ? 1 : 0;
+    if (r.opt.check_queue) {
+        HIP_TRY(grow(ws->chk, ws->chk_n, nmp));
+        q.chk = ws->chk;
+    }
     q.order_tiles = r.order_tiles;
     q.order_start = r.order_start;
     q.order_nt = r.order_nt;
@@ -570,6 +580,7 @@ int queue_launches(Run& r) {
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[1], r.s));
     HIP_TRY(launch_fill_u32(ws->ctl, 0u, (size_t)C_NSLOTS * kCtlStride, r.s));
     HIP_TRY(launch_fill_u32(ws->st, 0u, (size_t)r.total_mp, r.s));
+    if (r.q.chk) HIP_TRY(launch_fill_u32(r.q.chk, 0u, (size_t)r.total_mp, r.s));
     if (kind == Q_MUTEX) HIP_TRY(launch_ring_init(ws->ring, r.cap, r.s));
     if (!r.flags_ready)  // (after a restart the marker has changed: scan)
         HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
@@ -583,6 +594,7 @@ int queue_launches(Run& r) {
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[2], r.s));
     HIP_TRY(launch_queue_phase(r.ds, r.dtype, r.conn, kind, r.q, r.opt.workers_per_cu, r.s));
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[3], r.s));
+    if (r.q.chk && kind != Q_DQ) HIP_TRY(launch_state_left(r.q, r.s));
     return IWPP_OK;
 }
 
@@ -594,6 +606,8 @@ int read_counters(Run& r) {
     r.st.seeded += h[C_SEEDED * kCtlStride] + 0ll;  // (C_TAIL also counts the mutex ring's pushes)
     r.st.tile_visits += (long long)(*(const unsigned long long*)(h + C_VISITS_LO * kCtlStride));
     r.st.queue_rounds += (int)h[C_ROUND * kCtlStride];
+    r.st.repeat_visits += h[C_REPEAT * kCtlStride];
+    r.st.state_left += h[C_STATE_LEFT * kCtlStride];
     if (r.q.trace) {
         auto ph = [&](int slot) {
             return (double)(*(const unsigned long long*)(h + slot * kCtlStride)) * us_per_tick;
@@ -626,6 +640,12 @@ int queue_phase(Run& r) {
             HIP_TRY(hipEventElapsedTime(&b, ws->ev[2], ws->ev[3]));
             r.st.ms_seed += a;
             r.st.ms_queue += b;
+        }
+        if (ws->host_ctl[C_NOTRES * kCtlStride]) {  // not co-resident: nothing ran
+            if (r.q.grid_div >= 64) return fail(IWPP_ERR_HIP, "queue kernel never co-resident");
+            r.q.grid_div *= 2;
+            --attempt;
+            continue;
         }
         rc = read_counters(r);
         if (rc < 0) return rc;

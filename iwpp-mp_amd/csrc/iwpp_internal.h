@@ -35,6 +35,10 @@ enum Ctl : int {
     C_PH_NOTIFY,
     C_SEEDED,        // mutex ring: entries put in by the seeding pass (tickets 0..C_SEEDED-1)
     C_SCLAIM,        // mutex ring: next seeded ticket to claim (claimed in batches)
+    C_REPEAT,        // queue checks: visits that broke the variant's discipline
+    C_STATE_LEFT,    // queue checks: megapixel state words not clear after the phase
+    C_START,         // bsp: blocks past the start gate
+    C_NOTRES,        // bsp: the grid was not co-resident (relaunch it smaller)
     C_NSLOTS
 };
 
@@ -107,8 +111,15 @@ struct QueueArgs {
     int nmp;            // megapixels of all images
     int max_iters;      // local sweep rounds per visit
     int max_rounds;     // bsp round limit
+    int grid_div;       // bsp: the launch uses 1/grid_div of the co-resident grid
+    unsigned long long start_ticks;  // bsp: wall-clock ticks the start gate waits
     unsigned long long timeout_ticks;  // wall-clock ticks a wait may last
     int trace;          // accumulate the phase timers
+    // Queue-discipline checks (nullptr: off): one word per megapixel.  mutex:
+    // visits of a megapixel in progress at once (must be 1); DQ / UQ: the
+    // last round that visited it (UQ: at most once per round).  Breaches
+    // count in C_REPEAT.
+    uint32_t* chk;
     // Seeding order by tiles (nullptr = megapixel index order): rank r takes
     // tile order_tiles[r]; its megapixels occupy queue positions
     // order_start[r] .. order_start[r+1]-1, row by row.  One image only.
@@ -164,6 +175,8 @@ hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s);
 hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const QueueArgs& q,
                              hipStream_t s);
 hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s);
+// Counts the nonzero words of q.st[0..nmp) into C_STATE_LEFT (queue checks).
+hipError_t launch_state_left(const QueueArgs& q, hipStream_t s);
 hipError_t launch_queue_phase(const Descs& ds, int dtype, int conn, int kind,
                               const QueueArgs& q, int workers_per_cu, hipStream_t s);
 

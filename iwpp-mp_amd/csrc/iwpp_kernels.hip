@@ -11,6 +11,8 @@
 // identity (0, ~0).  Absent pixels are (0, 0): they neither rise nor conduct.
 #include "iwpp_internal.h"
 
+#include <cstdlib>
+
 namespace iwpp {
 
 #define DEV static __device__ __forceinline__
@@ -2206,7 +2208,10 @@ void k_queue_mutex(Descs ds, QueueArgs q) {
                 }
                 if (state != 1) break;
             }
-            if (lane == 0) a_xchg(&q.st[m], ST_RUNNING);
+            if (lane == 0) {
+                a_xchg(&q.st[m], ST_RUNNING);
+                if (q.chk && a_add(&q.chk[m], 1u) != 0u) a_add(ctl_at(q.ctl, C_REPEAT), 1u);
+            }
             mem_wait();
             pt.lap(0);
             const ImgDesc d0 = desc(ds, find_mp(ds, (int)m));
@@ -2225,7 +2230,11 @@ void k_queue_mutex(Descs ds, QueueArgs q) {
         tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
         uint32_t m2 = 0;
         const bool have2 = take_claimed(m2) || lq_pop(L, q, m2);
-        if (have2 && lane == 0) a_xchg(&q.st[m2], ST_RUNNING);
+        if (have2 && lane == 0) {
+            a_xchg(&q.st[m2], ST_RUNNING);
+            if (q.chk && a_add(&q.chk[m2], 1u) != 0u) a_add(ctl_at(q.ctl, C_REPEAT), 1u);
+        }
+        if (q.chk && lane == 0) a_add(&q.chk[m], 0u - 1u);  // before m's release below
         uint32_t nb = tile_notify_mask<T, CONN>(s, rose, im, mx, my);
         if (!conv) nb |= NB_SELF;
         mem_wait();  // this visit's stores are visible and m2 is taken
@@ -2270,25 +2279,64 @@ void k_queue_mutex(Descs ds, QueueArgs q) {
 }
 
 // ------------------------------------------------ DQ / UQ (BSP rounds) ------
-// Persistent cooperative kernel.  Round k consumes the read queue (dynamic
-// claiming by atomicAdd), appends to the write queue, then a grid barrier,
-// a queue flip by one thread, and a second barrier.
-DEV bool grid_barrier(uint32_t* ctl, uint32_t nblocks, unsigned long long limit) {
+// Persistent cooperative kernel.  Round k consumes the read queue (claimed
+// in batches), appends to the write queue (through a register buffer), and
+// ends with one grid barrier that also flips the queues.
+// Start gate of the round-based kernels: every block of the grid must be
+// running at once (the rounds end in grid barriers).  A cooperative launch
+// guarantees that; the ordinary launch used when the runtime refuses one
+// does not.  Blocks that find the others within start_ticks go on; otherwise
+// the gate raises C_NOTRES and every block leaves at once (late blocks see
+// the flag on arrival), and the host relaunches with a smaller grid -- a few
+// milliseconds lost instead of a barrier timing out.
+DEV bool start_gate(const QueueArgs& q, uint32_t nblocks) {
+    __shared__ int ok_s;
+    if (threadIdx.x == 0) {
+        ok_s = 0;
+        if (a_load(ctl_at(q.ctl, C_NOTRES)) == 0u) {
+            a_add(ctl_at(q.ctl, C_START), 1u);
+            const unsigned long long t0 = now_ticks();
+            for (uint32_t k = 0;; ++k) {
+                if (a_load(ctl_at(q.ctl, C_START)) >= nblocks) { ok_s = 1; break; }
+                if (a_load(ctl_at(q.ctl, C_NOTRES)) != 0u) break;
+                __builtin_amdgcn_s_sleep(2);
+                if ((k & 15u) == 15u && now_ticks() - t0 > q.start_ticks) {
+                    a_store(ctl_at(q.ctl, C_NOTRES), 1u);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return ok_s != 0;
+}
+
+// The end of a round: a grid barrier whose last arriving block also flips the
+// queues (the write queue becomes the next read queue) before it releases the
+// others, so a round costs one barrier.  Every append of the round is done
+// before its block arrives.
+DEV bool round_barrier(const QueueArgs& q, uint32_t nblocks, uint32_t round) {
     mem_wait();
     __syncthreads();
     __shared__ int ok_s;
     if (threadIdx.x == 0) {
         ok_s = 1;
-        const uint32_t gen = a_load(ctl_at(ctl, C_BAR_GEN));
-        const uint32_t arrived = a_add(ctl_at(ctl, C_BAR_COUNT), 1u);
+        const uint32_t gen = a_load(ctl_at(q.ctl, C_BAR_GEN));
+        const uint32_t arrived = a_add(ctl_at(q.ctl, C_BAR_COUNT), 1u);
         if (arrived == nblocks - 1u) {
-            a_store(ctl_at(ctl, C_BAR_COUNT), 0u);
+            uint32_t ws = a_load(ctl_at(q.ctl, C_WSIZE));
+            if (ws > q.cap) ws = q.cap;
+            a_store(ctl_at(q.ctl, C_RSIZE), ws);
+            a_store(ctl_at(q.ctl, C_WSIZE), 0u);
+            a_store(ctl_at(q.ctl, C_CURSOR), 0u);
+            a_store(ctl_at(q.ctl, C_ROUND), round + 1u);
+            a_store(ctl_at(q.ctl, C_BAR_COUNT), 0u);
             mem_wait();
-            a_store(ctl_at(ctl, C_BAR_GEN), gen + 1u);
+            a_store(ctl_at(q.ctl, C_BAR_GEN), gen + 1u);
         } else {
-            Waiter w(limit);
-            while (a_load(ctl_at(ctl, C_BAR_GEN)) == gen) {
-                if (!w.step(ctl)) { ok_s = 0; break; }
+            Waiter w(q.timeout_ticks);
+            while (a_load(ctl_at(q.ctl, C_BAR_GEN)) == gen) {
+                if (!w.step(q.ctl)) { ok_s = 0; break; }
             }
         }
     }
@@ -2306,6 +2354,7 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
     lds8* s = (lds8*)lds_raw + (threadIdx.x >> 6) * WAVE_LDS<T>;
     lds8* rose = s + TILE_BYTES<T>;
     unsigned long long visits = 0;
+    if (!start_gate(q, gridDim.x)) return;
     PhaseTimer pt(q.trace != 0);
     const uint32_t nwaves = gridDim.x * (kWG / 64);
     while (true) {
@@ -2337,7 +2386,8 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
             }
             wcount = 0;
         };
-        while (true) {
+        // Next claimed megapixel (claims a new batch when this one is used up).
+        auto take = [&](uint32_t& out) -> bool {
             if (next >= nclaimed) {
                 uint32_t c;
                 if (first_claim) {
@@ -2348,22 +2398,40 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
                     if (lane == 0) cc = a_add(ctl_at(q.ctl, C_CURSOR), B);
                     c = nwaves * B + first_lane(cc);
                 }
-                if (c >= nr) break;
+                if (c >= nr) {
+                    nclaimed = next = 0;
+                    return false;
+                }
                 nclaimed = (int)min(B, nr - c);
                 next = 0;
                 claimed = lane < nclaimed ? ld_coh(&rq[c + (uint32_t)lane]) : 0u;
             }
-            const uint32_t m = rdlane(claimed, next++);
-            if (EXCL && lane == 0) a_xchg(&q.st[m], 0u);  // may be queued again for the next round
-            mem_wait();
-            pt.lap(0);
+            out = rdlane(claimed, next++);
+            if (lane == 0) {
+                if (EXCL) a_xchg(&q.st[out], 0u);  // may be queued again for the next round
+                if (q.chk && a_xchg(&q.chk[out], round + 1u) == round + 1u)
+                    a_add(ctl_at(q.ctl, C_REPEAT), 1u);
+            }
+            return true;
+        };
+        bool more = true;
+        uint32_t m = 0;
+        bool have = false;  // m's tile is in LDS
+        while (true) {
+            if (!have) {
+                if (!more || !take(m)) break;
+                mem_wait();
+                pt.lap(0);
+                const ImgDesc d0 = desc(ds, find_mp(ds, (int)m));
+                const int ml0 = (int)m - d0.mp_base;
+                tile_fetch<T>(image_of(d0), ml0 % d0.nmx, ml0 / d0.nmx, s);
+                pt.lap(1);
+            }
             const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
             const Image im = image_of(dsc);
             const int ml = (int)m - dsc.mp_base;
             const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
-            tile_fetch<T>(im, mx, my, s);
             const Border ob = tile_border<T>(s);
-            pt.lap(1);
             bool changed;
             const bool conv = tile_converge<T, CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
             pt.lap(2);
@@ -2378,12 +2446,23 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
             // pixels, so any rise re-queues the megapixel (as the reference does).
             if (!conv || (!EXCL && rose_any)) nb |= NB_SELF;
             ++visits;
-            bool push = false;
-            uint32_t target = 0;
-            if (lane < 9 && ((nb >> lane) & 1u)) {
+            uint32_t target = 0, old = 0;
+            const bool cand = lane < 9 && ((nb >> lane) & 1u);
+            if (cand) {
                 target = (uint32_t)(dsc.mp_base + neighbour_index(ml, dsc.nmx, 1u << lane));
-                push = EXCL ? (a_or(&q.st[target], 1u) == 0u) : true;
+                if (EXCL) old = a_or(&q.st[target], 1u);
             }
+            // the next megapixel's loads go out behind the atomics (its fetch
+            // follows this visit's completed stores)
+            uint32_t m2 = 0;
+            const bool have2 = more && take(m2);
+            more = have2;
+            if (have2) {
+                const ImgDesc d2 = desc(ds, find_mp(ds, (int)m2));
+                const int ml2 = (int)m2 - d2.mp_base;
+                tile_fetch<T>(image_of(d2), ml2 % d2.nmx, ml2 / d2.nmx, s);
+            }
+            const bool push = cand && (!EXCL || old == 0u);
             const uint64_t b = ballot(push);
             const int n = __popcll(b);
             if (n) {
@@ -2397,18 +2476,11 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
                 wcount += n;
             }
             pt.lap(4);
+            m = m2;
+            have = have2;
         }
         flush();
-        if (!grid_barrier(q.ctl, gridDim.x, q.timeout_ticks)) break;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            uint32_t ws = a_load(ctl_at(q.ctl, C_WSIZE));
-            if (ws > q.cap) ws = q.cap;
-            a_store(ctl_at(q.ctl, C_RSIZE), ws);
-            a_store(ctl_at(q.ctl, C_WSIZE), 0u);
-            a_store(ctl_at(q.ctl, C_CURSOR), 0u);
-            a_store(ctl_at(q.ctl, C_ROUND), round + 1u);
-        }
-        if (!grid_barrier(q.ctl, gridDim.x, q.timeout_ticks)) break;
+        if (!round_barrier(q, gridDim.x, round)) break;
         if (a_load(ctl_at(q.ctl, C_RSIZE)) == 0u) break;
         if (a_load(ctl_at(q.ctl, C_OVERFLOW)) != 0u) break;
         if (a_load(ctl_at(q.ctl, C_TIMEOUT)) != 0u) break;
@@ -2435,6 +2507,14 @@ __global__ void k_prepare_queue(QueueArgs q) {
         a_store(ctl_at(q.ctl, C_SEEDED), k);  // mutex: seeded tickets are claimed apart,
         a_store(ctl_at(q.ctl, C_HEAD), k);    // single pops start after them
     }
+}
+
+// Queue checks: how many megapixel state words are not clear.
+__global__ __launch_bounds__(kWG) void k_state_left(QueueArgs q) {
+    uint32_t n = 0;
+    for (int i = blockIdx.x * kWG + (int)threadIdx.x; i < q.nmp; i += gridDim.x * kWG)
+        n += q.st[i] != 0u;
+    if (n) a_add(ctl_at(q.ctl, C_STATE_LEFT), n);  // (none when the phase ended clean)
 }
 
 // ============================================================ launchers ====
@@ -2688,6 +2768,12 @@ hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_state_left(const QueueArgs& q, hipStream_t s) {
+    const int grid = std::max(1, std::min((q.nmp + kWG - 1) / kWG, cu_count() * 4));
+    hipLaunchKernelGGL(k_state_left, grid, kWG, 0, s, q);
+    return hipGetLastError();
+}
+
 hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s) {
     hipLaunchKernelGGL(k_prepare_queue, 1, 64, 0, s, q);
     return hipGetLastError();
@@ -2721,8 +2807,16 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
     if (e != hipSuccess) return e;
     if (per_cu <= 0) return hipErrorInvalidConfiguration;
     if (wpc > 0) per_cu = std::min(per_cu, wpc);
-    int grid = per_cu * cu_count();
+    int grid = per_cu * cu_count() / std::max(1, q.grid_div);
+    // Test hook (failure handling): IWPP_TEST_OVERSUBSCRIBE=k launches the
+    // round-based kernels k times larger than fits, without the cooperative
+    // launch, so that the start gate must send them back.
+    int over = 1;
+    if (kind != Q_MUTEX) {
+        if (const char* env = std::getenv("IWPP_TEST_OVERSUBSCRIBE")) over = std::max(1, std::atoi(env));
+    }
     grid = std::max(1, std::min(grid, (q.nmp + 3) / 4));
+    if (over > 1) grid = per_cu * cu_count() * over / std::max(1, q.grid_div);
     Descs dc = ds;
     QueueArgs qc = q;
     if (kind == Q_MUTEX) {
@@ -2730,13 +2824,15 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
         return hipGetLastError();
     }
     void* args[] = {&dc, &qc};
-    e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, lds_bytes, s);
-    if (e == hipSuccess) return e;
+    if (over == 1) {
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kWG), args, lds_bytes, s);
+        if (e == hipSuccess) return e;
+    }
     // The runtime refuses cooperative launches in some states (for instance
     // while another process holds the device).  The grid is sized to fit the
-    // device at once, and every device-side wait of the kernel is bounded in
-    // time, so an ordinary launch either runs co-resident as intended or ends
-    // with the timeout flag set -- it cannot hang.
+    // device at once; if it still does not run co-resident, the kernel's
+    // start gate sends every block back within q.start_ticks and the host
+    // relaunches a smaller grid (every device-side wait is bounded in time).
     (void)hipGetLastError();
     if (kind == Q_DQ)
         hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_DQ>), grid, kWG, lds_bytes, s, dc, qc);

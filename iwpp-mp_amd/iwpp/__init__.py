@@ -43,7 +43,7 @@ class Options(ctypes.Structure):
     _fields_ = [("sweep_rounds", ctypes.c_int), ("col_block_rows", ctypes.c_int),
                 ("max_local_iters", ctypes.c_int), ("queue_capacity", ctypes.c_int),
                 ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double),
-                ("trace_phases", ctypes.c_int)]
+                ("trace_phases", ctypes.c_int), ("check_queue", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -57,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("us_wait", ctypes.c_double), ("us_fetch", ctypes.c_double),
                 ("us_converge", ctypes.c_double), ("us_store", ctypes.c_double),
                 ("us_notify", ctypes.c_double), ("ms_rows", ctypes.c_float),
-                ("ms_cols", ctypes.c_float)]
+                ("ms_cols", ctypes.c_float), ("repeat_visits", ctypes.c_longlong),
+                ("state_left", ctypes.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
