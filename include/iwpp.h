@@ -119,6 +119,10 @@ typedef struct {
     int check_queue;      /* 1 = check the queue variant's discipline and fill
                              repeat_visits / state_left of iwpp_stats (costs
                              two atomics per visit and one extra launch) */
+    long long learned_min_pixels; /* IWPP_SCHED_LEARNED plans a single image only
+                             from this many pixels on (default 2^27; smaller
+                             images keep the plain rule: the plan's fixed cost
+                             exceeds what it can save there); 0 = always */
 } iwpp_options;
 
 void iwpp_default_options(iwpp_options* opt);

@@ -123,6 +123,10 @@ struct iwpp_workspace {
     hipStream_t own = nullptr;  // used when the caller passes the NULL stream
 };
 
+namespace {
+constexpr long long kLearnedMinPixels = 1ll << 27;  // one image: plan from 11585^2 up
+}
+
 extern "C" void iwpp_default_options(iwpp_options* o) {
     if (!o) return;
     o->sweep_rounds = -1;
@@ -133,6 +137,7 @@ extern "C" void iwpp_default_options(iwpp_options* o) {
     o->timeout_s = 60.0;
     o->trace_phases = 0;
     o->check_queue = 0;
+    o->learned_min_pixels = kLearnedMinPixels;
 }
 
 extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtype) {
@@ -824,8 +829,7 @@ int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
         HIP_TRY(launch_plan_features(one, nullptr, L.total, v16, r.dtype, L.tile, kFeatureSub,
                                      ws->feat_part, r.s));
         r.plan_tile_base = nullptr;
-        r.plan_order = L.total > 1 && L.total <= kOrderMax;
-        if (r.plan_order) HIP_TRY(grow(ws->order, ws->order_n, (size_t)2 * L.total + 1));
+        r.plan_order = false;  // seeding keeps index order (see learned_plan_end_single)
     } else {
         bool v16 = true;
         for (const ImgDesc& d : r.h)
@@ -839,11 +843,11 @@ int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
                                      kFeatureSub, ws->feat_part, r.s));
         r.plan_tile_base = ws->order;
     }
-    r.plan_predict = true;
     r.plan_tiles = L.total;
     r.plan_tile = L.tile;
     r.plan_seq = ++ws->plan_seq;
     if (r.plan_seq == 0) r.plan_seq = ++ws->plan_seq;  // 0 is the flag's initial value
+    r.plan_predict = true;  // launched after the first row pass (sweep_phase)
     *pending = true;
     return IWPP_OK;
 }
@@ -897,9 +901,10 @@ int learned_plan_end_batch(Run& r, SchedulePlan& plan) {
     return IWPP_OK;
 }
 
-// One image: R = area-weighted mean of the predicted rounds; a tile whose own
-// prediction is at least two rounds lower skips the rounds from its
-// prediction on (never the first, which has already been launched); the
+// One image: tile t runs k_t = its predicted rounds (at least the first,
+// which has already been launched); the image runs R = the largest k_t that
+// tiles of at least 1/64 of the image ask for (a tile asking for more gets R),
+// and a tile with k_t < R skips the rounds from k_t on (tile masks); the
 // seeding pass visits tiles by decreasing predicted priority.
 int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     iwpp_workspace* ws = r.ws;
@@ -913,44 +918,32 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
                                                 sizeof(ImgDesc),
                                             64);  // tiles[nt], starts[nt+1]
     uint8_t* h_masks = (uint8_t*)(h_order + 2 * nt + 1);
-    const int rounds = image_rounds(d, pred, 0, (uint32_t)nt, tile);
     std::vector<int> k(nt);
+    std::vector<double> area_at(6, 0.0);  // image area asking for exactly r rounds
+    double total_area = 0;
     for (int t = 0; t < nt; ++t) {
-        const int kt = std::max(1, clamp_rounds(pred[(size_t)t * 2]));
-        k[t] = kt <= rounds - 2 ? kt : rounds;
+        const int tx = t % ntx, ty = t / ntx;
+        const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
+                            (double)(std::min(d.H, (ty + 1) * tile) - ty * tile);
+        k[t] = std::max(1, clamp_rounds(pred[(size_t)t * 2]));
+        area_at[k[t]] += area;
+        total_area += area;
     }
+    int rounds = 4;
+    while (rounds > 1 && area_at[rounds] < total_area / 64.0) {
+        area_at[rounds - 1] += area_at[rounds];
+        --rounds;
+    }
+    for (int t = 0; t < nt; ++t) k[t] = std::min(k[t], rounds);
     plan.sweep_rounds = rounds;
     r.st.schedule = IWPP_SCHED_LEARNED;
     int skipped = 0;
     for (int t = 0; t < nt; ++t) skipped += std::max(0, rounds - k[t]);
     r.st.sweeps_skipped_tiles = skipped;
-    if (nt == 1) return IWPP_OK;  // one tile: uniform rounds, nothing to order
-    const int mpt = tile / kMP;   // megapixels per tile side
-    r.order_tiles = ws->order;
-    r.order_start = ws->order + nt;
-    r.order_nt = nt;
-    r.order_ntx = ntx;
-    r.order_mpt = mpt;
-    if (!r.plan_order) {  // more tiles than the network kernel orders: sort here
-    std::vector<int> tiles(nt);
-    for (int t = 0; t < nt; ++t) tiles[t] = t;
-    std::stable_sort(tiles.begin(), tiles.end(), [&](int a, int b) {
-        return pred[(size_t)a * 2 + 1] > pred[(size_t)b * 2 + 1];
-    });
-    uint32_t pos = 0;
-    for (int rk = 0; rk < nt; ++rk) {
-        const int t = tiles[rk], tx = t % ntx, ty = t / ntx;
-        h_order[rk] = (uint32_t)t;
-        h_order[nt + rk] = pos;
-        pos += (uint32_t)(std::min(mpt, d.nmx - tx * mpt) * std::min(mpt, d.nmy - ty * mpt));
-    }
-    h_order[2 * nt] = pos;
-    HIP_TRY(grow(ws->order, ws->order_n, (size_t)2 * nt + 1));
-    HIP_TRY(hipMemcpyAsync(ws->order, h_order, ((size_t)2 * nt + 1) * 4, hipMemcpyHostToDevice,
-                           r.s));
-    r.order_tiles = ws->order;
-    r.order_start = ws->order + nt;
-    }
+    if (nt == 1) return IWPP_OK;  // one tile: uniform rounds, no masks
+    // The network's second output (tile priority) is not applied: seeding
+    // the queue tile by tile in decreasing priority was measured at 32768^2
+    // to cost more visits than index order (137K against 131K, +0.085 ms).
     if (skipped > 0) {  // per-round tile masks, only when some tile stops early
         const size_t nm = (size_t)rounds * nt;
         for (int rr = 0; rr < rounds; ++rr)
@@ -998,7 +991,16 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
     rc = setup_images(r, imgs, n);
     if (rc) return rc;
     bool planning = false;
-    if (schedule == IWPP_SCHED_LEARNED && !r.graph) {
+    // The learned plan costs about 30-50 us (tile features, network, and the
+    // host planning between the first downward and upward column passes,
+    // which the GPU waits for when those passes are short).  Measured on
+    // 6144^2 and 8192^2 images, that is 5-10% of the whole reconstruction and
+    // more than the best schedule saves over the plain rule, so a single image
+    // below opt.learned_min_pixels keeps the plain rule; batches always plan.
+    long long px_total = 0;
+    for (int i = 0; i < n; ++i) px_total += (long long)imgs[i].width * imgs[i].height;
+    if (schedule == IWPP_SCHED_LEARNED && !r.graph &&
+        (n > 1 || px_total >= r.opt.learned_min_pixels)) {
         rc = learned_plan_begin(r, plan, &planning);
         if (rc) return rc;
     }

@@ -7,7 +7,7 @@ namespace iwpp {
 
 constexpr int kMlpParams = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2;  // 5378
 constexpr int kMlpIn = 16, kMlpOut = 2;
-constexpr int kFeatureSub = 32;  // row subsampling of the tile features in a reconstruction
+constexpr int kFeatureSub = 64;  // row subsampling of the tile features in a reconstruction
 constexpr int kOrderMax = 512;  // most tiles whose seeding order the network kernel computes
 
 // BF16 round to nearest even from the IEEE-754 binary32 layout (NaN kept quiet).

@@ -1,9 +1,11 @@
 // Plain heuristic schedule, from measurements on MI355X: the mutex variant was
-// fastest or tied on 4096^2 u8 seeds and 4096^2 u16 h-dome images; the best
-// number of sweep rounds before the queue phase was 3 for 4096^2 seeds, 2 for
-// 32768^2 seeds and 1 for 16384^2 h-dome -- data dependent, which is what the
-// learned schedule decides per tile.  Here: 1 round below 2048 px, 3 up to
-// 8191, 2 from 8192.  The tile is 1024 (2048 when the short side is >= 8192).
+// fastest or tied on 4096^2 u8 seeds and 4096^2 u16 h-dome images.  Whole
+// reconstructions with k = 0, 1, 2, 3 uniform sweep rounds (ms): 1024^2 h-dome
+// 0.178 0.176 0.197 0.216; 4096^2 seeds 0.353 0.351 0.353 0.387; 6144^2 seeds
+// 0.496 0.518 0.494 0.544; 8192^2 seeds 0.698 0.690 0.596 0.668; 8192^2 h-dome
+// 0.707 0.684 0.642 0.657; 16384^2 u16 h-dome 1.97 1.51 1.97 2.68; 32768^2
+// seeds 6.24 3.09 4.07 5.48.  Here: 2 rounds for short sides of 6144..16383
+// pixels, 1 otherwise.  The tile is 1024 (2048 when the short side is >= 8192).
 #include "iwpp_sched.h"
 
 #include <algorithm>
@@ -61,7 +63,7 @@ int plan_schedule(int w, int h, int dtype, int conn, int tile, int queue, int sc
     const int side = std::min(w, h);
     out->tile = tile > 0 ? tile : (side >= 8192 ? 2048 : 1024);
     out->queue = queue == IWPP_QUEUE_AUTO ? IWPP_QUEUE_MUTEX : queue;
-    out->sweep_rounds = sweep_rounds >= 0 ? sweep_rounds : (side >= 8192 ? 2 : side >= 2048 ? 3 : 1);
+    out->sweep_rounds = sweep_rounds >= 0 ? sweep_rounds : (side >= 6144 && side < 16384 ? 2 : 1);
     return 0;
 }
 

@@ -43,7 +43,8 @@ class Options(ctypes.Structure):
     _fields_ = [("sweep_rounds", ctypes.c_int), ("col_block_rows", ctypes.c_int),
                 ("max_local_iters", ctypes.c_int), ("queue_capacity", ctypes.c_int),
                 ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double),
-                ("trace_phases", ctypes.c_int), ("check_queue", ctypes.c_int)]
+                ("trace_phases", ctypes.c_int), ("check_queue", ctypes.c_int),
+                ("learned_min_pixels", ctypes.c_longlong)]
 
 
 class Stats(ctypes.Structure):
@@ -275,9 +276,9 @@ class Workspace:
         self.close()
 
     def __del__(self):
-        if sys.is_finalizing():
-            return
         try:
+            if sys.is_finalizing():
+                return
             self.close()
         except Exception:
             pass
@@ -408,9 +409,9 @@ class BatchGraph:
         _untrack(self)
 
     def __del__(self):
-        if sys.is_finalizing():
-            return
         try:
+            if sys.is_finalizing():
+                return
             self.close()
         except Exception:
             pass

@@ -44,7 +44,7 @@ MODEL_PATH = os.path.join(MODEL_DIR, "sched_mlp.f32")
 META_PATH = os.path.join(MODEL_DIR, "sched_mlp.json")
 
 MAX_ROUNDS = 4
-FEATURE_SUB = 32  # row subsampling of the features (the library uses the same)
+FEATURE_SUB = 64  # row subsampling of the features (the library uses the same)
 FEATURE_SET = 4   # feature definitions (2: feature 15 = log2(image pixels) / 32;
                   # 4: left/right border inflow on the sampled rows only)
 
@@ -207,15 +207,41 @@ def image_timings(mask, marker, conn, rounds=MAX_ROUNDS, trials=5, workspace=Non
 TIE = 1.02  # timings within 2% count as equal (the smaller k wins)
 
 
-def labels(tr, timings):
-    """(k*, priority) per tile: k* = the fewest uniform sweep rounds whose
-    measured reconstruction of the tile's image is within TIE of the
-    fastest; priority = the
-    tile's mean reconstructed value over the image's largest mask value."""
+def cost_model(timings, dirty_total):
+    """Fit of an image's whole-reconstruction time with k uniform sweep rounds,
+    t(k) = a + b k + c D(k), D(k) = megapixels still dirty after k rounds:
+    b = ms per sweep round over the whole image, c = ms per dirty megapixel
+    left to the queue phase.  None when the fit is not physical (b or c <= 0)."""
     t = np.asarray(timings, dtype=np.float64)
-    k = float(np.flatnonzero(t <= TIE * t.min())[0])  # fewest rounds within TIE of the best
+    D = np.asarray(dirty_total, dtype=np.float64)[: t.size]
+    A = np.stack([np.ones_like(t), np.arange(t.size, dtype=np.float64), D], 1)
+    (a, b, c), *_ = np.linalg.lstsq(A, t, rcond=None)
+    return (a, b, c) if b > 0 and c > 0 else None
+
+
+def labels(tr, timings):
+    """(k, priority) per tile.
+    k: with a physical cost model of the image (cost_model), each tile's own
+    argmin over k of  k * b * (tile pixels / image pixels) + c * d_t(k)
+    (its share of the sweep rounds against the queue work its dirty
+    megapixels leave), ties to fewer rounds; otherwise the image's k* = the
+    fewest uniform rounds within TIE of the fastest measured.
+    priority: the tile's mean reconstructed value over the image's largest
+    mask value."""
+    t = np.asarray(timings, dtype=np.float64)
+    k_img = float(np.flatnonzero(t <= TIE * t.min())[0])  # fewest rounds within TIE of the best
     prio = (tr["final_mean"] / max(tr["maxval"], 1.0)).astype(np.float32)
-    return np.stack([np.full(prio.shape, k, np.float32), prio], 1)
+    d = np.asarray(tr["dirty"], dtype=np.float64)[:, : t.size] if "dirty" in tr else None
+    fit = cost_model(t, d.sum(0)) if d is not None else None
+    if fit is None:
+        k = np.full(prio.shape, k_img, np.float32)
+    else:
+        _, b, c = fit
+        share = np.asarray(tr["pixels"], dtype=np.float64) / float(np.sum(tr["pixels"]))
+        ks = np.arange(t.size, dtype=np.float64)
+        cost = ks[None, :] * b * share[:, None] + c * d
+        k = np.argmin(cost + 1e-12 * ks[None, :], axis=1).astype(np.float32)
+    return np.stack([k, prio], 1)
 
 
 # ------------------------------------------------------------- training -----
@@ -304,10 +330,11 @@ def training_cases(counts=None, seed_base=None):
 
 def build_dataset(cases, rounds=MAX_ROUNDS, log=None):
     """Features and labels of every tile of `cases` (traces and timings on
-    the GPU).  Returns X, Y and per-image information."""
+    the GPU).  Returns X, Y, per-image information, and the traces' dirty
+    counts (tiles x rounds + 1) and tile pixels (for relabelling)."""
     from . import Workspace as _W
     ws = _W()
-    X, Y, info = [], [], []
+    X, Y, info, Ds, Ps, Rs = [], [], [], [], [], []
     for name, mask, marker, conn in cases:
         t0 = time.time()
         tr = trace(mask, marker, conn, rounds)
@@ -315,14 +342,18 @@ def build_dataset(cases, rounds=MAX_ROUNDS, log=None):
         y = labels(tr, tm)
         X.append(tr["features"])
         Y.append(y)
+        Ds.append(tr["dirty"])
+        Ps.append(tr["pixels"])
         info.append({"image": name, "tiles": int(y.shape[0]), "ms_by_rounds": tm.round(4).tolist(),
-                     "k": int(y[0, 0])})
+                     "dirty_by_rounds": tr["dirty"].sum(0).tolist(),
+                     "k_hist": np.bincount(y[:, 0].astype(int), minlength=rounds + 1).tolist()})
         if log:
             log(f"trace {name}: {tr['features'].shape[0]} tiles, dirty "
                 f"{tr['dirty'].sum(0).tolist()}, ms by rounds {np.round(tm, 3).tolist()}, "
                 f"{time.time() - t0:.1f} s")
     ws.close()
-    return np.concatenate(X), np.concatenate(Y), info
+    return (np.concatenate(X), np.concatenate(Y), info, np.concatenate(Ds).astype(np.int64),
+            np.concatenate(Ps).astype(np.int64))
 
 
 def validation_cases(seed_base=None):
@@ -394,20 +425,22 @@ def main(argv=None):
         d = np.load(os.path.join(args.relabel, "sched_dataset.npz"))
         with open(os.path.join(args.relabel, "sched_mlp.json")) as f:
             info = json.load(f)["images"]
-        X, Y = d["X"], d["Y"].copy()
+        X, Y, D, P = d["X"], d["Y"].copy(), d["D"], d["P"]
         pos = 0
         for im in info:
-            t = np.asarray(im["ms_by_rounds"], dtype=np.float64)
-            k = int(np.flatnonzero(t <= TIE * t.min())[0])
-            Y[pos:pos + im["tiles"], 0] = k
-            im["k"] = k
-            pos += im["tiles"]
+            n = im["tiles"]
+            tr = {"final_mean": np.zeros(n), "maxval": 1.0, "dirty": D[pos:pos + n],
+                  "pixels": P[pos:pos + n]}
+            Y[pos:pos + n, 0] = labels(tr, im["ms_by_rounds"])[:, 0]
+            im["k_hist"] = np.bincount(Y[pos:pos + n, 0].astype(int),
+                                       minlength=MAX_ROUNDS + 1).tolist()
+            pos += n
         assert pos == X.shape[0], "dataset and timings disagree"
         log(f"relabelled {len(info)} images from {args.relabel}")
     else:
         cases = training_cases({"c4": args.c4})
         log(f"{len(cases)} training images generated in {time.time() - t0:.1f} s")
-        X, Y, info = build_dataset(cases, log=lambda m: None if " c4_" in m else log(m))
+        X, Y, info, D, P = build_dataset(cases, log=lambda m: None if " c4_" in m else log(m))
     log(f"dataset: {X.shape[0]} tiles, k* histogram "
         f"{np.bincount(Y[:, 0].astype(int), minlength=MAX_ROUNDS + 1).tolist()}")
     candidates = []
@@ -435,7 +468,7 @@ def main(argv=None):
                                              minlength=MAX_ROUNDS + 1).tolist(),
             "init_seed": seed, "validation_ms_per_init": val}
     save_model(w, meta, os.path.join(args.out, "sched_mlp.f32"))
-    np.savez_compressed(os.path.join(args.out, "sched_dataset.npz"), X=X, Y=Y)
+    np.savez_compressed(os.path.join(args.out, "sched_dataset.npz"), X=X, Y=Y, D=D, P=P)
     log(f"model written to {args.out} ({time.time() - t0:.1f} s)")
 
 

@@ -213,6 +213,31 @@ def test_labels_pick_fastest_round():
     np.testing.assert_allclose(y[:, 1], [10 / 255, 20 / 255, 30 / 255], rtol=1e-6)
 
 
+def test_labels_per_tile_cost_model():
+    # timings made exactly from t(k) = a + b k + c D(k): the fit recovers
+    # (a, b, c), and each tile's label is the argmin of its own share of the
+    # rounds against the queue work its dirty megapixels leave (closed form)
+    a, b, c = 0.2, 1.5, 0.05
+    dirty = np.array([[100, 90, 10, 5, 0],      # a tile a second round clears
+                      [100, 2, 1, 1, 0],        # one round is enough
+                      [0, 0, 0, 0, 0]],         # nothing to do
+                     dtype=np.int64)
+    px = np.array([1000, 1000, 1000])
+    D = dirty.sum(0)
+    t = a + b * np.arange(5) + c * D
+    fit = learned.cost_model(t, D)
+    np.testing.assert_allclose(fit, (a, b, c), rtol=1e-6)
+    tr = {"final_mean": np.zeros(3), "maxval": 1.0, "dirty": dirty, "pixels": px}
+    y = learned.labels(tr, t)
+    share = px / px.sum()
+    want = [int(np.argmin(np.arange(5) * b * share[i] + c * dirty[i])) for i in range(3)]
+    assert y[:, 0].tolist() == want
+    # per tile, a round costs b/3 = 0.5 ms; a dirty megapixel c = 0.05 ms
+    assert want == [2, 1, 0]
+    # an unphysical fit (time falling with more queue work) keeps the image rule
+    assert learned.cost_model(a + b * np.arange(5) - c * D, D) is None
+
+
 def test_tile_sums():
     flags = np.arange(6 * 5).reshape(6, 5)
     s = learned._tile_sums(flags, 128)  # 2 x 2 megapixels per tile
