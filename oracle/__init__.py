@@ -34,6 +34,12 @@ def lib():
         L.oracle_pass_y.argtypes = [vp, vp, vp, sz, sz, sz, i, i, i, i, i, i, i]
         L.oracle_unstable_blocks.argtypes = [vp, vp, sz, sz, i, i, i, i, i, i, vp]
         L.oracle_unstable_blocks.restype = ctypes.c_long
+        L.qm_reconstruct.argtypes = [vp, vp, sz, sz, i, i, i, i, i, i, i, ctypes.c_uint64,
+                                     ctypes.c_long, i, i, ctypes.POINTER(QMStats)]
+        L.qm_dq_fill_swap_kat.argtypes = [i]
+        L.qm_dq_fill_swap_kat.restype = ctypes.c_long
+        L.qm_uq_unique_kat.argtypes = [i, i, ctypes.c_uint64]
+        L.qm_uq_unique_kat.restype = ctypes.c_long
         _LIB = L
     return _LIB
 
@@ -98,6 +104,46 @@ def pass_y(src, mask, conn, direction, block_w, block_h):
                                _pitch(D), _pitch(I), w, h, _dt(S), conn, direction,
                                block_w, block_h), "oracle_pass_y")
     return D
+
+
+class QMStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in
+                ("visits", "seeded", "rounds", "restarts", "overlap", "repeat_in_round",
+                 "state_left", "steps")]
+
+
+QUEUE_VARIANTS = {"mutex": 0, "dq": 1, "uq": 2}
+
+
+QUEUE_FAULTS = {None: 0, "mutex_drop_renotify": 1, "uq_keep_flag": 2}
+
+
+def queue_model(marker, mask, conn, variant, mp=8, workers=4, seed=1, cap=0, max_sweeps=0,
+                fault=None):
+    """The CPU queue-phase model (oracle/queue_model.c): reconstruction through
+    a queue variant with simulated concurrent workers.  fault plants a
+    protocol mistake (tests of the checks).  Returns (new array, stats dict)."""
+    J = np.ascontiguousarray(marker).copy()
+    I = np.ascontiguousarray(mask)
+    h, w = J.shape
+    st = QMStats()
+    _check(lib().qm_reconstruct(J.ctypes.data, I.ctypes.data, _pitch(J), _pitch(I), w, h, _dt(J),
+                                conn, QUEUE_VARIANTS[variant], mp, workers, seed, cap,
+                                max_sweeps, QUEUE_FAULTS[fault], ctypes.byref(st)),
+           "qm_reconstruct")
+    return J, {k: getattr(st, k) for k, _ in QMStats._fields_}
+
+
+def queue_fill_swap_kat(g):
+    """Coordinates not appearing exactly twice after the reference's double
+    queue fill/swap sequence, on the model's queue operations (0 = pass)."""
+    return _check(lib().qm_dq_fill_swap_kat(g), "qm_dq_fill_swap_kat")
+
+
+def queue_unique_kat(g, dup, seed=1):
+    """Flags left set plus coordinates not exactly once after extracting the
+    unique entries of a duplicated write queue (0 = pass)."""
+    return _check(lib().qm_uq_unique_kat(g, dup, seed), "qm_uq_unique_kat")
 
 
 def unstable_blocks(marker, mask, conn, mp=64):

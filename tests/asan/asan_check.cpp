@@ -19,6 +19,7 @@
 #include "iwpp_synth.h"
 extern "C" {
 #include "iwpp_oracle.h"
+#include "queue_model.h"
 }
 
 static int failures = 0;
@@ -154,6 +155,38 @@ static void sched_checks() {
     CHECK(!iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024), &q, &r));
 }
 
+// The CPU queue-phase model: the reference's queue KATs, and every variant
+// over ragged shapes, both pixel types, small queue capacities (overflow and
+// restart) and capped local sweeps, against the oracle.
+template <class T>
+static void queue_model_checks(int dtype) {
+    CHECK(qm_dq_fill_swap_kat(1) == 0);
+    CHECK(qm_dq_fill_swap_kat(64) == 0);
+    CHECK(qm_uq_unique_kat(16, 3, 7) == 0);
+    const int shapes[][2] = {{1, 1}, {37, 1}, {1, 53}, {45, 29}, {64, 70}};
+    for (auto& s : shapes)
+        for (int conn : {4, 8})
+            for (int variant : {QM_MUTEX, QM_DQ, QM_UQ}) {
+                const int w = s[0], h = s[1];
+                std::vector<T> I(w * h), J(w * h);
+                for (auto& v : I) v = (T)(rnd() % (dtype == 1 ? 256 : 4096));
+                for (size_t k = 0; k < J.size(); ++k) J[k] = rnd() % 50 == 0 ? I[k] : 0;
+                std::vector<T> want = J, got = J;
+                CHECK(oracle_reconstruct(want.data(), I.data(), w * sizeof(T), w * sizeof(T), w, h,
+                                         dtype, conn) == 0);
+                qm_stats st;
+                CHECK(qm_reconstruct(got.data(), I.data(), w * sizeof(T), w * sizeof(T), w, h,
+                                     dtype, conn, variant, 8, 5, rnd(),
+                                     variant == QM_MUTEX ? 0 : 3, 4, QM_FAULT_NONE, &st) == 0);
+                CHECK(got == want);
+                CHECK(st.state_left == 0 && st.overlap == 0);
+            }
+    std::vector<T> z(4);
+    qm_stats st;
+    CHECK(qm_reconstruct(z.data(), z.data(), 2 * sizeof(T), 2 * sizeof(T), 2, 2, dtype, 6, 0, 8,
+                         1, 1, 0, 0, 0, &st) < 0);
+}
+
 int main(int argc, char** argv) {
     const std::string dir = argc > 1 ? argv[1] : ".";
     pgm_checks(dir);
@@ -161,6 +194,8 @@ int main(int argc, char** argv) {
     sched_checks();
     oracle_checks<uint8_t>(IWPP_U8);
     oracle_checks<uint16_t>(IWPP_U16);
+    queue_model_checks<uint8_t>(IWPP_U8);
+    queue_model_checks<uint16_t>(IWPP_U16);
     if (failures) {
         fprintf(stderr, "%d checks failed\n", failures);
         return 1;

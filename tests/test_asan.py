@@ -1,5 +1,5 @@
 """Host sanitizer run: the CPU-side code (PGM I/O, synthetic generators,
-schedule planner, oracle) built with AddressSanitizer and
+schedule planner, oracle, CPU queue-phase model) built with AddressSanitizer and
 UndefinedBehaviorSanitizer (tests/asan/) and driven over ragged shapes,
 malformed files and random cases.  CPU only."""
 import os
