@@ -101,6 +101,9 @@ typedef struct {
      * clear when the queue phase ends (must be 0). */
     long long repeat_visits;
     long long state_left;
+    int col_block_rows;      /* rows per block of the column passes used */
+    int seed_class;          /* seed class the autotuner cache was looked up with
+                                (-1: not measured, no cached tuning applies) */
 } iwpp_stats;
 
 /* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
@@ -181,16 +184,21 @@ int iwpp_batch_graph_stats(iwpp_graph* g, iwpp_stats* stats, void* stream);
 void iwpp_batch_graph_destroy(iwpp_graph* g);
 
 /* Autotuner.  Times every (queue variant, sweep rounds 0..4) on these very
- * images (best of `trials` runs each, HIP events), restores the input markers
- * afterwards, and remembers the winner for this workload class (dtype,
- * connectivity, single/batch, log2 pixels, log2 widest side): later calls
- * with queue = IWPP_QUEUE_AUTO and sweep_rounds = -1 use it.  No reference
- * counterpart (the reference fixes everything at build time). */
+ * images, then, for the winner, the column passes' block height (128, 256,
+ * 512, 1024 rows against the size rule; the sweep passes' tile), best of
+ * `trials` runs each (HIP events); restores the input markers afterwards,
+ * and remembers the winner for this workload class (dtype, connectivity,
+ * single/batch, log2 pixels, log2 widest side, seed class of the markers:
+ * sparse seeds / partial / dense): later calls with queue = IWPP_QUEUE_AUTO,
+ * sweep_rounds = -1 or col_block_rows = 0 use it.  No reference counterpart
+ * (the reference fixes everything at build time). */
 typedef struct {
     int queue;
     int sweep_rounds;
     int tile;
-    float ms;  /* best measured time of one reconstruction */
+    float ms;            /* best measured time of one reconstruction */
+    int col_block_rows;  /* 0 = the size rule's height won */
+    int seed_class;      /* 0 sparse seeds (< 1% of pixels), 1 partial, 2 dense */
 } iwpp_tuning;
 int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int connectivity, int trials,
                   const iwpp_options* opt, iwpp_tuning* best, iwpp_workspace* ws, void* stream);

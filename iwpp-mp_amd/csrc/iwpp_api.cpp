@@ -90,6 +90,8 @@ struct iwpp_workspace {
     size_t qb_n = 0;
     uint32_t* chk = nullptr;    // queue-discipline checks (per megapixel)
     size_t chk_n = 0;
+    unsigned long long* cls = nullptr;  // seed-class counters (autotuner key)
+    size_t cls_n = 0;
     unsigned long long* ring = nullptr;  // mutex ring
     size_t ring_n = 0;
     uint32_t* ctl = nullptr;    // control block
@@ -172,6 +174,7 @@ extern "C" void iwpp_workspace_destroy(iwpp_workspace* ws) {
     (void)hipFree(ws->qa);
     (void)hipFree(ws->qb);
     (void)hipFree(ws->chk);
+    (void)hipFree(ws->cls);
     (void)hipFree(ws->ring);
     (void)hipFree(ws->ctl);
     (void)hipFree(ws->tiles);
@@ -290,6 +293,8 @@ struct Run {
     uint32_t plan_seq = 0;
     int timed_rounds = 0;  // sweep rounds with per-pass events (ws->ev_round)
     bool scan_seed = false;  // seed from k_unstable_scan only (no strip-edge buffer)
+    int seed_class = -1;     // autotuner key of the markers (-1: not measured)
+    bool no_tune = false;    // the autotuner's own trials: no cached choices
 };
 
 int learned_plan_predict(Run& r);
@@ -958,38 +963,80 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     return IWPP_OK;
 }
 
+// Seed class of the run's markers (autotuner key), from every 64th row.
+int measure_seed_class(Run& r) {
+    iwpp_workspace* ws = r.ws;
+    int rc = ensure_ctl(ws);
+    if (rc) return rc;
+    HIP_TRY(grow(ws->cls, ws->cls_n, 2));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(ws->cls), 0u, 4, r.s));
+    Descs ds = r.ds;
+    HIP_TRY(launch_marker_class(ds, r.dtype, ws->cls, r.s));
+    HIP_TRY(hipMemcpyAsync(ws->host_ctl, ws->cls, 2 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, r.s));
+    HIP_TRY(hipStreamSynchronize(r.s));
+    const unsigned long long* h = reinterpret_cast<const unsigned long long*>(ws->host_ctl);
+    r.seed_class = seed_class(h[0], h[1]);
+    return IWPP_OK;
+}
+
+// Plan, then the autotuner's measured choices for this workload class over
+// the heuristic defaults (queue = AUTO, sweep_rounds < 0, col_block_rows = 0),
+// then the images' descriptors (which depend on the column block height).
+int plan_and_setup(Run& r, const iwpp_image* imgs, int n, int tile, int queue, int schedule,
+                   SchedulePlan* plan) {
+    int maxw = 0, maxh = 0;
+    long long px = 0;
+    for (int i = 0; i < n; ++i) {
+        maxw = std::max(maxw, imgs[i].width);
+        maxh = std::max(maxh, imgs[i].height);
+        px += (long long)imgs[i].width * imgs[i].height;
+    }
+    if (plan_schedule(n == 1 ? imgs[0].width : maxw, n == 1 ? imgs[0].height : maxh, r.dtype,
+                      r.conn, tile, queue, schedule, r.opt.sweep_rounds, plan))
+        return fail(IWPP_ERR_ARG, "no schedule for this configuration");
+    bool ready = false;
+    if (!r.no_tune && tune_any() &&
+        (queue == IWPP_QUEUE_AUTO || r.opt.sweep_rounds < 0 || r.opt.col_block_rows == 0)) {
+        int rc = setup_images(r, imgs, n);
+        if (rc) return rc;
+        ready = true;
+        if (r.seed_class < 0 && !r.capturing) {
+            rc = measure_seed_class(r);
+            if (rc) return rc;
+        }
+        TuneValue tv{};
+        if (r.seed_class >= 0 &&
+            tune_lookup(tune_key(r.dtype, r.conn, n, px, std::max(maxw, maxh), r.seed_class), &tv)) {
+            if (queue == IWPP_QUEUE_AUTO) plan->queue = r.graph ? Q_MUTEX : tv.queue;
+            if (r.opt.sweep_rounds < 0) plan->sweep_rounds = tv.sweep_rounds;
+            if (r.opt.col_block_rows == 0 && tv.col_block_rows > 0) {
+                r.opt.col_block_rows = tv.col_block_rows;
+                ready = false;  // block height changed: descriptors again
+            }
+        }
+    }
+    r.queue = plan->queue;
+    if (!ready) return setup_images(r, imgs, n);
+    return IWPP_OK;
+}
+
 // Shared body of the single-image and batch entry points.  With graph = true
 // only the launches are issued (nothing synchronises, no restart).
 int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, int schedule,
                     const iwpp_options* opt) {
     iwpp_default_options(&r.opt);
     if (opt) r.opt = *opt;
-    int maxw = 0, maxh = 0;
-    for (int i = 0; i < n; ++i) {
-        maxw = std::max(maxw, imgs[i].width);
-        maxh = std::max(maxh, imgs[i].height);
-    }
     SchedulePlan plan;
-    int rc = plan_schedule(n == 1 ? imgs[0].width : maxw, n == 1 ? imgs[0].height : maxh,
-                           r.dtype, r.conn, tile, queue, schedule, r.opt.sweep_rounds, &plan);
-    if (rc) return fail(IWPP_ERR_ARG, "no schedule for this configuration");
-    {  // measured choices of the autotuner override the heuristic defaults
-        long long px = 0;
-        for (int i = 0; i < n; ++i) px += (long long)imgs[i].width * imgs[i].height;
-        int tq = 0, tsw = 0;
-        if (tune_lookup(tune_key(r.dtype, r.conn, n, px, std::max(maxw, maxh)), &tq, &tsw)) {
-            if (queue == IWPP_QUEUE_AUTO) plan.queue = r.graph ? Q_MUTEX : tq;
-            if (r.opt.sweep_rounds < 0) plan.sweep_rounds = tsw;
-        }
-    }
-    r.queue = plan.queue;
+    int rc = plan_and_setup(r, imgs, n, tile, queue, schedule, &plan);
+    if (rc) return rc;
     r.st.queue = plan.queue;
     r.st.tile = plan.tile;
     r.st.schedule = IWPP_SCHED_PLAIN;
+    r.st.col_block_rows = r.h.empty() ? 0 : r.h[0].R;
+    r.st.seed_class = r.seed_class;
     if (r.graph && r.queue != Q_MUTEX)
         return fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
-    rc = setup_images(r, imgs, n);
-    if (rc) return rc;
     bool planning = false;
     // The learned plan costs about 30-50 us (tile features, network, and the
     // host planning between the first downward and upward column passes,
@@ -1151,23 +1198,20 @@ extern "C" iwpp_graph* iwpp_batch_graph_create(const iwpp_image* imgs, int n, in
         iwpp_default_options(&dry.opt);
         if (opt) dry.opt = *opt;
         SchedulePlan plan;
-        int maxw = 0, maxh = 0;
-        for (int i = 0; i < n; ++i) {
-            maxw = std::max(maxw, imgs[i].width);
-            maxh = std::max(maxh, imgs[i].height);
+        if (plan_and_setup(dry, imgs, n, tile, queue, schedule, &plan)) {
+            delete g;
+            return nullptr;
         }
-        if (plan_schedule(maxw, maxh, dtype, conn, tile, queue, schedule, dry.opt.sweep_rounds,
-                          &plan) ||
-            plan.queue != Q_MUTEX) {
+        if (plan.queue != Q_MUTEX) {
             fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
             delete g;
             return nullptr;
         }
-        dry.queue = plan.queue;
-        if (setup_images(dry, imgs, n) || setup_queue(dry)) {
+        if (setup_queue(dry)) {
             delete g;
             return nullptr;
         }
+        r.seed_class = dry.seed_class;  // the capture takes the same tuned choices
     }
     if (s == nullptr) {
         fail(IWPP_ERR_ARG, "graph capture needs a stream other than the NULL stream");
@@ -1273,32 +1317,65 @@ extern "C" int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int conn,
     hipError_t he = copy_all(true);
     if (he == hipSuccess) he = hipEventCreate(&e0);
     if (he == hipSuccess) he = hipEventCreate(&e1);
-    iwpp_tuning top{IWPP_QUEUE_MUTEX, 0, 0, 1e30f};
+    // the workload's seed class, for the cache key
+    int sclass = 0;
+    if (he == hipSuccess) {
+        Run r0{};
+        r0.dtype = dtype;
+        r0.conn = conn;
+        r0.ws = ws;
+        r0.s = s;
+        iwpp_default_options(&r0.opt);
+        rc = setup_images(r0, imgs, n);
+        if (rc == IWPP_OK) rc = measure_seed_class(r0);
+        sclass = r0.seed_class;
+    }
+    // One trial configuration: best of `trials` timed reconstructions.
+    auto trial = [&](int queue, int sw, int rows) -> float {
+        float best_ms = 1e30f;
+        for (int t = 0; t < std::max(1, trials) && rc == IWPP_OK && he == hipSuccess; ++t) {
+            if ((he = copy_all(false)) != hipSuccess) break;
+            if ((he = hipEventRecord(e0, s)) != hipSuccess) break;
+            Run r{};
+            r.dtype = dtype;
+            r.conn = conn;
+            r.ws = ws;
+            r.s = s;
+            r.no_tune = true;
+            iwpp_options o;
+            iwpp_default_options(&o);
+            if (opt) o = *opt;
+            o.sweep_rounds = sw;
+            o.col_block_rows = rows;
+            rc = reconstruct_run(r, imgs, n, 0, queue, IWPP_SCHED_PLAIN, &o);
+            if (rc) break;
+            if ((he = hipEventRecord(e1, s)) != hipSuccess) break;
+            if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
+            float ms = 0;
+            if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            best_ms = std::min(best_ms, ms);
+        }
+        return best_ms;
+    };
+    // Stage 1: queue variant x sweep rounds, column blocks from the size
+    // rule; stage 2: the column block height (the sweep passes' tile: how far
+    // a pass carries a rise before the queue takes over) for the winner.
+    iwpp_tuning top{IWPP_QUEUE_MUTEX, 0, 0, 1e30f, 0, sclass};
     const int queues[3] = {IWPP_QUEUE_MUTEX, IWPP_QUEUE_DQ, IWPP_QUEUE_UQ};
     for (int qi = 0; qi < 3 && he == hipSuccess && rc == IWPP_OK; ++qi) {
         for (int sw = 0; sw <= 4 && he == hipSuccess && rc == IWPP_OK; ++sw) {
-            float best_ms = 1e30f;
-            for (int t = 0; t < std::max(1, trials) && rc == IWPP_OK; ++t) {
-                if ((he = copy_all(false)) != hipSuccess) break;
-                if ((he = hipEventRecord(e0, s)) != hipSuccess) break;
-                Run r{};
-                r.dtype = dtype;
-                r.conn = conn;
-                r.ws = ws;
-                r.s = s;
-                iwpp_options o;
-                iwpp_default_options(&o);
-                if (opt) o = *opt;
-                o.sweep_rounds = sw;
-                rc = reconstruct_run(r, imgs, n, 0, queues[qi], IWPP_SCHED_PLAIN, &o);
-                if (rc) break;
-                if ((he = hipEventRecord(e1, s)) != hipSuccess) break;
-                if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
-                float ms = 0;
-                if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-                best_ms = std::min(best_ms, ms);
+            const float ms = trial(queues[qi], sw, 0);
+            if (ms < top.ms) top = iwpp_tuning{queues[qi], sw, 0, ms, 0, sclass};
+        }
+    }
+    if (top.sweep_rounds > 0) {
+        const int rows_c[4] = {128, 256, 512, 1024};
+        for (int k = 0; k < 4 && he == hipSuccess && rc == IWPP_OK; ++k) {
+            const float ms = trial(top.queue, top.sweep_rounds, rows_c[k]);
+            if (ms < 0.98f * top.ms) {  // a clear win over the size rule's height
+                top.ms = ms;
+                top.col_block_rows = rows_c[k];
             }
-            if (best_ms < top.ms) top = iwpp_tuning{queues[qi], sw, 0, best_ms};
         }
     }
     if (he == hipSuccess) he = copy_all(false);
@@ -1312,7 +1389,8 @@ extern "C" int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int conn,
     plan_schedule(maxside, maxside, dtype, conn, 0, top.queue, IWPP_SCHED_PLAIN, top.sweep_rounds,
                   &plan);
     top.tile = plan.tile;
-    tune_store(tune_key(dtype, conn, n, px, maxside), top.queue, top.sweep_rounds);
+    tune_store(tune_key(dtype, conn, n, px, maxside, sclass),
+               TuneValue{top.queue, top.sweep_rounds, top.col_block_rows});
     if (best) *best = top;
     return IWPP_OK;
 }
@@ -1494,6 +1572,7 @@ extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, i
     r.conn = conn;
     r.ws = ws;
     r.queue = Q_MUTEX;
+    r.no_tune = true;
     iwpp_default_options(&r.opt);
     r.opt.col_block_rows = block_rows;
     HIP_TRY(call_stream(ws, stream, &r.s));
@@ -1610,6 +1689,7 @@ extern "C" int iwpp_reconstruct_noalloc(void* marker, const void* mask, size_t p
                                 // ordinary launch)
     r.timed = false;
     r.scan_seed = true;
+    r.no_tune = true;  // (measuring the seed class would need scratch of its own)
     const iwpp_image img{marker, mask, pm, pk, w, h};
     NoAllocScope guard;
     rc = reconstruct_run(r, &img, 1, 0, IWPP_QUEUE_MUTEX, IWPP_SCHED_PLAIN, &o);

@@ -175,6 +175,10 @@ hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s);
 hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const QueueArgs& q,
                              hipStream_t s);
 hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s);
+// Autotuner key: counts[0] += pixels with min(marker, mask) > 0, counts[1] +=
+// pixels sampled, over every 64th row of every image (counts zeroed before).
+hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,
+                               hipStream_t s);
 // Counts the nonzero words of q.st[0..nmp) into C_STATE_LEFT (queue checks).
 hipError_t launch_state_left(const QueueArgs& q, hipStream_t s);
 hipError_t launch_queue_phase(const Descs& ds, int dtype, int conn, int kind,

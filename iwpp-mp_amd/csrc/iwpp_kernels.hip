@@ -2517,6 +2517,28 @@ __global__ __launch_bounds__(kWG) void k_state_left(QueueArgs q) {
     if (n) a_add(ctl_at(q.ctl, C_STATE_LEFT), n);  // (none when the phase ended clean)
 }
 
+// Seed class of the markers (autotuner key): over every 64th row of every
+// image, the pixels where min(marker, mask) > 0 and the pixels sampled
+// (counts[0], counts[1]).  Block (i, b) samples image i's rows b, b + nb, ...
+template <class T>
+__global__ __launch_bounds__(kWG) void k_marker_class(Descs ds, unsigned long long* counts) {
+    const ImgDesc d = desc(ds, (int)blockIdx.x);
+    unsigned long long nz = 0, n = 0;
+    for (int r = (int)blockIdx.y; r * 64 < d.H; r += (int)gridDim.y) {
+        const int y = r * 64;
+        const T* J = reinterpret_cast<const T*>(d.J + (size_t)y * d.pj);
+        const T* I = reinterpret_cast<const T*>(d.I + (size_t)y * d.pi);
+        for (int x = (int)threadIdx.x; x < d.W; x += kWG) {
+            nz += min(*gp(J + x), *gp(I + x)) > 0;
+            ++n;
+        }
+    }
+    if (n) {
+        __hip_atomic_fetch_add(gp(counts), nz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gp(counts + 1), n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // ============================================================ launchers ====
 static int cu_count() {
     static int n = 0;
@@ -2765,6 +2787,14 @@ hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const int grid = (int)std::min<size_t>((n + kWG - 1) / kWG, (size_t)cu_count() * 8);
     hipLaunchKernelGGL(k_ring_init, grid, kWG, 0, s, ring, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,
+                               hipStream_t s) {
+    const dim3 grid((unsigned)ds.n, 16);
+    if (dtype == 1) hipLaunchKernelGGL((k_marker_class<uint8_t>), grid, kWG, 0, s, ds, counts);
+    else hipLaunchKernelGGL((k_marker_class<uint16_t>), grid, kWG, 0, s, ds, counts);
     return hipGetLastError();
 }
 

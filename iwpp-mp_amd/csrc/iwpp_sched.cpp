@@ -19,7 +19,7 @@ namespace iwpp {
 
 namespace {
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tune;
+std::map<TuneKey, TuneValue> g_tune;
 int ilog2(long long v) {
     int k = 0;
     while (v > 1) {
@@ -31,23 +31,31 @@ int ilog2(long long v) {
 }  // namespace
 
 bool TuneKey::operator<(const TuneKey& o) const {
-    return std::tie(dtype, conn, batch, log2_px, log2_side) <
-           std::tie(o.dtype, o.conn, o.batch, o.log2_px, o.log2_side);
+    return std::tie(dtype, conn, batch, log2_px, log2_side, seed_class) <
+           std::tie(o.dtype, o.conn, o.batch, o.log2_px, o.log2_side, o.seed_class);
 }
-TuneKey tune_key(int dtype, int conn, int n, long long px, int max_side) {
-    return TuneKey{dtype, conn, n > 1 ? 1 : 0, ilog2(px), ilog2(max_side)};
+int seed_class(unsigned long long nonzero, unsigned long long sampled) {
+    if (sampled == 0) return 0;
+    if (nonzero * 100 < sampled) return 0;
+    return nonzero * 2 < sampled ? 1 : 2;
 }
-void tune_store(const TuneKey& k, int queue, int sweep_rounds) {
+TuneKey tune_key(int dtype, int conn, int n, long long px, int max_side, int sc) {
+    return TuneKey{dtype, conn, n > 1 ? 1 : 0, ilog2(px), ilog2(max_side), sc};
+}
+void tune_store(const TuneKey& k, const TuneValue& v) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[k] = {queue, sweep_rounds};
+    g_tune[k] = v;
 }
-bool tune_lookup(const TuneKey& k, int* queue, int* sweep_rounds) {
+bool tune_lookup(const TuneKey& k, TuneValue* v) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(k);
     if (it == g_tune.end()) return false;
-    *queue = it->second.first;
-    *sweep_rounds = it->second.second;
+    *v = it->second;
     return true;
+}
+bool tune_any() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return !g_tune.empty();
 }
 void tune_clear() {
     std::lock_guard<std::mutex> lk(g_tune_mu);

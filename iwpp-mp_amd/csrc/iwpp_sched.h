@@ -16,16 +16,26 @@ struct SchedulePlan {
 int plan_schedule(int w, int h, int dtype, int conn, int tile, int queue, int schedule,
                   int sweep_rounds, SchedulePlan* out);
 
-// Autotuner cache: the measured best (queue, sweep rounds) for a workload key
-// (dtype, connectivity, single image or batch, log2 of the pixel count, log2 of
-// the widest side).  plan_schedule uses it for queue = AUTO / sweep_rounds < 0.
+// Autotuner cache: the measured best (queue, sweep rounds, column block
+// height) for a workload key (dtype, connectivity, single image or batch,
+// log2 of the pixel count, log2 of the widest side, seed class).  The
+// reconstruction uses it for queue = AUTO / sweep_rounds < 0 /
+// col_block_rows = 0.
 struct TuneKey {
-    int dtype, conn, batch, log2_px, log2_side;
+    int dtype, conn, batch, log2_px, log2_side, seed_class;
     bool operator<(const TuneKey& o) const;
 };
-TuneKey tune_key(int dtype, int conn, int n, long long px, int max_side);
-void tune_store(const TuneKey& k, int queue, int sweep_rounds);
-bool tune_lookup(const TuneKey& k, int* queue, int* sweep_rounds);
+struct TuneValue {
+    int queue, sweep_rounds, col_block_rows;
+};
+// Seed class of a marker from the fraction f of (sampled) pixels where
+// min(marker, mask) > 0: 0 = sparse seeds (f < 1%), 1 = partial (f < 50%),
+// 2 = dense (h-dome-like markers).
+int seed_class(unsigned long long nonzero, unsigned long long sampled);
+TuneKey tune_key(int dtype, int conn, int n, long long px, int max_side, int seed_class);
+void tune_store(const TuneKey& k, const TuneValue& v);
+bool tune_lookup(const TuneKey& k, TuneValue* v);
+bool tune_any();
 void tune_clear();
 
 }  // namespace iwpp

@@ -59,7 +59,8 @@ class Stats(ctypes.Structure):
                 ("us_converge", ctypes.c_double), ("us_store", ctypes.c_double),
                 ("us_notify", ctypes.c_double), ("ms_rows", ctypes.c_float),
                 ("ms_cols", ctypes.c_float), ("repeat_visits", ctypes.c_longlong),
-                ("state_left", ctypes.c_longlong)]
+                ("state_left", ctypes.c_longlong), ("col_block_rows", ctypes.c_int),
+                ("seed_class", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -348,7 +349,8 @@ def reconstruct_batch(pairs, connectivity=8, tile=0, queue="auto", schedule="pla
 
 class Tuning(ctypes.Structure):
     _fields_ = [("queue", ctypes.c_int), ("sweep_rounds", ctypes.c_int), ("tile", ctypes.c_int),
-                ("ms", ctypes.c_float)]
+                ("ms", ctypes.c_float), ("col_block_rows", ctypes.c_int),
+                ("seed_class", ctypes.c_int)]
 
 
 def autotune(pairs, connectivity=8, trials=2, options=None, workspace=None, stream=None):
@@ -361,7 +363,8 @@ def autotune(pairs, connectivity=8, trials=2, options=None, workspace=None, stre
                                ctypes.byref(t), workspace.handle if workspace else None,
                                _stream(stream)))
     inv = {v: k for k, v in QUEUES.items()}
-    return {"queue": inv[t.queue], "sweep_rounds": t.sweep_rounds, "tile": t.tile, "ms": t.ms}
+    return {"queue": inv[t.queue], "sweep_rounds": t.sweep_rounds, "tile": t.tile, "ms": t.ms,
+            "col_block_rows": t.col_block_rows, "seed_class": t.seed_class}
 
 
 class BatchGraph:

@@ -148,11 +148,15 @@ static void sched_checks() {
             CHECK(iwpp::plan_schedule(w, w, IWPP_U8, 8, 0, q, IWPP_SCHED_PLAIN, -1, &p) == 0);
             CHECK(p.tile % 64 == 0 && p.sweep_rounds >= 0);
         }
-    iwpp::tune_store(iwpp::tune_key(1, 8, 1, 1 << 20, 1024), 2, 3);
-    int q = -1, r = -1;
-    CHECK(iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024), &q, &r) && q == 2 && r == 3);
+    iwpp::tune_store(iwpp::tune_key(1, 8, 1, 1 << 20, 1024, 0), iwpp::TuneValue{2, 3, 256});
+    iwpp::TuneValue tv{};
+    CHECK(iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024, 0), &tv) && tv.queue == 2 &&
+          tv.sweep_rounds == 3 && tv.col_block_rows == 256);
+    CHECK(!iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024, 2), &tv));  // other seed class
+    CHECK(iwpp::seed_class(0, 100) == 0 && iwpp::seed_class(5, 100) == 1 &&
+          iwpp::seed_class(60, 100) == 2 && iwpp::seed_class(0, 0) == 0);
     iwpp::tune_clear();
-    CHECK(!iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024), &q, &r));
+    CHECK(!iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024, 0), &tv));
 }
 
 // The CPU queue-phase model: the reference's queue KATs, and every variant

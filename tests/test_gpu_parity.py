@@ -392,9 +392,38 @@ def test_autotune_restores_markers_and_tunes(torch_cuda):
     assert best["queue"] in ("mutex", "dq", "uq") and 0 <= best["sweep_rounds"] <= 4
     for (dj, _), b in zip(pairs, before):
         assert np.array_equal(dj.numpy(), b)
+    assert best["seed_class"] in (0, 1, 2) and best["col_block_rows"] in (0, 128, 256, 512, 1024)
     st = iwpp.reconstruct_batch(pairs, 8, 0, "auto")
     assert st["queue"] == {"mutex": 0, "dq": 1, "uq": 2}[best["queue"]]
     assert st["sweep_rounds"] == best["sweep_rounds"]
+    assert st["seed_class"] == best["seed_class"]
+    if best["col_block_rows"]:
+        assert st["col_block_rows"] == best["col_block_rows"], (st, best)
     for (J, I), (dj, _) in zip(cases, pairs):
         assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, 8))
     iwpp.lib().iwpp_autotune_clear()
+
+
+def test_autotune_keys_on_seed_class(torch_cuda):
+    # two workloads of the same sizes, one of sparse seeds and one of h-dome
+    # markers, get separate cache entries; each later 'auto' call finds its own
+    rng = np.random.default_rng(14)
+    shapes = [(300, 400), (256, 512)]
+    seeds = [rand_pair(rng, h, w, np.uint8, "seeds") for h, w in shapes]
+    hdome = [rand_pair(rng, h, w, np.uint8, "hdome") for h, w in shapes]
+    got = {}
+    for name, cases in (("seeds", seeds), ("hdome", hdome)):
+        pairs = [(up(J), up(I)) for J, I in cases]
+        say("autotune seed class", name)
+        got[name] = iwpp.autotune(pairs, 8, 1)
+    assert got["seeds"]["seed_class"] == 0 and got["hdome"]["seed_class"] == 2, got
+    try:
+        for name, cases in (("seeds", seeds), ("hdome", hdome)):
+            pairs = [(up(J), up(I)) for J, I in cases]
+            st = iwpp.reconstruct_batch(pairs, 8, 0, "auto")
+            assert st["seed_class"] == got[name]["seed_class"], (name, st)
+            assert st["sweep_rounds"] == got[name]["sweep_rounds"], (name, st, got)
+            for (J, I), (dj, _) in zip(cases, pairs):
+                assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, 8))
+    finally:
+        iwpp.lib().iwpp_autotune_clear()
