@@ -560,6 +560,242 @@ __global__ __launch_bounds__(NWV * 64) void k_row_pass(Image im, TileMask tm, ui
     if (any_change && changed) *changed = 1u;
 }
 
+// Row pass with the whole row held in registers by one 256-lane workgroup:
+// NCH chunks of 256 lanes x 16 bytes, read once, swept forward and backward,
+// written once (only the vectors that changed).  A direction takes one
+// barrier: every (chunk, wave) pair publishes its summary (P, Q) in LDS, then
+// each wave folds the summaries that precede it in scan order to get the
+// value entering each of its chunks; within a wave the lanes' exclusive
+// prefixes come from a DPP scan made before the barrier.  All of a row's
+// loads are issued together, with no branch around them (a lane with nothing
+// to read re-reads the row's first vector, and its data is replaced by
+// zeros), so they are in flight at once.  Pixel handling (walls for tiles
+// that are off, clipping, packed pairs) is LanePx's, as in k_row_pass.  A
+// wave's part of a chunk whose tiles are all off is not read or written
+// (outside the clipping round): zero marker and mask make it a wall.
+// An empty asm that claims to change x: values the compiler derived from x
+// earlier are then recomputed from it rather than kept alive in registers
+// until their next use.
+DEV void reg_fence(uint32_t& x) { asm volatile("" : "+v"(x)); }
+template <int NCH, int NW>
+DEV void reg_fence(uint32_t (&a)[NCH][NW], int ch) {
+#pragma unroll
+    for (int i = 0; i < NW; ++i) reg_fence(a[ch][i]);
+}
+// Inclusive scan, across the lanes of a wave, of NE (<= 64) summaries
+// (P, Q) stored in LDS in row order; lane l holds the l-th in scan
+// direction D.  Returns P: with the value 0 entering the row, the value
+// leaving the l-th summary.
+template <int NE, int D>
+DEV uint32_t entry_scan(const uint32_t* sp, const uint32_t* sq, int lane) {
+    uint32_t P = 0, Q = 0xFFFFFFFFu;
+    if (lane < NE) {
+        const int e = D > 0 ? lane : NE - 1 - lane;
+        P = sp[e];
+        Q = sq[e];
+    }
+    scan_lanes_fwd(P, Q);
+    return P;
+}
+DEV void ld16(const uint8_t* base, uint32_t off, uint32_t (&w)[4]) {
+    const u32x4 t = *gp(reinterpret_cast<const u32x4*>(base + off));
+    w[0] = t.x;
+    w[1] = t.y;
+    w[2] = t.z;
+    w[3] = t.w;
+}
+
+template <class T, int NT, int NCH, bool CLIP>
+__global__ __launch_bounds__(NT) void k_row_pass_held(Image im, TileMask tm, uint32_t* changed) {
+    using L = LanePx<T, 16>;
+    constexpr int NWV = NT / 64, PPL = 16 / (int)sizeof(T);  // pixels per lane vector
+    static_assert(NCH <= 16 && NCH * NWV <= 64, "chunk flags fit a mask, entries a wave");
+    __shared__ uint32_t sP[2][NCH][NWV], sQ[2][NCH][NWV];
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    bool any_change = false;
+    for (int y = blockIdx.x; y < im.H; y += gridDim.x) {
+        // The thread index and the row length pass through opaque copies in
+        // every iteration: hoisted out of the loop, the per-lane offsets and
+        // conditions derived from them would stay live in registers across
+        // the whole kernel.
+        int tid = threadIdx.x, rowbytes = im.W * (int)sizeof(T);
+        asm volatile("" : "+v"(tid));
+        asm volatile("" : "+s"(rowbytes));
+        const int lane = tid & 63;
+        uint8_t* jr = im.J + (size_t)y * im.pj;
+        const uint8_t* ir = im.I + (size_t)y * im.pi;
+        // per-chunk lane bits: `read` = the vector is read, `wall` = no
+        // conduction through it
+        uint32_t read = 0, wall = 0;
+        if (tm.active) {
+            const uint8_t* trow = tm.active + (size_t)(y / tm.th) * tm.ntx;
+            uint32_t on = 0;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                // every lane loads (no branch around the loads)
+                const int x0 = (ch * NT + tid) * PPL;
+                const uint8_t f = *gp(trow + min(x0, im.W - 1) / tm.tw);
+                if (x0 < im.W && f != 0) on |= 1u << ch;
+            }
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                const bool inside = (ch * NT + tid) * 16 < rowbytes;
+                const bool lane_on = on >> ch & 1u;
+                const bool wave_on = wave_any(lane_on);  // (every lane votes)
+                if (inside && (CLIP || wave_on)) read |= 1u << ch;
+                if (!lane_on) wall |= 1u << ch;
+            }
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+                if ((ch * NT + tid) * 16 < rowbytes) read |= 1u << ch;
+            wall = ~read;
+        }
+        uint32_t jv[NCH][4], iv[NCH][4];
+        uint32_t xp[NCH], xq[NCH];  // lane-exclusive prefix within the wave, then the value in
+        uint32_t hpq[NCH];          // summary of the lane's first-scanned half (packed pairs)
+        uint32_t dch = 0;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const uint32_t o = (read >> ch & 1u) ? (uint32_t)(ch * NT + tid) * 16u : 0u;
+            ld16(jr, o, jv[ch]);
+            ld16(ir, o, iv[ch]);
+        }
+        // The chunk flags, re-read opaquely at each use: otherwise the
+        // compiler splits them into one register per chunk up front.
+        const auto bit = [](uint32_t m, int ch) {
+            asm volatile("" : "+v"(m));
+            return m >> ch & 1u;
+        };
+        // ---- A: clip, walls, forward summaries of every chunk
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            reg_fence(jv, ch);
+            reg_fence(iv, ch);
+            {
+                // vectors not read, and the bytes of the last vector past the
+                // row's end, become zero (marker and mask), so they are walls
+                const int lim = bit(read, ch) ? rowbytes - (ch * NT + tid) * 16 : 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int vb = lim - 4 * i;
+                    const uint32_t keep = vb >= 4 ? 0xFFFFFFFFu
+                                        : vb <= 0 ? 0u : (0xFFFFFFFFu >> (8 * (4 - vb)));
+                    jv[ch][i] &= keep;
+                    iv[ch][i] &= keep;
+                }
+            }
+            L px;
+            px.load(jv[ch], iv[ch]);
+            if (CLIP && px.clip()) {  // the clipped marker is kept (and written back)
+                px.store(jv[ch]);
+                dch |= 1u << ch;
+            }
+            // a wall's mask is zero from here on: no rise, no conduction
+            const uint32_t cm = bit(wall, ch) ? 0u : 0xFFFFFFFFu;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) iv[ch][i] &= cm;
+#pragma unroll
+            for (int k = 0; k < L::NP; ++k) px.c[k] &= cm;
+            uint32_t Pi, Qi;
+            px.template reduce<1>(Pi, Qi);
+            hpq[ch] = px.hp | px.hq << 16;
+            scan_lanes_fwd(Pi, Qi);
+            xp[ch] = from_left(Pi, 0u);
+            xq[ch] = from_left(Qi, 0xFFFFFFFFu);
+            if (lane == 63) {
+                sP[0][ch][wv] = Pi;
+                sQ[0][ch][wv] = Qi;
+            }
+            reg_fence(xp[ch]);  // one chunk at a time: fewer live registers
+            reg_fence(xq[ch]);
+            reg_fence(hpq[ch]);
+        }
+        __syncthreads();
+        {
+            const uint32_t inc = entry_scan<NCH * NWV, 1>(&sP[0][0][0], &sQ[0][0][0], lane);
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                const int pos = ch * NWV + wv;  // this (chunk, wave) in scan order
+                const uint32_t xin = pos > 0 ? rdlane(inc, pos - 1) : 0u;
+                xp[ch] = max(xp[ch], min(xin, xq[ch]));
+            }
+        }
+        // ---- B: forward scan of every chunk, then its backward summaries
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            reg_fence(jv, ch);
+            reg_fence(iv, ch);
+            L px;
+            px.load(jv[ch], iv[ch]);
+            px.hp = hpq[ch] & 0xFFFFu;
+            px.hq = hpq[ch] >> 16;
+            const bool d = px.template apply<1>(xp[ch]);
+            uint32_t Pi, Qi;
+            px.template reduce<-1>(Pi, Qi);
+            hpq[ch] = px.hp | px.hq << 16;
+            if (d) {
+                px.store(jv[ch]);
+                dch |= 1u << ch;
+            }
+            scan_lanes_bwd(Pi, Qi, lane);
+            xp[ch] = from_right(Pi, 0u);
+            xq[ch] = from_right(Qi, 0xFFFFFFFFu);
+            if (lane == 0) {
+                sP[1][ch][wv] = Pi;
+                sQ[1][ch][wv] = Qi;
+            }
+            reg_fence(jv, ch);
+            reg_fence(xp[ch]);
+            reg_fence(xq[ch]);
+            reg_fence(hpq[ch]);
+        }
+        __syncthreads();
+        {
+            const uint32_t inc = entry_scan<NCH * NWV, -1>(&sP[1][0][0], &sQ[1][0][0], lane);
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                const int pos = NCH * NWV - 1 - (ch * NWV + wv);
+                const uint32_t xin = pos > 0 ? rdlane(inc, pos - 1) : 0u;
+                xp[ch] = max(xp[ch], min(xin, xq[ch]));
+            }
+        }
+        // ---- C: backward scan of every chunk
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            reg_fence(jv, ch);
+            reg_fence(iv, ch);
+            L px;
+            px.load(jv[ch], iv[ch]);
+            px.hp = hpq[ch] & 0xFFFFu;
+            px.hq = hpq[ch] >> 16;
+            if (px.template apply<-1>(xp[ch])) {
+                px.store(jv[ch]);
+                dch |= 1u << ch;
+            }
+            reg_fence(jv, ch);
+        }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            if (!(dch >> ch & 1u)) continue;
+            const int o = (ch * NT + tid) * 16;
+            if (o + 16 <= rowbytes) {
+                *gp(reinterpret_cast<u32x4*>(jr + o)) = u32x4{jv[ch][0], jv[ch][1], jv[ch][2], jv[ch][3]};
+            } else {  // the row's last, partial vector
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const int ob = o + k * (int)sizeof(T);
+                    if (ob < rowbytes)
+                        *gp(reinterpret_cast<T*>(jr + ob)) =
+                            (T)(jv[ch][k * (int)sizeof(T) / 4] >> (8 * ((k * (int)sizeof(T)) % 4)));
+                }
+            }
+        }
+        any_change |= dch != 0;
+    }
+    if (any_change && changed) *changed = 1u;
+}
+
 // Streaming row pass for wide rows: one chunk (256 lanes x VB bytes) in
 // registers at a time, carried along the row.  DIR = +1 / -1 sweeps one
 // direction; DIR = 0 sweeps forward then backward in the same launch, the
@@ -2591,19 +2827,33 @@ static void row_pass_wave(const Descs& ds, int dtype, bool clip, int rowbytes, u
 }
 constexpr int kWaveRowMax = 8192;
 
+constexpr int kHeldLanes = 256;
+
 template <class T, int VB, bool CLIP>
 static void row_pass_t(const Image& im, const TileMask& tm, uint32_t* changed, hipStream_t s) {
-    // u16 rows up to 32 KiB are held in registers (256 or 1024 lanes, 1-2
-    // chunks per lane); u8 rows beyond 4 KiB and anything wider are streamed
-    // chunk by chunk, forward and backward in one launch.  Measured on
-    // MI355X (packed-pair scans): 16384-wide u16 rows 3.9 TB/s in registers
-    // (1024 lanes x 2 chunks) and 3.9 TB/s streamed; 32768-wide u8 rows
-    // 4.5 TB/s streamed vs 3.0 TB/s in registers (1024 x 2), 2.1 TB/s
-    // (512 x 4) and 1.3 TB/s (256 x 8); 4096-wide u8 rows 2.6 TB/s in
-    // registers vs 2.4 TB/s streamed.
+    // Rows of 4-32 KiB (16-byte lanes) are held in registers by one
+    // 256-lane workgroup (k_row_pass_held): read once, written once.
+    // Measured on MI355X, one isolated pass, algorithmic bytes (3 per u8
+    // pixel, 6 per u16): 32768-wide u8 rows 0.580 ms = 5.6 TB/s held
+    // (512 lanes 5.3, 1024 lanes 4.9) against 0.705 ms = 4.6 TB/s streamed
+    // with the backward re-read; 16384-wide u16 rows 0.309 ms = 5.2 TB/s
+    // against 0.425 ms for the earlier 1024-lane form with a barrier per
+    // chunk; 6000-wide u8 rows 3.7 TB/s against 3.2 streamed.  Wider rows
+    // are streamed chunk by chunk, forward and backward in one launch.
     const int rowbytes = im.W * (int)sizeof(T);
     const int grid = std::min(im.H, 65535);
-    if (rowbytes <= 256 * VB)
+    if (VB == 16 && rowbytes > 256 * VB && rowbytes <= 32768) {
+        constexpr int NT = kHeldLanes, CB = NT * VB;  // lanes, bytes per chunk
+        if (rowbytes <= CB)
+            hipLaunchKernelGGL((k_row_pass_held<T, NT, 1, CLIP>), grid, NT, 0, s, im, tm, changed);
+        else if (rowbytes <= 2 * CB)
+            hipLaunchKernelGGL((k_row_pass_held<T, NT, 2, CLIP>), grid, NT, 0, s, im, tm, changed);
+        else if (rowbytes <= 4 * CB)
+            hipLaunchKernelGGL((k_row_pass_held<T, NT, 4, CLIP>), grid, NT, 0, s, im, tm, changed);
+        else  // (the 8-chunk form exists for the lane counts that can need it)
+            hipLaunchKernelGGL((k_row_pass_held<T, NT, NT <= 512 ? 8 : 4, CLIP>), grid, NT, 0, s, im,
+                               tm, changed);
+    } else if (rowbytes <= 256 * VB)
         hipLaunchKernelGGL((k_row_pass<T, VB, 1, 4, CLIP>), grid, 256, 0, s, im, tm, changed);
     else if (sizeof(T) == 2 && rowbytes <= 2 * 256 * VB)
         hipLaunchKernelGGL((k_row_pass<T, VB, 2, 4, CLIP>), grid, 256, 0, s, im, tm, changed);

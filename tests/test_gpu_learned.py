@@ -173,17 +173,41 @@ def _model_with_bias(seed, k_bias, prio_scale=1.0):
 MODELS = [("zero_rounds", -5.0), ("four_rounds", 9.0), ("mixed_a", 1.2), ("mixed_b", 2.4)]
 
 
+def _spread_model(marker, mask, tile, conn):
+    """A model whose rounds output spreads this image's tiles over 0.5..3.5
+    along the feature that differs most between them, so that later sweep
+    rounds run with tile masks."""
+    J, I = iwpp.DeviceImage.from_numpy(marker), iwpp.DeviceImage.from_numpy(mask)
+    _, feat = iwpp.tile_features(J, I, tile, conn, learned.FEATURE_SUB)
+    f = feat.cpu().numpy()
+    j = int(np.argmax(f.max(0) - f.min(0)))
+    lo, hi = float(f[:, j].min()), float(f[:, j].max())
+    s = 1.0 / max(hi - lo, 1e-6)
+    p = np.zeros(M.N_PARAMS, np.float32)
+    p[M.O_W1 + j] = s           # hidden unit 0 of layer 1: (x_j - lo) / (hi - lo)
+    p[M.O_B1] = -lo * s
+    p[M.O_W2] = 1.0             # unit 0 of layer 2 passes it on
+    p[M.O_W3] = 3.0             # rounds = 0.5 + 3 * unit
+    p[M.O_B3] = 0.5
+    return p
+
+
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("conn", [4, 8])
 @pytest.mark.parametrize("queue", ["mutex", "dq", "uq"])
 def test_learned_reconstruction_exact(torch_cuda, model_guard, dtype, conn, queue):
     rng = np.random.default_rng(zlib.crc32(f"{dtype.__name__}/{conn}/{queue}".encode()))
+    masked_wide = 0  # runs of the wide shape (rows held in registers) with tile masks
     for mi, (name, kb) in enumerate(MODELS):
         iwpp.set_model(_model_with_bias(mi + 3, kb))
         for shape, tile, kind in (((300, 517), 64, "seeds"), ((700, 390), 128, "hdome"),
-                                  ((129, 1000), 64, "seeds")):
+                                  ((129, 1000), 64, "seeds"), ((90, 5000), 256, "seeds")):
             say("learned", name, dtype.__name__, conn, queue, shape, tile, kind)
             marker, mask = rand_case(rng, *shape, dtype, kind)
+            if shape[1] == 5000:  # tiles that differ: h-dome marker on the left third
+                hd = np.maximum(mask.astype(np.int64) - int(mask.max()) // 5, 0)
+                marker[:, :1700] = hd[:, :1700].astype(dtype)
+                iwpp.set_model(_spread_model(marker, mask, tile, conn))
             want = oracle.reconstruct(marker, mask, conn)
             J, I = iwpp.DeviceImage.from_numpy(marker), iwpp.DeviceImage.from_numpy(mask)
             st = iwpp.reconstruct(J, I, connectivity=conn, tile=tile, queue=queue,
@@ -192,6 +216,8 @@ def test_learned_reconstruction_exact(torch_cuda, model_guard, dtype, conn, queu
             assert st["schedule"] == iwpp.SCHEDULES["learned"]
             got = J.numpy()
             assert np.array_equal(got, want), (name, shape, tile, kind, int((got != want).sum()))
+            masked_wide += shape[1] == 5000 and st["sweeps_skipped_tiles"] > 0
+    assert masked_wide > 0  # the mixed models skip tiles in later rounds
 
 
 def test_learned_without_model_is_plain(torch_cuda, model_guard):
