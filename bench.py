@@ -35,7 +35,8 @@ def log(*a):
 
 def profile_traffic(workload, prefixes, largest=False):
     """HBM bytes per launch of a kernel family from the committed rocprofv3
-    summary of this workload (profiles/<workload>_kernels.md: FETCH_SIZE and
+    summary of this workload and queue (profiles/<workload>_<queue>_kernels.md,
+    or profiles/<workload>_kernels.md for the batch: FETCH_SIZE and
     WRITE_SIZE passes, calibrated to bytes there), averaged over the family's
     rows weighted by their dispatches.  The out-of-place column pass (template
     mode 0) is the benchmark's isolated single-pass timing, not a launch of
@@ -47,8 +48,14 @@ def profile_traffic(workload, prefixes, largest=False):
         return None, f"no {os.path.relpath(path, ROOT)}"
     col = 8 if largest else 7
     num = den = 0.0
+    table = False  # inside the traffic table (the first one whose header names FETCH_SIZE)
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if not table:
+            table = cells[0] == "kernel" and any("FETCH_SIZE" in c for c in cells)
+            continue
+        if not line.strip():
+            break
         if len(cells) <= col or not cells[0].startswith("`"):
             continue
         name = cells[0].strip("`")
@@ -62,7 +69,7 @@ def profile_traffic(workload, prefixes, largest=False):
     if den == 0:
         return None, f"no {'/'.join(prefixes)} rows in {os.path.relpath(path, ROOT)}"
     what = "largest dispatch of each kernel" if largest else "median dispatch of each kernel"
-    return num / den, (f"{os.path.relpath(path, ROOT)}: (2.0 x FETCH_SIZE + WRITE_SIZE) of the "
+    return num / den, (f"{os.path.relpath(path, ROOT)}: calibrated FETCH_SIZE + WRITE_SIZE of the "
                        f"{what}, weighted by dispatches over the {'/'.join(prefixes)} kernels")
 
 
@@ -263,7 +270,7 @@ def main():
         "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
             (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_", "k_unstable_scan")),
     }
-    roofline, per_kernel = single_kernel_roofline(candidates, case.name)
+    roofline, per_kernel = single_kernel_roofline(candidates, f"{case.name}_{used_queue}")
     sweep_avg = phases["ms_sweep"] / max(1, 3 * rounds)
     roofline["sweep_family"] = {
         "kernels": "k_row_pass*, k_col_pass (every pass of the sweep phase)",
