@@ -107,6 +107,12 @@ constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHL2 = 0x102, DPP_ROW_SHL4 = 0x104,
 constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
 constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
+// The same with 0 for lanes whose source is out of range (bound_ctrl): a move
+// the compiler can fold into the VOP2 instruction that consumes it.
+template <int CTRL>
+DEV uint32_t dpp_z(uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)src, CTRL, 0xF, 0xF, true);
+}
 // lane i receives v[i-1]; lane 0 receives fill.
 DEV uint32_t from_left(uint32_t v, uint32_t fill) { return dpp<DPP_WAVE_SHR1>(fill, v); }
 // lane i receives v[i+1]; lane 63 receives fill.
@@ -437,6 +443,30 @@ DEV int find_cw(const Descs& a, int w) { return a.img_of_cw ? (int)a.img_of_cw[w
 DEV int find_sw(const Descs& a, int w) { return a.img_of_sw ? (int)a.img_of_sw[w] : search_sw(a, w); }
 DEV int find_mp(const Descs& a, int g) { return a.img_of_mp ? (int)a.img_of_mp[g] : 0; }
 DEV Image image_of(const ImgDesc& d) { return Image{d.J, d.I, d.pj, d.pi, d.W, d.H}; }
+DEV uint64_t first_lane64(uint64_t v) {
+    return (uint64_t)first_lane((uint32_t)v) | (uint64_t)first_lane((uint32_t)(v >> 32)) << 32;
+}
+// The descriptor of the image owning global megapixel m, for the queue
+// kernels: the kernel argument for a single image (ONE: scalar registers, no
+// load), else the device table, the fields a visit uses taken through
+// readfirstlane so that the address arithmetic built on them stays scalar.
+template <bool ONE>
+DEV ImgDesc mp_desc(const Descs& a, uint32_t m) {
+    if constexpr (ONE) {
+        return a.one;
+    } else {
+        ImgDesc d = desc(a, find_mp(a, (int)m));
+        d.J = (uint8_t*)first_lane64((uint64_t)d.J);
+        d.I = (const uint8_t*)first_lane64((uint64_t)d.I);
+        d.pj = (size_t)first_lane64((uint64_t)d.pj);
+        d.pi = (size_t)first_lane64((uint64_t)d.pi);
+        d.W = (int)first_lane((uint32_t)d.W);
+        d.H = (int)first_lane((uint32_t)d.H);
+        d.nmx = (int)first_lane((uint32_t)d.nmx);
+        d.mp_base = (int)first_lane((uint32_t)d.mp_base);
+        return d;
+    }
+}
 
 // ------------------------------------------------------------------ clip ----
 template <class T, int VB>
@@ -1310,11 +1340,8 @@ template <class T>
 constexpr int toff(int r, int c) {  // r in [-1, 64], c in [-4, 67]
     return (r + 1) * TileFmt<T>::RB + (c + 4) * TileFmt<T>::PXB;
 }
-// 16-bit tile pixel (marker | mask << 8) <-> register word (marker | mask << 16)
+// 16-bit tile pixel (marker | mask << 8) -> register word (marker | mask << 16)
 DEV uint32_t widen8(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c010c00u); }
-DEV uint32_t narrow8(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0c0c0200u); }
-// two register words -> one tile word of two 16-bit pixels (a low, b high)
-DEV uint32_t narrow8x2(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x06040200u); }
 
 // Pixel at byte offset o of the tile as a word (marker | mask << 16).
 template <class T>
@@ -1323,12 +1350,20 @@ DEV uint32_t tld_at(const lds8* p) {
     else return *reinterpret_cast<const lds32*>(p);
 }
 template <class T>
-DEV void tst_at(lds8* p, uint32_t w) {
-    if constexpr (sizeof(T) == 1) *reinterpret_cast<lds16*>(p) = (uint16_t)narrow8(w);
-    else *reinterpret_cast<lds32*>(p) = w;
+DEV uint32_t tld(const lds8* s, int r, int c) { return tld_at<T>(s + toff<T>(r, c)); }
+// The marker half of the pixel at byte offset o (zero-extended; one LDS load,
+// no repacking), and a store of the marker half alone (the mask half of a
+// tile pixel never changes).
+template <class T>
+DEV uint32_t tld_marker_at(const lds8* p) {
+    if constexpr (sizeof(T) == 1) return *p;
+    else return *reinterpret_cast<const lds16*>(p);
 }
 template <class T>
-DEV uint32_t tld(const lds8* s, int r, int c) { return tld_at<T>(s + toff<T>(r, c)); }
+DEV void tst_marker_at(lds8* p, uint32_t w) {
+    if constexpr (sizeof(T) == 1) *p = (uint8_t)w;
+    else *reinterpret_cast<lds16*>(p) = (uint16_t)w;
+}
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 DEV u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
@@ -1340,9 +1375,14 @@ DEV uint32_t upd(uint32_t w, uint32_t p) {
     const u16x2 t = __builtin_elementwise_min(W.yx, P.xx);  // (min(m,p), min(j,p) <= m)
     return as1(__builtin_elementwise_max(W, t));
 }
-// Maximum of the low halves (one v_max3_u16); the high half is left zero.
-DEV uint32_t max3_lo(uint32_t a, uint32_t b, uint32_t c) {
-    return (uint32_t)max(max((uint16_t)a, (uint16_t)b), (uint16_t)c);
+// The largest low half among p and its left and right lanes' (0 past the
+// wave's ends).  Two 16-bit maxima, each with its lane move folded in as a
+// DPP operand; the empty asm between them keeps the compiler from merging
+// them into one three-operand maximum, which cannot take a DPP operand.
+DEV uint32_t max_lr_lo(uint32_t p) {
+    uint32_t t = max((uint16_t)dpp_z<DPP_WAVE_SHR1>(p), (uint16_t)p);
+    asm volatile("" : "+v"(t));
+    return max((uint16_t)dpp_z<DPP_WAVE_SHL1>(p), (uint16_t)t);
 }
 // min(J(p), I(q)) > J(q) for a pixel value p and a packed neighbour word q.
 DEV bool raises(uint32_t p, uint32_t q) { return min(p, q >> 16) > (q & 0xFFFFu); }
@@ -1400,13 +1440,34 @@ DEV void tile_fetch(const Image& im, int mx, int my, lds8* s) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         uint32_t gj[BIT][WPL], gi[BIT][WPL];
+        if (rows_inside) {
+            // a scalar base per row group and a 32-bit lane offset: no 64-bit
+            // address arithmetic per load (idle lanes re-read row 2's words)
+            const int y = y0 - 1 + b * BIT * RPI;
+            const uint8_t* bj = im.J + (size_t)y * im.pj;
+            const uint8_t* bi = im.I + (size_t)y * im.pi;
+            const int rr = min(rl, RPI - 1);
+            const uint32_t sj = RPI * (uint32_t)im.pj, si = RPI * (uint32_t)im.pi;
+            uint32_t oj = (uint32_t)rr * (uint32_t)im.pj, oi = (uint32_t)rr * (uint32_t)im.pi;
 #pragma unroll
-        for (int u = 0; u < BIT; ++u) {
-            const int yc = min(max(y0 - 1 + (b * BIT + u) * RPI + rl, 0), im.H - 1);
+            for (int u = 0; u < BIT; ++u) {
 #pragma unroll
-            for (int w = 0; w < WPL; ++w) {
-                gj[u][w] = ld_coh(im.J + (size_t)yc * im.pj + off[w]);
-                gi[u][w] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + off[w]));
+                for (int w = 0; w < WPL; ++w) {
+                    gj[u][w] = ld_coh(bj + (oj + (uint32_t)off[w]));
+                    gi[u][w] = *gp(reinterpret_cast<const uint32_t*>(bi + (oi + (uint32_t)off[w])));
+                }
+                oj += sj;
+                oi += si;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < BIT; ++u) {
+                const int yc = min(max(y0 - 1 + (b * BIT + u) * RPI + rl, 0), im.H - 1);
+#pragma unroll
+                for (int w = 0; w < WPL; ++w) {
+                    gj[u][w] = ld_coh(im.J + (size_t)yc * im.pj + off[w]);
+                    gi[u][w] = *gp(reinterpret_cast<const uint32_t*>(im.I + (size_t)yc * im.pi + off[w]));
+                }
             }
         }
         if (rl < RPI) {
@@ -1482,9 +1543,10 @@ DEV void sweep_rows_pair(lds8* s, int lane, bool (&ch)[2], bool last_forward) {
         p = nw;
     }
     if (ch[0] || ch[1]) {
-        if constexpr (sizeof(T) == 1) {
+        if constexpr (sizeof(T) == 1) {  // marker bytes only
+            lds8* rb = reinterpret_cast<lds8*>(row);
 #pragma unroll
-            for (int k = 2; k <= 33; ++k) row[k] = narrow8x2(w[2 * k - 3], w[2 * k - 2]);
+            for (int c = 0; c < kMP; ++c) rb[(c + 4) * 2] = (uint8_t)w[c + 1];
         } else {
 #pragma unroll
             for (int c = 0; c < kMP; ++c) row[c + 4] = w[c + 1];
@@ -1494,15 +1556,11 @@ DEV void sweep_rows_pair(lds8* s, int lane, bool (&ch)[2], bool last_forward) {
 
 // Column sweeps over the interior, lane = column: top to bottom, then bottom
 // to top, the same way.  8-connectivity adds the previous row's left/right
-// neighbours (DPP), lanes 0 and 63 taking them from the halo columns.
-// 8-connectivity reads the halo columns' two pixels of the previous row at
-// every step; they are read from LDS G steps at a time, at the start of each
-// group (scheduling barriers keep them there: read one step ahead, each step
-// waited on its LDS read; read all at once, 128 live values would halve the
-// waves per SIMD).
-template <class T, int CONN, int G>
+// neighbours (DPP); the halo columns' share of the left and right columns
+// has been folded in once per visit (tile_raise_sides), so lanes 0 and 63
+// take 0 past the wave's ends.
+template <class T, int CONN>
 DEV void sweep_cols_pair(lds8* s, int lane, bool (&ch)[2], bool last_forward) {
-    static_assert(kMP % G == 0, "whole groups");
     constexpr int RB = TileFmt<T>::RB;
     lds8* col = s + (lane + 4) * TileFmt<T>::PXB;  // row -1 of the lane's column
     uint32_t w[66];
@@ -1514,42 +1572,49 @@ DEV void sweep_cols_pair(lds8* s, int lane, bool (&ch)[2], bool last_forward) {
         const int DIR = d == 0 ? 1 : -1;
         uint32_t p = d == 0 ? w[0] : w[65];
 #pragma unroll
-        for (int k0 = 0; k0 < kMP; k0 += G) {
-            uint32_t hl[G], hr[G];  // halo columns of the rows before this group's rows
-            if constexpr (CONN == 8) {
-#pragma unroll
-                for (int i = 0; i < G; ++i) {
-                    const int rp = DIR > 0 ? k0 + i : kMP + 1 - (k0 + i);  // row rp - 1
-                    hl[i] = tld_at<T>(s + toff<T>(rp - 1, -1));
-                    hr[i] = tld_at<T>(s + toff<T>(rp - 1, kMP));
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-#pragma unroll
-            for (int i = 0; i < G; ++i) {
-                const int k = k0 + i;
-                const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
-                uint32_t q = p;
-                if constexpr (CONN == 8)  // upd reads only q's low half
-                    q = max3_lo(p, from_left(p, hl[i]), from_right(p, hr[i]));
-                const uint32_t nw = upd(w[r], q);
-                ch[d] |= nw != w[r];
-                w[r] = nw;
-                p = nw;
-            }
-            if constexpr (CONN == 8) __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < kMP; ++k) {
+            const int r = DIR > 0 ? 1 + k : kMP - k;  // w index of the row updated
+            const uint32_t q = CONN == 8 ? max_lr_lo(p) : p;  // upd reads only q's low half
+            const uint32_t nw = upd(w[r], q);
+            ch[d] |= nw != w[r];
+            w[r] = nw;
+            p = nw;
         }
         if (d == 0 && last_forward && !wave_any(ch[0])) return;  // converged
     }
     if (ch[0] || ch[1]) {
 #pragma unroll
-        for (int r = 1; r <= kMP; ++r) tst_at<T>(col + r * RB, w[r]);
+        for (int r = 1; r <= kMP; ++r) tst_marker_at<T>(col + r * RB, w[r]);
     }
 }
 
-// Halo reads per group of column steps.
+// 8-connectivity: raises the left and right columns' pixels by their
+// neighbours in the halo columns (rows r - 1, r, r + 1), lane = row.  The
+// halo does not change during a visit and the marker only rises, so once is
+// enough; the column sweeps then leave the halo columns out.  Returns whether
+// a pixel rose.
 template <class T>
-constexpr int kHaloGroup = sizeof(T) == 1 ? 8 : 16;
+DEV bool tile_raise_sides(lds8* s) {
+    const int r = lane_id();
+    bool d = false;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int c = side ? kMP - 1 : 0, h = side ? kMP : -1;  // the column, its halo column
+        const uint32_t hm = max(max(tld_marker_at<T>(s + toff<T>(r - 1, h)),
+                                    tld_marker_at<T>(s + toff<T>(r, h))),
+                                tld_marker_at<T>(s + toff<T>(r + 1, h)));
+        lds8* pp = s + toff<T>(r, c);
+        const uint32_t w = tld_at<T>(pp);
+        const uint32_t nw = upd(w, hm);
+        if (nw != w) {
+            tst_marker_at<T>(pp, nw);
+            d = true;
+        }
+    }
+    wave_lds_sync();
+    return wave_any(d);
+}
+
 // Waves per SIMD the queue kernels are compiled for: what the LDS tile
 // allows (uint8 tiles: four; uint16 tiles: two).
 template <class T>
@@ -1562,15 +1627,15 @@ constexpr int kQueueWaves = sizeof(T) == 1 ? 4 : 2;
 // the registers; when two quiet sweeps precede a pair and its forward sweep
 // is quiet too, the tile has converged and the backward sweep is left out.
 // `changed`: some sweep raised a pixel.
-template <class T, int CONN, int G>
+template <class T, int CONN>
 DEV bool tile_converge(lds8* s, int max_sweeps, bool& changed) {
     const int lane = lane_id();
     int quiet = -1;
-    changed = false;
+    changed = CONN == 8 && tile_raise_sides<T>(s);
     for (int k = 0; k < max_sweeps; k += 2) {
         bool ch[2];
         if ((k & 2) == 0) sweep_rows_pair<T>(s, lane, ch, quiet == 2);
-        else sweep_cols_pair<T, CONN, G>(s, lane, ch, quiet == 2);
+        else sweep_cols_pair<T, CONN>(s, lane, ch, quiet == 2);
         wave_lds_sync();
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
@@ -1642,14 +1707,14 @@ DEV void tile_store_excl(const lds8* s, lds8* rose, Border ob, bool changed,
     if (xw < im.W) {
         const bool whole = xw + PPW <= im.W;
         const lds8* src = s + toff<T>(rl, wl * PPW);
-        uint8_t* dst = im.J + (size_t)(y0 + rl) * im.pj + (size_t)xw * sizeof(T);
-        const size_t step = (size_t)RPI * im.pj;
+        uint8_t* base = im.J + (size_t)y0 * im.pj;  // scalar base, 32-bit lane offset
+        const uint32_t lo = (uint32_t)rl * (uint32_t)im.pj + (uint32_t)(xw * (int)sizeof(T));
 #pragma unroll
         for (int it = 0; it < kMP / RPI; ++it) {
             const int y = y0 + it * RPI + rl;
             if (y >= im.H) continue;
             const uint32_t f = tile_marker_word<T>(src + it * RPI * RB);
-            uint8_t* wp = dst + it * step;
+            uint8_t* wp = base + (size_t)(it * RPI) * im.pj + lo;
             if (whole) {
                 st_coh(wp, f);
             } else {
@@ -2328,7 +2393,7 @@ DEV void pending_settle(const QueueArgs& q, uint32_t& owed, uint32_t nwaves) {
 // this visit's stores are complete.  Taking m2 before m is released only holds
 // two megapixels at once; a notification that reaches m2 while it is taken
 // queues it again, as for any running megapixel.
-template <class T, int CONN>
+template <class T, int CONN, bool ONE>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(kQueueWaves<T>)))
 void k_queue_mutex(Descs ds, QueueArgs q) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
@@ -2450,18 +2515,18 @@ void k_queue_mutex(Descs ds, QueueArgs q) {
             }
             mem_wait();
             pt.lap(0);
-            const ImgDesc d0 = desc(ds, find_mp(ds, (int)m));
+            const ImgDesc d0 = mp_desc<ONE>(ds, m);
             const int ml0 = (int)m - d0.mp_base;
             tile_fetch<T>(image_of(d0), ml0 % d0.nmx, ml0 / d0.nmx, s);
             pt.lap(1);
         }
-        const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
+        const ImgDesc dsc = mp_desc<ONE>(ds, m);
         const Image im = image_of(dsc);
         const int ml = (int)m - dsc.mp_base;
         const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
         const Border ob = tile_border<T>(s);
         bool changed;
-        const bool conv = tile_converge<T, CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
+        const bool conv = tile_converge<T, CONN>(s, 4 * q.max_iters, changed);
         pt.lap(2);
         tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
         uint32_t m2 = 0;
@@ -2486,7 +2551,7 @@ void k_queue_mutex(Descs ds, QueueArgs q) {
         }
         if (lane == 9) old = a_and(&q.st[m], ~ST_RUNNING);
         if (have2) {  // m2's loads go out behind the atomics
-            const ImgDesc d2 = desc(ds, find_mp(ds, (int)m2));
+            const ImgDesc d2 = mp_desc<ONE>(ds, m2);
             const int ml2 = (int)m2 - d2.mp_base;
             tile_fetch<T>(image_of(d2), ml2 % d2.nmx, ml2 / d2.nmx, s);
         }
@@ -2581,7 +2646,7 @@ DEV bool round_barrier(const QueueArgs& q, uint32_t nblocks, uint32_t round) {
     return ok_s != 0;
 }
 
-template <class T, int CONN, int KIND>
+template <class T, int CONN, int KIND, bool ONE>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(kQueueWaves<T>)))
 void k_queue_bsp(Descs ds, QueueArgs q) {
     constexpr bool EXCL = KIND == Q_UQ;
@@ -2658,18 +2723,18 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
                 if (!more || !take(m)) break;
                 mem_wait();
                 pt.lap(0);
-                const ImgDesc d0 = desc(ds, find_mp(ds, (int)m));
+                const ImgDesc d0 = mp_desc<ONE>(ds, m);
                 const int ml0 = (int)m - d0.mp_base;
                 tile_fetch<T>(image_of(d0), ml0 % d0.nmx, ml0 / d0.nmx, s);
                 pt.lap(1);
             }
-            const ImgDesc dsc = desc(ds, find_mp(ds, (int)m));
+            const ImgDesc dsc = mp_desc<ONE>(ds, m);
             const Image im = image_of(dsc);
             const int ml = (int)m - dsc.mp_base;
             const int mx = ml % dsc.nmx, my = ml / dsc.nmx;
             const Border ob = tile_border<T>(s);
             bool changed;
-            const bool conv = tile_converge<T, CONN, kHaloGroup<T>>(s, 4 * q.max_iters, changed);
+            const bool conv = tile_converge<T, CONN>(s, 4 * q.max_iters, changed);
             pt.lap(2);
             bool rose_any = changed;
             if constexpr (EXCL) tile_store_excl<T>(s, rose, ob, changed, im, mx, my);
@@ -2694,7 +2759,7 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
             const bool have2 = more && take(m2);
             more = have2;
             if (have2) {
-                const ImgDesc d2 = desc(ds, find_mp(ds, (int)m2));
+                const ImgDesc d2 = mp_desc<ONE>(ds, m2);
                 const int ml2 = (int)m2 - d2.mp_base;
                 tile_fetch<T>(image_of(d2), ml2 % d2.nmx, ml2 / d2.nmx, s);
             }
@@ -3069,13 +3134,13 @@ hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const Queu
     return hipGetLastError();
 }
 
-template <class T, int CONN>
+template <class T, int CONN, bool ONE>
 static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, int wpc,
                                 hipStream_t s) {
     const void* fn;
-    if (kind == Q_MUTEX) fn = (const void*)k_queue_mutex<T, CONN>;
-    else if (kind == Q_DQ) fn = (const void*)k_queue_bsp<T, CONN, Q_DQ>;
-    else fn = (const void*)k_queue_bsp<T, CONN, Q_UQ>;
+    if (kind == Q_MUTEX) fn = (const void*)k_queue_mutex<T, CONN, ONE>;
+    else if (kind == Q_DQ) fn = (const void*)k_queue_bsp<T, CONN, Q_DQ, ONE>;
+    else fn = (const void*)k_queue_bsp<T, CONN, Q_UQ, ONE>;
     int per_cu = 0;
     // tiles, and the mutex kernel's local queue (u16: 75 KiB, beyond the 64 KiB
     // default: say so)
@@ -3100,7 +3165,7 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
     Descs dc = ds;
     QueueArgs qc = q;
     if (kind == Q_MUTEX) {
-        hipLaunchKernelGGL((k_queue_mutex<T, CONN>), grid, kWG, lds_bytes, s, dc, qc);
+        hipLaunchKernelGGL((k_queue_mutex<T, CONN, ONE>), grid, kWG, lds_bytes, s, dc, qc);
         return hipGetLastError();
     }
     void* args[] = {&dc, &qc};
@@ -3115,20 +3180,28 @@ static hipError_t queue_phase_t(const Descs& ds, int kind, const QueueArgs& q, i
     // relaunches a smaller grid (every device-side wait is bounded in time).
     (void)hipGetLastError();
     if (kind == Q_DQ)
-        hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_DQ>), grid, kWG, lds_bytes, s, dc, qc);
+        hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_DQ, ONE>), grid, kWG, lds_bytes, s, dc, qc);
     else
-        hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_UQ>), grid, kWG, lds_bytes, s, dc, qc);
+        hipLaunchKernelGGL((k_queue_bsp<T, CONN, Q_UQ, ONE>), grid, kWG, lds_bytes, s, dc, qc);
     return hipGetLastError();
+}
+
+template <bool ONE>
+static hipError_t queue_phase_one(const Descs& ds, int dtype, int conn, int kind,
+                                  const QueueArgs& q, int wpc, hipStream_t s) {
+    if (dtype == 1) {
+        return conn == 8 ? queue_phase_t<uint8_t, 8, ONE>(ds, kind, q, wpc, s)
+                         : queue_phase_t<uint8_t, 4, ONE>(ds, kind, q, wpc, s);
+    }
+    return conn == 8 ? queue_phase_t<uint16_t, 8, ONE>(ds, kind, q, wpc, s)
+                     : queue_phase_t<uint16_t, 4, ONE>(ds, kind, q, wpc, s);
 }
 
 hipError_t launch_queue_phase(const Descs& ds, int dtype, int conn, int kind,
                               const QueueArgs& q, int wpc, hipStream_t s) {
-    if (dtype == 1) {
-        return conn == 8 ? queue_phase_t<uint8_t, 8>(ds, kind, q, wpc, s)
-                         : queue_phase_t<uint8_t, 4>(ds, kind, q, wpc, s);
-    }
-    return conn == 8 ? queue_phase_t<uint16_t, 8>(ds, kind, q, wpc, s)
-                     : queue_phase_t<uint16_t, 4>(ds, kind, q, wpc, s);
+    // a single image's descriptor is the kernel argument itself
+    return ds.d == nullptr ? queue_phase_one<true>(ds, dtype, conn, kind, q, wpc, s)
+                           : queue_phase_one<false>(ds, dtype, conn, kind, q, wpc, s);
 }
 
 }  // namespace iwpp
