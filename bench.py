@@ -295,6 +295,25 @@ def main():
         "basis": "row_pass / col_pass: one isolated launch each over the clipped marker, HIP events",
     }
 
+    # --- where the queue kernel's waves spend their time: one more (untimed)
+    # reconstruction with the in-kernel phase clocks on (summed over waves)
+    if main_sched == "learned":
+        learned.install()
+    work.buf.copy_(marker0.buf)
+    st_tr = iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue, main_sched,
+                             iwpp.default_options(sweep_rounds=args.sweeps, trace_phases=1), ws)
+    iwpp.set_model(None)
+    keys = ("us_wait", "us_fetch", "us_converge", "us_store", "us_notify")
+    tot_us = sum(st_tr[k] for k in keys)
+    queue_split = {
+        "basis": "queue kernel, one reconstruction with trace_phases: wave-microseconds per phase "
+                 "(wait = taking work, fetch = loading the tile, converge = local sweeps, store = "
+                 "write-back and its completion, notify = neighbour atomics, release and the next "
+                 "tile's loads), summed over waves",
+        **{k[3:]: st_tr[k] for k in keys},
+        "fractions": {k[3:]: (st_tr[k] / tot_us if tot_us > 0 else None) for k in keys},
+    }
+
     # --- CPU baseline: the oracle (a port of the sequential hybrid), one core
     cpu = None
     exact = None
@@ -338,6 +357,7 @@ def main():
         "cpu_baseline": cpu,
         "exact": exact,
         "phases_ms": phases,
+        "queue_phase_split": queue_split,
         "counters": counters,
         "sweep_kernels": sweep_kernels,
         "schedules": {k: {"ms_per_step": v[0], "mpixel_s": px / (v[0] * 1e-3) / 1e6,
