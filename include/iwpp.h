@@ -86,9 +86,10 @@ typedef struct {
     double us_converge;      /* local sweeps */
     double us_store;         /* write-back */
     double us_notify;        /* neighbour tests and queue pushes */
-    /* Split of ms_sweep (HIP events around each pass): the row passes, and
-     * the column passes (downward and upward, the learned schedule's network
-     * kernel excluded). */
+    /* Split of ms_sweep (HIP events around each pass).  Split sweeps: the
+     * row passes, and the column passes (downward and upward, the learned
+     * schedule's network kernel excluded).  Raster sweeps: the downward
+     * passes, and the upward passes. */
     float ms_rows;
     float ms_cols;
     /* With iwpp_options.check_queue (the reference's queue self-checks,
@@ -108,8 +109,8 @@ typedef struct {
 
 /* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
 typedef struct {
-    int sweep_rounds;     /* rounds of (row pass, column pass down, column pass up)
-                             before seeding the queue; -1 = the schedule decides */
+    int sweep_rounds;     /* sweep rounds (see split_sweeps) before seeding the
+                             queue; -1 = the schedule decides */
     int col_block_rows;   /* rows per block of the column passes (0 = from the
                              image size, default) */
     int max_local_iters;  /* local sweep rounds per megapixel visit before it
@@ -126,6 +127,11 @@ typedef struct {
                              from this many pixels on (default 2^27; smaller
                              images keep the plain rule: the plan's fixed cost
                              exceeds what it can save there); 0 = always */
+    int split_sweeps;     /* 0 (default): a sweep round is two raster passes over
+                             column blocks, downward then upward, each row
+                             scanned across its strip after its column step;
+                             1: a row pass (forward and backward) and two
+                             column passes (down, up) */
 } iwpp_options;
 
 void iwpp_default_options(iwpp_options* opt);
@@ -268,6 +274,19 @@ int iwpp_sweep_cols(const void* src, void* dst, const void* mask, size_t pitch_s
                     size_t pitch_dst, size_t pitch_mask, int width, int height, int dtype,
                     int connectivity, int dir, int block_rows, int* changed, void* stream);
 
+/* One raster pass, out of place, over the blocks of iwpp_sweep_cols: in each
+ * block the rows are taken in sweep order (dir +1 downward, -1 upward); a
+ * row gets the column step of iwpp_sweep_cols, then a forward and a backward
+ * scan across the block's columns that enter from src's pixels just left and
+ * right of the block in the same row.  Neighbours outside a block are read
+ * from src.  A reconstruction's sweep rounds are two such passes in place
+ * (down, up).  No reference counterpart as one pass: it fuses the
+ * reference's X and Y sweeps (IWPP MP DQ/src/MorphologicalReconstruction.cu:
+ * 817-1074) per block, as Vincent's raster scan does. */
+int iwpp_sweep_raster(const void* src, void* dst, const void* mask, size_t pitch_src,
+                      size_t pitch_dst, size_t pitch_mask, int width, int height, int dtype,
+                      int connectivity, int dir, int block_rows, int* changed, void* stream);
+
 /* Column block width in pixels used by iwpp_sweep_cols for these pointers
  * and this width (16-byte lanes for aligned rows of at least 16 KiB, else
  * 4-byte lanes). */
@@ -282,9 +301,10 @@ int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pitch_
                              size_t pitch_mask, int width, int height, int dtype,
                              int connectivity, uint8_t* flags, int* count, void* stream);
 
-/* One sweep round (row pass, column pass down, column pass up, all in place,
- * clipping first) over n images of one dtype, whose upward column pass sets
- * the megapixel seeding flags as a reconstruction's last sweep round does
+/* One sweep round (two raster passes, down and up; with split_sweeps a row
+ * pass and two column passes; all in place, clipping first) over n images of
+ * one dtype, whose upward pass sets the megapixel seeding flags as a
+ * reconstruction's last sweep round does
  * before its queue phase (the test of pairs inside each column block and
  * strip during the pass, then the test across strip edges and block
  * boundaries).  flags (device uint8): one per megapixel, the images' grids
@@ -295,7 +315,8 @@ int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pitch_
  * seeds every megapixel (IWPP MP/src/globalQueue.cu:83-97); this exposes the
  * pass that replaces it, so that its flags can be checked on their own. */
 int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int connectivity,
-                          int block_rows, uint8_t* flags, iwpp_workspace* ws, void* stream);
+                          int block_rows, int split_sweeps, uint8_t* flags, iwpp_workspace* ws,
+                          void* stream);
 
 /* Allocation-free form of iwpp_reconstruct_ex.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

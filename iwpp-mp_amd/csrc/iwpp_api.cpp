@@ -140,6 +140,7 @@ extern "C" void iwpp_default_options(iwpp_options* o) {
     o->trace_phases = 0;
     o->check_queue = 0;
     o->learned_min_pixels = kLearnedMinPixels;
+    o->split_sweeps = 0;
 }
 
 extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtype) {
@@ -451,8 +452,67 @@ Descs with_total(Descs d, int total, int round = 0) {
     return d;
 }
 
-// Sweep phase: rounds k0..k1-1 of row pass + column pass down + column pass
-// up (round 0 clips the marker to the mask in its row pass).
+// The learned plan's end on the host (the network's outputs), while the
+// downward pass of the first round runs.
+int plan_end_now(Run& r) {
+    if (!r.plan_end) return IWPP_OK;
+    SchedulePlan& plan = *r.plan_end;
+    r.plan_end = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = r.ds.n == 1 ? learned_plan_end_single(r, plan) : learned_plan_end_batch(r, plan);
+    if (rc) return rc;
+    r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t0).count();
+    r.seed_round = plan.sweep_rounds - 1;
+    return IWPP_OK;
+}
+
+// Whether round k's last pass sets the seeding flags (k_unstable_scan is
+// then skipped): the last round of every image, no tile schedule, a first
+// queue attempt.  Clears the flags and sizes the border-column buffer.
+int seeding_pass(Run& r, int k, bool* seed) {
+    bool s = !r.scan_seed && k == r.seed_round && !r.round_masks;
+    for (int i = 0; s && i < r.ds.n; ++i) s = r.h[i].rounds > k;
+    *seed = s;
+    if (s) {
+        const size_t words = ((size_t)r.total_mp + 3) / 4;
+        HIP_TRY(grow(r.ws->flags, r.ws->flags_n, words * 4));
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(r.ws->flags), 0u, words, r.s));
+        HIP_TRY(grow(r.ws->edges, r.ws->edges_n, std::max<size_t>(1, r.total_edges)));
+    }
+    return IWPP_OK;
+}
+
+// One raster round: a downward raster pass (clipping in round 0) and an
+// upward one (setting the seeding flags when it is the last pass).  The
+// learned plan's network runs before the downward pass (it reads only the
+// tile features), so that the host completes the plan while that pass runs
+// and knows before the upward pass whether it is the last.
+int raster_round(Run& r, int k, hipEvent_t* ev) {
+    const int rp = learned_plan_predict(r);
+    if (rp) return rp;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], r.s));
+    HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
+                                  k == 0 ? COL_RASTER_CLIP : COL_RASTER, r.wide, nullptr, r.s));
+    if (ev) {
+        HIP_TRY(hipEventRecord(ev[1], r.s));
+        HIP_TRY(hipEventRecord(ev[2], r.s));
+    }
+    int rc = plan_end_now(r);
+    if (rc) return rc;
+    bool seed = false;
+    rc = seeding_pass(r, k, &seed);
+    if (rc) return rc;
+    HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
+                                  seed ? COL_RASTER_SEED : COL_RASTER, r.wide, nullptr, r.s,
+                                  seed ? r.ws->flags : nullptr, seed ? r.ws->edges : nullptr));
+    r.flags_ready = seed;
+    return IWPP_OK;
+}
+
+// Sweep phase: rounds k0..k1-1, each either two raster passes (down, up; the
+// default) or a row pass + column pass down + column pass up
+// (opt.split_sweeps); round 0 clips the marker to the mask in its first pass.
 int sweep_phase(Run& r, int k0, int k1) {
     if (k1 <= 0) {  // clip only
         if (r.ds.n == 1) {
@@ -481,8 +541,13 @@ int sweep_phase(Run& r, int k0, int k1) {
             }
             ev = er.data() + 3 * k;
             r.timed_rounds = k + 1;
-            HIP_TRY(hipEventRecord(ev[0], r.s));
         }
+        if (!r.opt.split_sweeps) {
+            const int rs = raster_round(r, k, ev);
+            if (rs) return rs;
+            continue;
+        }
+        if (ev) HIP_TRY(hipEventRecord(ev[0], r.s));
         if (r.ds.n == 1) {
             const ImgDesc& d = r.h[0];
             HIP_TRY(launch_row_pass(Image{d.J, d.I, d.pj, d.pi, d.W, d.H}, r.dtype, k == 0, d.tm,
@@ -499,27 +564,12 @@ int sweep_phase(Run& r, int k0, int k1) {
         if (ev) HIP_TRY(hipEventRecord(ev[2], r.s));
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
                                       COL_IN_PLACE, r.wide, nullptr, r.s));
-        if (r.plan_end) {  // the host plans while the downward pass runs
-            SchedulePlan& plan = *r.plan_end;
-            r.plan_end = nullptr;
-            const auto t0 = std::chrono::steady_clock::now();
-            const int rc = r.ds.n == 1 ? learned_plan_end_single(r, plan)
-                                       : learned_plan_end_batch(r, plan);
-            if (rc) return rc;
-            r.st.ms_plan = (float)std::chrono::duration<double, std::milli>(
-                               std::chrono::steady_clock::now() - t0).count();
-            r.seed_round = plan.sweep_rounds - 1;
-        }
-        // the last pass sets the seeding flags (k_unstable_scan is skipped)
-        // when it is the last of every image (no tile schedule)
-        bool seed = !r.scan_seed && k == r.seed_round && !r.round_masks;
-        for (int i = 0; seed && i < r.ds.n; ++i) seed = r.h[i].rounds > k;
-        if (seed) {
-            const size_t words = ((size_t)r.total_mp + 3) / 4;
-            HIP_TRY(grow(r.ws->flags, r.ws->flags_n, words * 4));
-            HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(r.ws->flags), 0u, words, r.s));
-            HIP_TRY(grow(r.ws->edges, r.ws->edges_n, std::max<size_t>(1, r.total_edges)));
-        }
+        // the host plans while the downward pass runs
+        int rc = plan_end_now(r);
+        if (rc) return rc;
+        bool seed = false;
+        rc = seeding_pass(r, k, &seed);
+        if (rc) return rc;
         HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
                                       seed ? COL_IN_PLACE_SEED : COL_IN_PLACE, r.wide, nullptr,
                                       r.s, seed ? r.ws->flags : nullptr,
@@ -727,8 +777,9 @@ int ensure_plan_flag(iwpp_workspace* ws) {
     return IWPP_OK;
 }
 
-// Launches the network of the learned plan (after the first row pass: it
-// reads only the feature partials of the input marker).
+// Launches the network of the learned plan: it reads only the feature
+// partials of the input marker, so it goes before the first raster pass (or,
+// with split sweeps, between the first row pass and the column passes).
 int learned_plan_predict(Run& r) {
     if (!r.plan_predict) return IWPP_OK;
     r.plan_predict = false;
@@ -1486,15 +1537,17 @@ extern "C" int iwpp_sweep_rows(void* marker, const void* mask, size_t pm, size_t
     return IWPP_OK;
 }
 
-extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, size_t ps,
-                               size_t pd, size_t pk, int w, int h, int dtype, int conn, int dir,
-                               int block_rows, int* changed, void* stream) {
+namespace {
+// One out-of-place blocked pass (column or raster) of a single image.
+int blocked_pass(const void* src, void* dst, const void* mask, size_t ps, size_t pd, size_t pk,
+                 int w, int h, int dtype, int conn, int dir, int block_rows, int mode,
+                 int* changed, void* stream) {
     g_err.clear();
     int rc = check_image(src, mask, ps, pk, w, h, dtype);
     if (rc) return rc;
     rc = check_image(dst, mask, pd, pk, w, h, dtype);
     if (rc) return rc;
-    if (src == dst) return fail(IWPP_ERR_ARG, "column pass is out of place: src == dst");
+    if (src == dst) return fail(IWPP_ERR_ARG, "the pass is out of place: src == dst");
     if (conn != 4 && conn != 8) return fail(IWPP_ERR_ARG, "connectivity must be 4 or 8");
     if (dir != 1 && dir != -1) return fail(IWPP_ERR_ARG, "dir must be +1 or -1");
     const bool wide = col_wide(src, dst, mask, ps, pd, pk, w, dtype);
@@ -1513,9 +1566,24 @@ extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, siz
     d.R = block_rows > 0 ? block_rows : col_auto_rows(w, h, dtype, wide);
     d.tm = TileMask{nullptr, kMP, kMP, 1};
     ds.total = d.strips * ((h + d.R - 1) / d.R);
-    HIP_TRY(launch_col_pass_batch(ds, dtype, conn, dir, COL_J_TO_P, wide, (uint32_t*)changed,
+    HIP_TRY(launch_col_pass_batch(ds, dtype, conn, dir, mode, wide, (uint32_t*)changed,
                                   (hipStream_t)stream));
     return IWPP_OK;
+}
+}  // namespace
+
+extern "C" int iwpp_sweep_cols(const void* src, void* dst, const void* mask, size_t ps,
+                               size_t pd, size_t pk, int w, int h, int dtype, int conn, int dir,
+                               int block_rows, int* changed, void* stream) {
+    return blocked_pass(src, dst, mask, ps, pd, pk, w, h, dtype, conn, dir, block_rows,
+                        COL_J_TO_P, changed, stream);
+}
+
+extern "C" int iwpp_sweep_raster(const void* src, void* dst, const void* mask, size_t ps,
+                                 size_t pd, size_t pk, int w, int h, int dtype, int conn,
+                                 int dir, int block_rows, int* changed, void* stream) {
+    return blocked_pass(src, dst, mask, ps, pd, pk, w, h, dtype, conn, dir, block_rows,
+                        COL_RASTER_J_TO_P, changed, stream);
 }
 
 extern "C" int iwpp_sweep_cols_block_width(const void* src, const void* dst, const void* mask,
@@ -1552,8 +1620,8 @@ extern "C" int iwpp_unstable_megapixels(const void* marker, const void* mask, si
 }
 
 extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int conn,
-                                     int block_rows, uint8_t* flags, iwpp_workspace* ws,
-                                     void* stream) {
+                                     int block_rows, int split_sweeps, uint8_t* flags,
+                                     iwpp_workspace* ws, void* stream) {
     g_err.clear();
     if (!imgs || n <= 0) return fail(IWPP_ERR_ARG, "empty batch");
     if (!flags) return fail(IWPP_ERR_ARG, "null flags");
@@ -1575,6 +1643,7 @@ extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, i
     r.no_tune = true;
     iwpp_default_options(&r.opt);
     r.opt.col_block_rows = block_rows;
+    r.opt.split_sweeps = split_sweeps ? 1 : 0;
     HIP_TRY(call_stream(ws, stream, &r.s));
     rc = setup_images(r, imgs, n);
     if (rc) return rc;

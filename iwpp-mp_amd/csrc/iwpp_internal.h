@@ -150,7 +150,25 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
 // border columns as they leave the pass: edges[(strip * 2 + side) * H + y] =
 // marker | mask << 16 of the strip's first (side 0) or last (side 1) pixel of
 // row y (strips * 2 * H words, from edges + edge_base for each image).
-enum ColMode : int { COL_J_TO_P = 0, COL_P_TO_J = 1, COL_IN_PLACE = 2, COL_IN_PLACE_SEED = 3 };
+// Raster modes (COL_RASTER_*): the same blocks, and after each row's column
+// step the row is scanned forward and backward across its strip (the pixels
+// just left and right of the strip, in the same row, enter the scans as lower
+// bounds), so that one pass propagates down (or up) and sideways:
+// COL_RASTER_J_TO_P out of place (every pixel of P written; pixels outside
+// the block or strip read from J), COL_RASTER in place, COL_RASTER_CLIP in
+// place clipping the marker to the mask first (pixels read from outside the
+// block or strip are clipped as they are read), COL_RASTER_SEED in place with
+// the seeding flags and border columns of COL_IN_PLACE_SEED.
+enum ColMode : int {
+    COL_J_TO_P = 0,
+    COL_P_TO_J = 1,
+    COL_IN_PLACE = 2,
+    COL_IN_PLACE_SEED = 3,
+    COL_RASTER_J_TO_P = 4,
+    COL_RASTER = 5,
+    COL_RASTER_CLIP = 6,
+    COL_RASTER_SEED = 7
+};
 // The seeding test across the strip edges (from `edges`) and block
 // boundaries of a COL_IN_PLACE_SEED pass, completing its flags (ds.total =
 // all edge-test waves, seed_edge_waves() per image from se_base).

@@ -1076,7 +1076,17 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
 // must be zero before the pass.  For k_seed_edges the pass writes the strip's
 // border columns as it leaves them: edges[(strip * 2 + side) * H + y] =
 // marker | mask << 16 of the strip's first (side 0) or last (side 1) pixel.
-template <class T, int VB, int CONN, int DIR, bool INPLACE, bool SEED = false>
+//
+// RASTER: after its column step, each row is scanned forward and backward
+// across the strip (lane scans of (P, Q) summaries, as in the row passes),
+// entering from the pixels just left and right of the strip in the same row
+// (read from S: lower bounds of the reconstruction, in place or not).  A pass
+// then carries a rise down (or up) and sideways within its block, along the
+// staircase paths a column pass followed by a row pass would need rounds for:
+// Vincent's raster scan, per block.  CLIP: the marker is clipped to the mask
+// as it is read, the pixels read from outside the block or strip included.
+template <class T, int VB, int CONN, int DIR, bool INPLACE, bool SEED = false,
+          bool RASTER = false, bool CLIP = false>
 DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, size_t ps,
                    size_t pd, size_t pi, int W, int H, int R, int strip, int seg,
                    const TileMask& tm, uint32_t* changed, uint8_t* flags = nullptr,
@@ -1084,6 +1094,7 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
     using V = VecIO<T, VB>;
     using L = LanePx<T, VB>;
     constexpr int NW = V::NW, NP = L::NP;
+    static_assert(!CLIP || RASTER, "clipping passes are raster passes");
     const int lane = lane_id();
     const int rowbytes = W * (int)sizeof(T);
     const int xb0 = strip * 64 * VB;  // first byte of the strip
@@ -1104,14 +1115,33 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
     uint32_t prev[NP];
     {
         const int yh = DIR > 0 ? y0 - 1 : y1;
-        uint32_t w[NW];
-        if (yh >= 0 && yh < H) V::load(S + (size_t)yh * ps, o, rowbytes, w);
-        else {
+        uint32_t w[NW], wm[NW];
+        if (yh >= 0 && yh < H) {
+            V::load(S + (size_t)yh * ps, o, rowbytes, w);
+            if constexpr (CLIP) V::load(I + (size_t)yh * pi, o, rowbytes, wm);
+        } else {
 #pragma unroll
-            for (int i = 0; i < NW; ++i) w[i] = 0;
+            for (int i = 0; i < NW; ++i) w[i] = wm[i] = 0;
         }
         L::split(w, prev);
+        if constexpr (CLIP) {
+            uint32_t pm[NP];
+            L::split(wm, pm);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) prev[k] = L::vmin(prev[k], pm[k]);
+        }
     }
+    // One pixel of S in every lane (0 outside the image), clipped to its mask
+    // under CLIP: loaded from a clamped address with no branch (y and x are
+    // the same in every lane).
+    auto px_at = [&](int y, int x) -> uint32_t {
+        const bool in = y >= 0 && y < H && x >= 0 && x < W;
+        const size_t yc = (size_t)min(max(y, 0), H - 1), xc = (size_t)min(max(x, 0), W - 1);
+        uint32_t v = *gp(reinterpret_cast<const T*>(S + yc * ps + xc * sizeof(T)));
+        if constexpr (CLIP)
+            v = min(v, (uint32_t)*gp(reinterpret_cast<const T*>(I + yc * pi + xc * sizeof(T))));
+        return in ? v : 0u;
+    };
     bool any_change = false;
     // ---- seeding state (SEED): r1 = the row computed before (final), c1 its
     // mask, v1 what r1 offers a vertical neighbour inside the strip
@@ -1159,109 +1189,168 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             *gp(edges + ((size_t)strip * 2 + 1) * H + yr) = eR;
         }
     };
-    // Rows go in groups of CU: the group's loads (source, mask and, for 8-
-    // connectivity, the strip's outside neighbours in the previous source row)
-    // are issued together before any of its rows is computed, so a wave keeps
-    // 2*CU loads in flight instead of waiting for each row.  Rows past the
-    // block are loaded from its last row (in bounds) and not computed.
-    constexpr int CU = VB == 16 ? 4 : 8;  // 16-byte lanes: fewer rows, same bytes in flight
+    // Rows go in groups of CU, double-buffered: the loads of the next group
+    // (marker, mask, and the single pixels beside the strip) are issued
+    // before the rows of this group are computed, so that a wave's memory
+    // latency overlaps its own arithmetic (a raster row's scans take about as
+    // long as the loads of a group).  Every load is unconditional: rows past
+    // the block re-read its last row (in bounds, not computed), pixels
+    // outside the image come from clamped addresses and are replaced by 0.
+    // 8-connectivity takes the strip's outside neighbours in the previous
+    // row from memory in a column pass; a raster pass has them already (the
+    // previous row's entry pixels).
+    constexpr int CU = RASTER ? (VB == 16 ? 2 : 4) : (VB == 16 ? 4 : 8);
     const int n = y1 - y0;
-    for (int i0 = 0; i0 < n; i0 += CU) {
-        uint32_t jw[CU][NW], iw[CU][NW], le[CU], re[CU];
+    struct Group {
+        uint32_t jw[CU][NW], iw[CU][NW], lc[CU], rc[CU];
+    };
+    auto load_group = [&](int i0, Group& g) {
 #pragma unroll
         for (int u = 0; u < CU; ++u) {
             const int i = min(i0 + u, n - 1);
             const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
-            V::load(S + (size_t)y * ps, o, rowbytes, jw[u]);
-            V::load(I + (size_t)y * pi, o, rowbytes, iw[u]);
-            le[u] = re[u] = 0;
-            if constexpr (CONN == 8) {
-                const int yp = y - DIR;
-                const bool yp_in = yp >= 0 && yp < H;
-                if (lane == 0 && yp_in && xs0 > 0)
-                    le[u] = *gp(reinterpret_cast<const T*>(S + (size_t)yp * ps + (size_t)(xs0 - 1) * sizeof(T)));
-                if (lane == 63 && yp_in && xs1 < W)
-                    re[u] = *gp(reinterpret_cast<const T*>(S + (size_t)yp * ps + (size_t)xs1 * sizeof(T)));
+            V::load(S + (size_t)y * ps, o, rowbytes, g.jw[u]);
+            V::load(I + (size_t)y * pi, o, rowbytes, g.iw[u]);
+            g.lc[u] = g.rc[u] = 0;
+            if constexpr (RASTER) {  // the pixels the row scans enter from, in this row
+                g.lc[u] = px_at(y, xs0 - 1);
+                g.rc[u] = px_at(y, xs1);
+            } else if constexpr (CONN == 8) {  // the outside neighbours in the previous row
+                g.lc[u] = px_at(y - DIR, xs0 - 1);
+                g.rc[u] = px_at(y - DIR, xs1);
             }
         }
+    };
+    // raster: the previous row's entry pixels (for the block's first row, the
+    // row before the block)
+    uint32_t pl = 0, pr = 0;
+    if constexpr (RASTER && CONN == 8) {
+        pl = px_at(DIR > 0 ? y0 - 1 : y1, xs0 - 1);
+        pr = px_at(DIR > 0 ? y0 - 1 : y1, xs1);
+    }
+    auto row_step = [&](int i, const uint32_t (&jw)[NW], const uint32_t (&iw)[NW], uint32_t lc,
+                        uint32_t rc) {
+        const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
+        const uint32_t le = RASTER ? pl : lc, re = RASTER ? pr : rc;
+        uint32_t v[NP], c[NP], m[NP];
+        L::split(jw, v);
+        L::split(iw, c);
+        bool d = false;
+        if constexpr (CLIP) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t t = L::vmin(v[k], c[k]);
+                d |= t != v[k];
+                v[k] = t;
+            }
+        }
+        if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) c[k] = 0;
+        }
+        if constexpr (CONN == 8) {
+            // left neighbour of the lane's first register, right neighbour
+            // of its last: the adjacent lane's pixel (outside the strip:
+            // the loaded one) joined with this lane's own pixel
+            uint32_t left, right;
+            if constexpr (L::PAIRS) {
+                left = from_left(prev[NP - 1] >> 16, le) | (prev[NP - 1] << 16);
+                right = (prev[0] >> 16) | (from_right(prev[0] & 0xFFFFu, re) << 16);
+            } else {
+                left = from_left(prev[NP - 1], le);
+                right = from_right(prev[0], re);
+            }
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t a = k > 0 ? prev[k - 1] : left;
+                const uint32_t b = k < NP - 1 ? prev[k + 1] : right;
+                m[k] = L::vmax(prev[k], L::vmax(a, b));
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) m[k] = prev[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const uint32_t out = L::vmax(v[k], L::vmin(m[k], c[k]));
+            d |= out != v[k];
+            prev[k] = out;
+        }
+        if constexpr (RASTER) {
+            // the row across the strip, forward then backward, entering
+            // from the pixels beside the strip
+            L px;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                px.v[k] = prev[k];
+                px.c[k] = c[k];
+            }
+            uint32_t Pi, Qi;
+            px.template reduce<1>(Pi, Qi);
+            scan_lanes_fwd(Pi, Qi);
+            d |= px.template apply<1>(max(from_left(Pi, 0u), min(lc, from_left(Qi, 0xFFFFFFFFu))));
+            px.template reduce<-1>(Pi, Qi);
+            scan_lanes_bwd(Pi, Qi, lane);
+            d |= px.template apply<-1>(max(from_right(Pi, 0u), min(rc, from_right(Qi, 0xFFFFFFFFu))));
+#pragma unroll
+            for (int k = 0; k < NP; ++k) prev[k] = px.v[k];
+        }
+        any_change |= d;
+        if (!INPLACE || d) {  // in place: only vectors that rose are written
+            uint32_t ow[NW];
+            L::join(prev, ow);
+            V::store(D + (size_t)y * pd, o, rowbytes, ow);
+        }
+        if constexpr (SEED) {
+            {  // the strip's border pixels (final) into lane y % 64
+                const uint32_t f = rdlane(prev[0], 0), cf = rdlane(c[0], 0);
+                const uint32_t l = rdlane(prev[NP - 1], 63), cl = rdlane(c[NP - 1], 63);
+                if ((y >> 6) != eband) {
+                    edge_flush();
+                    eband = y >> 6;
+                }
+                const bool here = lane == (y & 63);
+                eL = here ? (f & 0xFFFFu) | (cf << 16) : eL;
+                eR = here ? (L::PAIRS ? (l >> 16) | (cl & 0xFFFF0000u) : l | (cl << 16)) : eR;
+            }
+            // row y against its own neighbours and row r1 = y - DIR, and
+            // r1 against row y (the row before y in the block: i > 0)
+            uint32_t h[NP];
+            nbmax(prev, h, false);
+            bool uy = false, ur = false;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t dn = CONN == 8 ? L::vmax(h[k], prev[k]) : prev[k];
+                const uint32_t q = i > 0 ? L::vmax(h[k], v1[k]) : h[k];
+                uy |= L::vmax(L::vmin(q, c[k]), prev[k]) != prev[k];
+                if (i > 0) ur |= L::vmax(L::vmin(dn, c1[k]), r1[k]) != r1[k];
+                v1[k] = dn;
+                r1[k] = prev[k];
+                c1[k] = c[k];
+            }
+            if (i > 0) flag_band(y - DIR, ur);
+            flag_band(y, uy);
+        }
+        if constexpr (RASTER) {
+            pl = lc;
+            pr = rc;
+        }
+    };
+    auto run_group = [&](int i0, const Group& g) {
 #pragma unroll
         for (int u = 0; u < CU; ++u) {
             if (i0 + u >= n) break;  // the same in every lane
-            const int i = i0 + u;
-            const int y = DIR > 0 ? y0 + i : y1 - 1 - i;
-            uint32_t v[NP], c[NP], m[NP];
-            L::split(jw[u], v);
-            L::split(iw[u], c);
-            if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
-#pragma unroll
-                for (int k = 0; k < NP; ++k) c[k] = 0;
-            }
-            if constexpr (CONN == 8) {
-                // left neighbour of the lane's first register, right neighbour
-                // of its last: the adjacent lane's pixel (outside the strip:
-                // the loaded one) joined with this lane's own pixel
-                uint32_t left, right;
-                if constexpr (L::PAIRS) {
-                    left = from_left(prev[NP - 1] >> 16, le[u]) | (prev[NP - 1] << 16);
-                    right = (prev[0] >> 16) | (from_right(prev[0] & 0xFFFFu, re[u]) << 16);
-                } else {
-                    left = from_left(prev[NP - 1], le[u]);
-                    right = from_right(prev[0], re[u]);
-                }
-#pragma unroll
-                for (int k = 0; k < NP; ++k) {
-                    const uint32_t a = k > 0 ? prev[k - 1] : left;
-                    const uint32_t b = k < NP - 1 ? prev[k + 1] : right;
-                    m[k] = L::vmax(prev[k], L::vmax(a, b));
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < NP; ++k) m[k] = prev[k];
-            }
-            bool d = false;
-#pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                const uint32_t out = L::vmax(v[k], L::vmin(m[k], c[k]));
-                d |= out != v[k];
-                prev[k] = out;
-            }
-            any_change |= d;
-            if (!INPLACE || d) {  // in place: only vectors that rose are written
-                uint32_t ow[NW];
-                L::join(prev, ow);
-                V::store(D + (size_t)y * pd, o, rowbytes, ow);
-            }
-            if constexpr (SEED) {
-                {  // the strip's border pixels (final) into lane y % 64
-                    const uint32_t f = rdlane(prev[0], 0), cf = rdlane(c[0], 0);
-                    const uint32_t l = rdlane(prev[NP - 1], 63), cl = rdlane(c[NP - 1], 63);
-                    if ((y >> 6) != eband) {
-                        edge_flush();
-                        eband = y >> 6;
-                    }
-                    const bool here = lane == (y & 63);
-                    eL = here ? (f & 0xFFFFu) | (cf << 16) : eL;
-                    eR = here ? (L::PAIRS ? (l >> 16) | (cl & 0xFFFF0000u) : l | (cl << 16)) : eR;
-                }
-                // row y against its own neighbours and row r1 = y - DIR, and
-                // r1 against row y (the row before y in the block: i > 0)
-                uint32_t h[NP];
-                nbmax(prev, h, false);
-                bool uy = false, ur = false;
-#pragma unroll
-                for (int k = 0; k < NP; ++k) {
-                    const uint32_t dn = CONN == 8 ? L::vmax(h[k], prev[k]) : prev[k];
-                    const uint32_t q = i > 0 ? L::vmax(h[k], v1[k]) : h[k];
-                    uy |= L::vmax(L::vmin(q, c[k]), prev[k]) != prev[k];
-                    if (i > 0) ur |= L::vmax(L::vmin(dn, c1[k]), r1[k]) != r1[k];
-                    v1[k] = dn;
-                    r1[k] = prev[k];
-                    c1[k] = c[k];
-                }
-                if (i > 0) flag_band(y - DIR, ur);
-                flag_band(y, uy);
-            }
+            row_step(i0 + u, g.jw[u], g.iw[u], g.lc[u], g.rc[u]);
         }
+    };
+    Group ga, gb;
+    load_group(0, ga);
+    for (int i0 = 0; i0 < n; i0 += 2 * CU) {
+        load_group(i0 + CU, gb);
+        run_group(i0, ga);
+        if (i0 + CU >= n) break;
+        load_group(i0 + 2 * CU, ga);
+        run_group(i0 + CU, gb);
     }
     if constexpr (SEED) {
         if (n > 0) {
@@ -1295,9 +1384,21 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, u
         col_block<T, VB, CONN, DIR, false>(d.P, d.J, d.I, d.pp, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
     else if constexpr (MODE == 2)
         col_block<T, VB, CONN, DIR, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
-    else  // in place, with the seeding flags
+    else if constexpr (MODE == 3)  // in place, with the seeding flags
         col_block<T, VB, CONN, DIR, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm,
                                                changed, flags, d.mp_base, d.nmx, edges + d.edge_base);
+    else if constexpr (MODE == 4)  // raster, read J write P
+        col_block<T, VB, CONN, DIR, false, false, true>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip,
+                                                       seg, d.tm, changed);
+    else if constexpr (MODE == 5)  // raster, in place
+        col_block<T, VB, CONN, DIR, true, false, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip,
+                                                      seg, d.tm, changed);
+    else if constexpr (MODE == 6)  // raster, in place, clipping first
+        col_block<T, VB, CONN, DIR, true, false, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R,
+                                                            strip, seg, d.tm, changed);
+    else  // raster, in place, with the seeding flags
+        col_block<T, VB, CONN, DIR, true, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg,
+                                                     d.tm, changed, flags, d.mp_base, d.nmx, edges + d.edge_base);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --
@@ -3024,7 +3125,16 @@ static void col_pass_dir(const Descs& ds, int mode, uint32_t* changed, uint8_t* 
     if (mode == 0) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 0>), grid, kWG, 0, s, ds, changed, flags, edges);
     else if (mode == 1) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 1>), grid, kWG, 0, s, ds, changed, flags, edges);
     else if (mode == 2) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 2>), grid, kWG, 0, s, ds, changed, flags, edges);
-    else hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 3>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else if (mode == 3) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 3>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else if (mode == 4) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 4>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else if (mode == 5) hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 5>), grid, kWG, 0, s, ds, changed, flags, edges);
+    else if (mode == 6) {  // the clipping pass is the first (downward) pass
+        if constexpr (DIR > 0)
+            hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 6>), grid, kWG, 0, s, ds, changed, flags, edges);
+    } else {  // the seeding pass is the last (upward) pass
+        if constexpr (DIR < 0)
+            hipLaunchKernelGGL((k_col_pass<T, VB, CONN, DIR, 7>), grid, kWG, 0, s, ds, changed, flags, edges);
+    }
 }
 
 template <class T, int VB>
@@ -3043,7 +3153,11 @@ hipError_t launch_col_pass_batch(const Descs& ds, int dtype, int conn, int dir, 
                                  bool wide, uint32_t* changed, hipStream_t s, uint8_t* flags,
                                  uint32_t* edges) {
     if (ds.total <= 0) return hipSuccess;
-    if (mode == COL_IN_PLACE_SEED && (!flags || !edges)) return hipErrorInvalidValue;
+    if ((mode == COL_IN_PLACE_SEED || mode == COL_RASTER_SEED) && (!flags || !edges))
+        return hipErrorInvalidValue;
+    if (mode < COL_J_TO_P || mode > COL_RASTER_SEED || (mode == COL_RASTER_CLIP && dir < 0) ||
+        (mode == COL_RASTER_SEED && dir > 0))
+        return hipErrorInvalidValue;
     if (dtype == 1) {
         if (wide) col_pass_t<uint8_t, 16>(ds, conn, dir, mode, changed, flags, edges, s);
         else col_pass_t<uint8_t, 4>(ds, conn, dir, mode, changed, flags, edges, s);

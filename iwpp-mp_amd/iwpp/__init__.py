@@ -44,7 +44,7 @@ class Options(ctypes.Structure):
                 ("max_local_iters", ctypes.c_int), ("queue_capacity", ctypes.c_int),
                 ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double),
                 ("trace_phases", ctypes.c_int), ("check_queue", ctypes.c_int),
-                ("learned_min_pixels", ctypes.c_longlong)]
+                ("learned_min_pixels", ctypes.c_longlong), ("split_sweeps", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -78,6 +78,7 @@ SIGNATURES = {
     "iwpp_clip": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _VP]),
     "iwpp_sweep_rows": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _VP, _VP]),
     "iwpp_sweep_cols": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _VP, _VP]),
+    "iwpp_sweep_raster": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I, _I, _I, _I, _I, _VP, _VP]),
     "iwpp_sweep_cols_block_width": (_I, [_VP, _VP, _VP, _SZ, _SZ, _SZ, _I, _I]),
     "iwpp_unstable_megapixels": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I, _VP, _VP, _VP]),
     "iwpp_reconstruct_batch": (_I, [_VP, _I, _I, _I, _I, _I, _I, ctypes.POINTER(Options),
@@ -95,7 +96,7 @@ SIGNATURES = {
     "iwpp_mlp_backward": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, ctypes.c_float, ctypes.c_float,
                                _VP, _VP, _VP]),
     "iwpp_mlp_adam": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.c_float, _I, _VP]),
-    "iwpp_sweep_round_seed": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP]),
+    "iwpp_sweep_round_seed": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP]),
     "iwpp_reconstruct_noalloc": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I,
                                       ctypes.POINTER(Options), _VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
@@ -500,6 +501,13 @@ def sweep_cols(src, dst, mask, connectivity, direction, block_rows, changed=None
                                  _stream(stream)))
 
 
+def sweep_raster(src, dst, mask, connectivity, direction, block_rows, changed=None, stream=None):
+    _raise(lib().iwpp_sweep_raster(src.ptr, dst.ptr, mask.ptr, src.pitch, dst.pitch, mask.pitch,
+                                   src.w, src.h, src.dtype, connectivity, direction, block_rows,
+                                   changed.data_ptr() if changed is not None else None,
+                                   _stream(stream)))
+
+
 def sweep_cols_block_width(src, dst, mask):
     return _raise_nonneg(lib().iwpp_sweep_cols_block_width(src.ptr, dst.ptr, mask.ptr, src.pitch,
                                                            dst.pitch, mask.pitch, src.w,
@@ -526,17 +534,19 @@ def unstable_megapixels(marker, mask, connectivity, stream=None):
     return flags.cpu().numpy(), int(count.item())
 
 
-def sweep_round_seed(pairs, connectivity=8, block_rows=0, workspace=None, stream=None):
+def sweep_round_seed(pairs, connectivity=8, block_rows=0, split_sweeps=False, workspace=None,
+                     stream=None):
     """One in-place sweep round over a batch of (marker, mask) pairs whose
-    last column pass sets the megapixel seeding flags; returns one uint8
-    numpy flag grid (ceil(h/64), ceil(w/64)) per image."""
+    last pass (upward raster pass, or with split_sweeps the upward column
+    pass) sets the megapixel seeding flags; returns one uint8 numpy flag grid
+    (ceil(h/64), ceil(w/64)) per image."""
     torch = _torch()
     arr, dtype = _batch_array(pairs)
     grids = [((mk.h + MP_SIZE - 1) // MP_SIZE, (mk.w + MP_SIZE - 1) // MP_SIZE) for mk, _ in pairs]
     total = sum(a * b for a, b in grids)
     flags = torch.full((total,), 0xEE, dtype=torch.uint8, device=pairs[0][0].buf.device)
     _raise(lib().iwpp_sweep_round_seed(arr, len(pairs), dtype, connectivity, block_rows,
-                                       flags.data_ptr(), workspace.handle if workspace else None,
+                                       1 if split_sweeps else 0, flags.data_ptr(), workspace.handle if workspace else None,
                                        _stream(stream)))
     torch.cuda.synchronize()
     host = flags.cpu().numpy()

@@ -32,6 +32,7 @@ def lib():
         L.oracle_reconstruct.argtypes = [vp, vp, sz, sz, i, i, i, i]
         L.oracle_pass_x.argtypes = [vp, vp, sz, sz, i, i, i]
         L.oracle_pass_y.argtypes = [vp, vp, vp, sz, sz, sz, i, i, i, i, i, i, i]
+        L.oracle_pass_raster.argtypes = [vp, vp, vp, sz, sz, sz, i, i, i, i, i, i, i]
         L.oracle_unstable_blocks.argtypes = [vp, vp, sz, sz, i, i, i, i, i, i, vp]
         L.oracle_unstable_blocks.restype = ctypes.c_long
         L.qm_reconstruct.argtypes = [vp, vp, sz, sz, i, i, i, i, i, i, i, ctypes.c_uint64,
@@ -103,6 +104,19 @@ def pass_y(src, mask, conn, direction, block_w, block_h):
     _check(lib().oracle_pass_y(S.ctypes.data, D.ctypes.data, I.ctypes.data, _pitch(S),
                                _pitch(D), _pitch(I), w, h, _dt(S), conn, direction,
                                block_w, block_h), "oracle_pass_y")
+    return D
+
+
+def pass_raster(src, mask, conn, direction, block_w, block_h):
+    """One blocked, out-of-place raster pass (new array): each row's column
+    step, then its forward and backward scans across the block."""
+    S = np.ascontiguousarray(src)
+    I = np.ascontiguousarray(mask)
+    D = np.zeros_like(S)
+    h, w = S.shape
+    _check(lib().oracle_pass_raster(S.ctypes.data, D.ctypes.data, I.ctypes.data, _pitch(S),
+                                    _pitch(D), _pitch(I), w, h, _dt(S), conn, direction,
+                                    block_w, block_h), "oracle_pass_raster")
     return D
 
 

@@ -246,6 +246,67 @@ static const int NB4_DX[4] = {-1, 0, 1, 0};
 DEFINE_ORACLE(uint8_t, u8)
 DEFINE_ORACLE(uint16_t, u16)
 
+/* Raster pass over blocks (bw columns x bh rows), out of place: in each block
+ * the rows are taken in sweep order; a row first gets the column step of
+ * pass_y (from the block's previous output row; neighbours outside the block
+ * from src), then a forward and a backward scan across the block's columns,
+ * entering from src's pixels just left and right of the block in the same
+ * row (0 outside the image).  Vincent's raster scan (1993) with the row
+ * scanned both ways, restricted to one block; the pixels it reads outside a
+ * block are lower bounds of the reconstruction, so any order of such passes
+ * has the same fixed point. */
+#define DEFINE_RASTER(T, SUF)                                                                   \
+    static void pass_raster_##SUF(const void* sb, void* db, const void* ib, size_t ps,          \
+                                  size_t pd, size_t pi, int w, int h, int conn, int dir,        \
+                                  int bw, int bh) {                                             \
+        for (int by0 = 0; by0 < h; by0 += bh) {                                                 \
+            const int by1 = MIN(h, by0 + bh);                                                   \
+            for (int bx0 = 0; bx0 < w; bx0 += bw) {                                             \
+                const int bx1 = MIN(w, bx0 + bw);                                               \
+                const int ystart = dir > 0 ? by0 : by1 - 1;                                     \
+                const int yend = dir > 0 ? by1 : by0 - 1;                                       \
+                for (int y = ystart; y != yend; y += dir) {                                     \
+                    const int yp = y - dir;                                                     \
+                    const int prev_in_block = yp >= by0 && yp < by1;                            \
+                    const int prev_in_image = yp >= 0 && yp < h;                                \
+                    const T* S = CROWP(T, sb, ps, y);                                           \
+                    const T* I = CROWP(T, ib, pi, y);                                           \
+                    T* D = ROWP(T, db, pd, y);                                                  \
+                    const T* Pd = prev_in_image ? CROWP(T, db, pd, yp) : NULL;                  \
+                    const T* Ps = prev_in_image ? CROWP(T, sb, ps, yp) : NULL;                  \
+                    for (int x = bx0; x < bx1; ++x) {                                           \
+                        T m = 0;                                                                \
+                        if (prev_in_image) {                                                    \
+                            m = prev_in_block ? Pd[x] : Ps[x];                                  \
+                            if (conn == 8) {                                                    \
+                                if (x - 1 >= 0)                                                 \
+                                    m = MAX(m, (prev_in_block && x - 1 >= bx0) ? Pd[x - 1]      \
+                                                                               : Ps[x - 1]);    \
+                                if (x + 1 < w)                                                  \
+                                    m = MAX(m, (prev_in_block && x + 1 < bx1) ? Pd[x + 1]       \
+                                                                              : Ps[x + 1]);     \
+                            }                                                                   \
+                        }                                                                       \
+                        D[x] = MAX(S[x], MIN(m, I[x]));                                         \
+                    }                                                                           \
+                    T c = bx0 > 0 ? S[bx0 - 1] : 0;                                             \
+                    for (int x = bx0; x < bx1; ++x) {                                           \
+                        D[x] = MAX(D[x], MIN(c, I[x]));                                         \
+                        c = D[x];                                                               \
+                    }                                                                           \
+                    c = bx1 < w ? S[bx1] : 0;                                                   \
+                    for (int x = bx1 - 1; x >= bx0; --x) {                                      \
+                        D[x] = MAX(D[x], MIN(c, I[x]));                                         \
+                        c = D[x];                                                               \
+                    }                                                                           \
+                }                                                                               \
+            }                                                                                   \
+        }                                                                                       \
+    }
+
+DEFINE_RASTER(uint8_t, u8)
+DEFINE_RASTER(uint16_t, u16)
+
 int oracle_clip(void* marker, const void* mask, size_t pj, size_t pi, int w, int h, int dtype) {
     int rc = check_args(marker, mask, pj, pi, w, h, dtype);
     if (rc) return rc;
@@ -282,6 +343,18 @@ int oracle_pass_y(const void* src, void* dst, const void* mask, size_t ps, size_
     if ((dir != 1 && dir != -1) || bw <= 0 || bh <= 0) return -6;
     if (dtype == 1) pass_y_u8(src, dst, mask, ps, pd, pi, w, h, conn, dir, bw, bh);
     else pass_y_u16(src, dst, mask, ps, pd, pi, w, h, conn, dir, bw, bh);
+    return 0;
+}
+
+int oracle_pass_raster(const void* src, void* dst, const void* mask, size_t ps, size_t pd,
+                       size_t pi, int w, int h, int dtype, int conn, int dir, int bw, int bh) {
+    int rc = check_args(src, mask, ps, pi, w, h, dtype);
+    if (rc) return rc;
+    if (!dst || pd < (size_t)w * dtype) return -3;
+    if (conn != 4 && conn != 8) return -4;
+    if ((dir != 1 && dir != -1) || bw <= 0 || bh <= 0) return -6;
+    if (dtype == 1) pass_raster_u8(src, dst, mask, ps, pd, pi, w, h, conn, dir, bw, bh);
+    else pass_raster_u16(src, dst, mask, ps, pd, pi, w, h, conn, dir, bw, bh);
     return 0;
 }
 

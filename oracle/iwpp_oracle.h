@@ -58,6 +58,15 @@ int oracle_pass_y(const void* src, void* dst, const void* mask, size_t pitch_s,
                   size_t pitch_d, size_t pitch_i, int width, int height, int dtype,
                   int conn, int dir, int block_w, int block_h);
 
+/* One raster pass, blocked and out of place: the blocks of oracle_pass_y, and
+ * after each row's column step the row is scanned across its block, forward
+ *   out[y][x] = max(out[y][x], min(out[y][x-1], I[y][x]))
+ * from x = bx0 (out[y][bx0-1] read from src, 0 outside the image), then
+ * backward the same way from x = bx1-1 (out[y][bx1] from src). */
+int oracle_pass_raster(const void* src, void* dst, const void* mask, size_t pitch_s,
+                       size_t pitch_d, size_t pitch_i, int width, int height, int dtype,
+                       int conn, int dir, int block_w, int block_h);
+
 /* Stability test per megapixel block of mp_w x mp_h pixels.  flags[b] = 1 when
  * some pixel p of block b has a neighbour q with min(J(q), I(p)) > J(p).
  * flags has ceil(h/mp_h) * ceil(w/mp_w) entries, row-major.  Returns the

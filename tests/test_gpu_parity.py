@@ -91,6 +91,29 @@ def test_col_pass(torch_cuda, dtype, conn, pitch_kind):
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("pitch_kind", ["aligned", "odd"])
+def test_raster_pass(torch_cuda, dtype, conn, pitch_kind):
+    # one out-of-place raster pass (each row's column step, then its scans
+    # across the strip) against the oracle's pass, blocks of 1..1000 rows
+    rng = np.random.default_rng(zlib.crc32(f"raster/{dtype.__name__}/{conn}/{pitch_kind}".encode()))
+    for h, w in COL_SHAPES:
+        for rows in (1, 7, 64, 1000):
+            for d in (1, -1):
+                for kind in ("seeds", "hdome"):
+                    say("raster_pass", dtype.__name__, conn, pitch_kind, h, w, rows, d, kind)
+                    J, I = rand_pair(rng, h, w, dtype, kind)
+                    pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
+                    src, msk = up(J, pitch), up(I, pitch)
+                    dst = iwpp.DeviceImage(h, w, src.dtype, pitch)
+                    bw = iwpp.sweep_cols_block_width(src, dst, msk)
+                    iwpp.sweep_raster(src, dst, msk, conn, d, rows)
+                    torch_cuda.cuda.synchronize()
+                    want = oracle.pass_raster(J, I, conn, d, bw, rows)
+                    assert np.array_equal(dst.numpy(), want), (h, w, rows, d, bw, kind)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("conn", [4, 8])
 def test_unstable_megapixels(torch_cuda, dtype, conn):
     rng = np.random.default_rng(3)
     for h, w in [(1, 1), (64, 64), (65, 129), (200, 333), (512, 300), (130, 16500), (70, 8200)]:
@@ -153,17 +176,43 @@ def test_reconstruct_wide_rows(torch_cuda, queue, conn, dtype):
 
 @pytest.mark.parametrize("queue", VARIANTS)
 def test_reconstruct_unclipped_marker_and_odd_pitch(torch_cuda, queue):
+    # The first pass clips the marker; a raster pass also clips the pixels it
+    # reads from outside its block and strip (other waves' pixels, not yet
+    # clipped or already raised): several blocks and strips, both lane widths.
     rng = np.random.default_rng(5)
     for dtype in (np.uint8, np.uint16):
         for conn in (4, 8):
-            say("unclipped", queue, dtype.__name__, conn)
-            I = rng.integers(0, 200, (150, 211)).astype(dtype)
-            J = rng.integers(0, 255, (150, 211)).astype(dtype)  # not clipped
-            pitch = odd_pitch(211, dtype)
-            dj, di = up(J, pitch), up(I, pitch)
-            iwpp.reconstruct(dj, di, conn, 0, queue, "plain",
-                             iwpp.default_options(sweep_rounds=0))
-            assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn))
+            for h, w in ((150, 211), (150, 700), (97, 16500 // np.dtype(dtype).itemsize)):
+                for sweeps in (0, 1, 2):
+                    for split in (0, 1):
+                        say("unclipped", queue, dtype.__name__, conn, h, w, sweeps, split)
+                        I = rng.integers(0, 200, (h, w)).astype(dtype)
+                        J = rng.integers(0, 255, (h, w)).astype(dtype)  # not clipped
+                        pitch = odd_pitch(w, dtype) if w < 1000 else None
+                        dj, di = up(J, pitch), up(I, pitch)
+                        st = iwpp.reconstruct(dj, di, conn, 0, queue, "plain",
+                                              iwpp.default_options(sweep_rounds=sweeps,
+                                                                   col_block_rows=32,
+                                                                   split_sweeps=split))
+                        assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn)), st
+
+
+@pytest.mark.parametrize("queue", VARIANTS)
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_reconstruct_split_sweeps(torch_cuda, queue, conn, dtype):
+    # the sweep rounds of a row pass and two column passes (split_sweeps)
+    rng = np.random.default_rng(zlib.crc32(f"split/{queue}/{conn}/{dtype.__name__}".encode()))
+    size = np.dtype(dtype).itemsize
+    for h, w in ((1, 300), (300, 1), (200, 333), (640, 257), (150, 16397 // size)):
+        for kind in ("random", "seeds", "hdome"):
+            for sweeps in (1, 2):
+                say("split sweeps", queue, conn, dtype.__name__, h, w, kind, sweeps)
+                J, I = rand_pair(rng, h, w, dtype, kind)
+                dj, di = up(J), up(I)
+                st = iwpp.reconstruct(dj, di, conn, 0, queue, "plain",
+                                      iwpp.default_options(sweep_rounds=sweeps, split_sweeps=1))
+                assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn)), (h, w, kind, st)
 
 
 @pytest.mark.parametrize("queue", VARIANTS)
@@ -295,23 +344,24 @@ def _seed_batches(dtype):
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("conn", [4, 8])
-def test_fused_seeding_flags(torch_cuda, dtype, conn):
+@pytest.mark.parametrize("split", [False, True])
+def test_fused_seeding_flags(torch_cuda, dtype, conn, split):
     # The last column pass of a sweep phase sets the seeding flags itself
     # (tests inside its blocks and strips, then across strip edges and block
     # boundaries).  Its flags must be exactly those of the stability test on
     # the marker it leaves: a missing flag loses work, an extra one wastes a
     # visit.
-    rng = np.random.default_rng(zlib.crc32(f"seedflags/{dtype.__name__}/{conn}".encode()))
+    rng = np.random.default_rng(zlib.crc32(f"seedflags/{dtype.__name__}/{conn}/{split}".encode()))
     for name, shapes, pitch_kind in _seed_batches(dtype):
         for block_rows in (0, 32, 96):
             for kind in ("seeds", "hdome", "random"):
-                say("fused seeding", dtype.__name__, conn, name, block_rows, kind)
+                say("fused seeding", dtype.__name__, conn, split, name, block_rows, kind)
                 cases = [rand_pair(rng, h, w, dtype, kind) for h, w in shapes]
                 pairs = []
                 for (J, I), (h, w) in zip(cases, shapes):
                     pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
                     pairs.append((up(J, pitch), up(I, pitch)))
-                flags = iwpp.sweep_round_seed(pairs, conn, block_rows)
+                flags = iwpp.sweep_round_seed(pairs, conn, block_rows, split)
                 for (J, I), (dj, _), f in zip(cases, pairs, flags):
                     after = dj.numpy()
                     want = oracle.unstable_blocks(after, I, conn)

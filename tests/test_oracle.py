@@ -226,3 +226,124 @@ def test_pair50_window_hashes():
     for conn in (4, 8):
         out = oracle.reconstruct(d["marker"], d["mask"], conn)
         assert hashlib.sha256(out.tobytes()).hexdigest() == str(d[f"out{conn}_sha256"])
+
+
+def _raster_by_thresholds(S, I, conn, d, bw, bh):
+    """The raster pass by threshold decomposition: out(p) >= t iff p is
+    reached at level t, i.e. by a walk of allowed moves through pixels whose
+    mask is >= t, from a pixel whose src value is >= t.  Within a block and in
+    sweep order, a row is entered from the row before (straight, or also
+    diagonally with 8-connectivity; from src when that row lies outside the
+    block or image edge), from its own src pixels, and from src's pixel left
+    of the block (moving right) or right of it (moving left); in the row a
+    walk moves right any number of steps, then left.  An independent
+    formulation of the pass: reachability per level, no sequential update."""
+    h, w = S.shape
+    out = S.copy()
+    levels = sorted(set(int(v) for v in np.unique(S)) | set(int(v) for v in np.unique(I)))
+    for t in levels:
+        if t == 0:
+            continue
+        for by0 in range(0, h, bh):
+            by1 = min(h, by0 + bh)
+            for bx0 in range(0, w, bw):
+                bx1 = min(w, bx0 + bw)
+                rows = range(by0, by1) if d > 0 else range(by1 - 1, by0 - 1, -1)
+                final = {}  # row -> set of x reached at the end of the row (left moves done)
+                for y in rows:
+                    yp = y - d
+                    start = set()
+                    for x in range(bx0, bx1):
+                        if I[y, x] < t:
+                            continue
+                        if S[y, x] >= t:
+                            start.add(x)
+                            continue
+                        if 0 <= yp < h:
+                            nbr = [x - 1, x, x + 1] if conn == 8 else [x]
+                            for xq in nbr:
+                                if not 0 <= xq < w:
+                                    continue
+                                inside = by0 <= yp < by1 and bx0 <= xq < bx1
+                                if (xq in final[yp]) if inside else (S[yp, xq] >= t):
+                                    start.add(x)
+                                    break
+                    right = set()
+                    x = bx0
+                    carry = bx0 > 0 and S[y, bx0 - 1] >= t
+                    for x in range(bx0, bx1):
+                        carry = I[y, x] >= t and (carry or x in start)
+                        if carry:
+                            right.add(x)
+                    left = set()
+                    carry = bx1 < w and S[y, bx1] >= t
+                    for x in range(bx1 - 1, bx0 - 1, -1):
+                        carry = I[y, x] >= t and (carry or x in right)
+                        if carry:
+                            left.add(x)
+                    final[y] = left
+                    for x in left:
+                        out[y, x] = max(out[y, x], t)
+    return out
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_oracle_pass_raster_thresholds(conn):
+    rng = np.random.default_rng(40 + conn)
+    for h, w, bw, bh in [(6, 7, 1 << 20, 1 << 20), (9, 11, 4, 3), (1, 9, 1 << 20, 1 << 20),
+                         (8, 1, 1 << 20, 4), (12, 10, 5, 5)]:
+        I = rng.integers(0, 12, (h, w)).astype(np.uint8)
+        S = np.minimum(rng.integers(0, 12, (h, w)).astype(np.uint8), I)
+        S[rng.random((h, w)) < 0.6] = 0
+        for d in (+1, -1):
+            got = oracle.pass_raster(S, I, conn, d, bw, bh)
+            want = _raster_by_thresholds(S, I, conn, d, bw, bh)
+            assert np.array_equal(got, want), (h, w, bw, bh, d)
+
+
+def test_oracle_pass_raster_closed_forms():
+    rng = np.random.default_rng(44)
+    I = rng.integers(0, 256, (1, 40)).astype(np.uint8)
+    S = np.minimum(rng.integers(0, 256, (1, 40)).astype(np.uint8), I)
+    # one row, one block: the row pass
+    for conn in (4, 8):
+        assert np.array_equal(oracle.pass_raster(S, I, conn, 1, 1 << 20, 1 << 20),
+                              oracle.pass_x(S, I))
+    # one column: the column pass
+    I = rng.integers(0, 256, (40, 1)).astype(np.uint8)
+    S = np.minimum(rng.integers(0, 256, (40, 1)).astype(np.uint8), I)
+    for d in (1, -1):
+        assert np.array_equal(oracle.pass_raster(S, I, 8, d, 1 << 20, 7),
+                              oracle.pass_y(S, I, 8, d, 1 << 20, 7))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_oracle_raster_passes_reach_the_reconstruction(conn, dtype):
+    # Raster passes down and up, repeated until nothing changes, stop at the
+    # reconstruction (checked by brute force), blocked or not; each pass stays
+    # between its input and the reconstruction.
+    rng = np.random.default_rng(50 + conn)
+    for h, w, bw, bh in [(33, 47, 1 << 20, 1 << 20), (40, 29, 8, 6), (17, 64, 16, 32)]:
+        marker, mask = random_pair(rng, h, w, dtype, "seeds")
+        want = brute_force(marker, mask, conn)
+        J = np.minimum(marker, mask)
+        for it in range(10000):
+            before = J
+            J = oracle.pass_raster(J, mask, conn, +1, bw, bh)
+            J = oracle.pass_raster(J, mask, conn, -1, bw, bh)
+            assert (J >= before).all() and (J <= want).all()
+            if np.array_equal(J, before):
+                break
+        assert np.array_equal(J, want), (h, w, bw, bh)
+
+
+def test_oracle_pass_raster_planted_mistake_is_caught():
+    # A raster pass without its row scans (the column pass alone) differs
+    # from the threshold formulation on an input that needs a sideways step.
+    I = np.array([[5, 5, 5], [0, 0, 5]], np.uint8)
+    S = np.array([[5, 0, 0], [0, 0, 0]], np.uint8)
+    want = _raster_by_thresholds(S, I, 4, 1, 1 << 20, 1 << 20)
+    assert want[1, 2] == 5
+    assert not np.array_equal(oracle.pass_y(S, I, 4, 1, 1 << 20, 1 << 20), want)
+    assert np.array_equal(oracle.pass_raster(S, I, 4, 1, 1 << 20, 1 << 20), want)
