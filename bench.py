@@ -14,6 +14,7 @@ Prints ONE JSON line on stdout; progress goes to stderr.
                     [--schedule both|learned|plain] [--queue auto]
 """
 import argparse
+import fnmatch
 import json
 import os
 import statistics
@@ -33,16 +34,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profile_traffic(workload, prefixes, largest=False):
+def profile_traffic(workload, patterns, largest=False):
     """HBM bytes per launch of a kernel family from the committed rocprofv3
     summary of this workload and queue (profiles/<workload>_<queue>_kernels.md,
     or profiles/<workload>_kernels.md for the batch: FETCH_SIZE and
-    WRITE_SIZE passes, calibrated to bytes there), averaged over the family's
-    rows weighted by their dispatches.  The out-of-place column pass (template
-    mode 0) is the benchmark's isolated single-pass timing, not a launch of
-    the reconstruction, and is left out.  largest: each row's largest dispatch
-    (a batch bench, whose per-image loop shares the rows).  (None, reason)
-    when there is no summary."""
+    WRITE_SIZE passes, calibrated to bytes there), averaged over the rows whose
+    kernel name matches one of `patterns` (shell-style), weighted by their
+    dispatches.  largest: each row's largest dispatch (a batch bench, whose
+    per-image loop shares the rows).  (None, reason) when there is no
+    summary."""
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.md")
     if not os.path.exists(path):
         return None, f"no {os.path.relpath(path, ROOT)}"
@@ -59,18 +59,16 @@ def profile_traffic(workload, prefixes, largest=False):
         if len(cells) <= col or not cells[0].startswith("`"):
             continue
         name = cells[0].strip("`")
-        if not name.startswith(prefixes) or cells[col] == "-":
-            continue
-        if name.startswith("k_col_pass<") and name.endswith(", 0>"):
+        if not any(fnmatch.fnmatchcase(name, p) for p in patterns) or cells[col] == "-":
             continue
         calls = float(cells[1])
         num += calls * float(cells[col]) * 1e6
         den += calls
     if den == 0:
-        return None, f"no {'/'.join(prefixes)} rows in {os.path.relpath(path, ROOT)}"
+        return None, f"no {' | '.join(patterns)} rows in {os.path.relpath(path, ROOT)}"
     what = "largest dispatch of each kernel" if largest else "median dispatch of each kernel"
     return num / den, (f"{os.path.relpath(path, ROOT)}: calibrated FETCH_SIZE + WRITE_SIZE of the "
-                       f"{what}, weighted by dispatches over the {'/'.join(prefixes)} kernels")
+                       f"{what}, weighted by dispatches over the kernels {' | '.join(patterns)}")
 
 
 def cpu_model():
@@ -88,17 +86,17 @@ def cpu_model():
 def single_kernel_roofline(candidates, workload, largest=False):
     """Roofline of the single kernel with the largest average launch time.
     candidates: name -> (total ms per reconstruction, launches, algorithmic
-    bytes per launch, profile prefixes).  Returns (roofline dict, table)."""
+    bytes per launch, profile name patterns).  Returns (roofline dict, table)."""
     table = {}
-    for name, (tot, launches, alg, prefixes) in candidates.items():
+    for name, (tot, launches, alg, patterns) in candidates.items():
         avg = tot / launches if launches else 0.0
         ach = alg / (avg * 1e-3) / 1e9 if avg > 0 else None
         table[name] = {"ms_per_reconstruction": tot, "launches": launches, "avg_launch_ms": avg,
                        "algorithmic_bytes_per_launch": alg, "achieved_GBps": ach,
-                       "frac": ach / HBM_PEAK_GBS if ach else None, "prefixes": prefixes}
+                       "frac": ach / HBM_PEAK_GBS if ach else None, "patterns": patterns}
     dom = max(table, key=lambda k: table[k]["avg_launch_ms"])
     t = table[dom]
-    traffic, traffic_src = profile_traffic(workload, t["prefixes"], largest)
+    traffic, traffic_src = profile_traffic(workload, t["patterns"], largest)
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -115,7 +113,7 @@ def single_kernel_roofline(candidates, workload, largest=False):
         "traffic_basis": traffic_src,
     }
     for v in table.values():
-        v.pop("prefixes")
+        v.pop("patterns")
     return roofline, table
 
 
@@ -129,6 +127,9 @@ def main():
     ap.add_argument("--schedule", default="both", choices=["both", "plain", "learned"],
                     help="both: value is the learned schedule, the plain one is timed beside it")
     ap.add_argument("--sweeps", type=int, default=-1, help="sweep rounds (-1: schedule)")
+    ap.add_argument("--split-sweeps", action="store_true",
+                    help="sweep rounds of a row pass and two column passes (default: two raster "
+                         "passes)")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=5)
@@ -163,7 +164,8 @@ def main():
     marker0 = iwpp.DeviceImage.from_numpy(case.marker)
     work = iwpp.DeviceImage(marker0.h, marker0.w, marker0.dtype, marker0.pitch)
     ws = iwpp.Workspace(px, marker0.dtype)
-    opt = iwpp.default_options(sweep_rounds=args.sweeps)
+    split = 1 if args.split_sweeps else 0
+    opt = iwpp.default_options(sweep_rounds=args.sweeps, split_sweeps=split)
 
     from iwpp import learned
 
@@ -210,7 +212,7 @@ def main():
     uniform = {}
     if args.uniform_steps > 0 and args.sweeps < 0:
         for k in range(0, 4):
-            o = iwpp.default_options(sweep_rounds=k)
+            o = iwpp.default_options(sweep_rounds=k, split_sweeps=split)
             ms_k, st_k, out_k = run_schedule("plain", o, args.uniform_steps, 1,
                                              f"plain, {k} rounds")
             uniform[k] = {"ms_per_step": ms_k, "tile_visits": st_k[-1]["tile_visits"],
@@ -243,6 +245,8 @@ def main():
     ms_row = timed(lambda: iwpp.sweep_rows(work, mask), reset_marker)
     ms_col = timed(lambda: iwpp.sweep_cols(work, scratch, mask, case.connectivity, 1,
                                            opt.col_block_rows), reset_marker)
+    ms_raster = timed(lambda: iwpp.sweep_raster(work, scratch, mask, case.connectivity, 1,
+                                                opt.col_block_rows), reset_marker)
     bytes_sweep = 3 * tsize * px  # algorithmic bytes of one directional pass
     phases = {k: statistics.mean(s[k] for s in stats) for k in ("ms_sweep", "ms_seed", "ms_queue")}
     counters = {k: stats[-1][k] for k in ("tile_visits", "seeded", "sweep_rounds", "queue_rounds",
@@ -253,27 +257,43 @@ def main():
     # Per reconstruction: device time of each kernel family inside the timed
     # steps (the library's HIP events on its own stream), its launches, and
     # the algorithmic bytes one launch must move: 3*sizeof(T) per pixel a
-    # sweep pass owns (read marker, read mask, write marker; row and column
-    # passes alike), 3*sizeof(T) per pixel of each megapixel visit (64*64
-    # pixels) in the queue phase, 2*sizeof(T) per pixel for the seeding scan
-    # (an upper bound for the seeding phase: on a first queue attempt the last
-    # column pass sets the flags and k_seed_edges only reads the strip edges).
+    # sweep pass owns (read marker, read mask, write marker; raster, row and
+    # column passes alike), 3*sizeof(T) per pixel of each megapixel visit
+    # (64*64 pixels) in the queue phase, 2*sizeof(T) per pixel for the
+    # seeding scan (an upper bound for the seeding phase: on a first queue
+    # attempt the last pass sets the flags and k_seed_edges only reads the
+    # strip edges).  The rocprofv3 names of the in-place passes end in their
+    # template mode: raster 5 (6: clipping, 7: seeding), column 2 (3: seeding).
     queue_name = "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>")
     ms_rows = statistics.mean(s_["ms_rows"] for s_ in stats)
     ms_cols = statistics.mean(s_["ms_cols"] for s_ in stats)
+    if split:
+        sweeps = {
+            "k_row_pass (forward + backward, one launch per round)":
+                (ms_rows, rounds, bytes_sweep, ("k_row_pass*",)),
+            "k_col_pass (in place, down and up: two launches per round)":
+                (ms_cols, 2 * rounds, bytes_sweep, ("k_col_pass<*, [23]>",)),
+        }
+    else:
+        sweeps = {
+            "k_col_pass raster, downward (in place, clipping in round 0; one launch per round)":
+                (ms_rows, rounds, bytes_sweep, ("k_col_pass<*, 1, [56]>",)),
+            "k_col_pass raster, upward (in place, seeding flags in the last round; one launch "
+            "per round)":
+                (ms_cols, rounds, bytes_sweep, ("k_col_pass<*, -1, [57]>",)),
+        }
     candidates = {
-        "k_row_pass (forward + backward, one launch per round)":
-            (ms_rows, rounds, bytes_sweep, ("k_row_pass",)),
-        "k_col_pass (in place, down and up: two launches per round)":
-            (ms_cols, 2 * rounds, bytes_sweep, ("k_col_pass",)),
-        queue_name: (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize, ("k_queue_",)),
+        **sweeps,
+        queue_name: (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize, ("k_queue_*",)),
         "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
-            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_", "k_unstable_scan")),
+            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_*", "k_unstable_scan*")),
     }
     roofline, per_kernel = single_kernel_roofline(candidates, f"{case.name}_{used_queue}")
-    sweep_avg = phases["ms_sweep"] / max(1, 3 * rounds)
+    passes = (3 if split else 2) * rounds
+    sweep_avg = phases["ms_sweep"] / max(1, passes)
     roofline["sweep_family"] = {
-        "kernels": "k_row_pass*, k_col_pass (every pass of the sweep phase)",
+        "kernels": ("k_row_pass*, k_col_pass" if split else "k_col_pass (raster)") +
+                   " (every pass of the sweep phase)",
         "avg_launch_ms": sweep_avg,
         "achieved": bytes_sweep / (sweep_avg * 1e-3) / 1e9 if sweep_avg > 0 else None,
         "frac": bytes_sweep / (sweep_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if sweep_avg > 0 else None,
@@ -291,8 +311,12 @@ def main():
         "col_pass_ms": ms_col,
         "col_pass_GBps": bytes_sweep / (ms_col * 1e-3) / 1e9,
         "col_pass_frac": bytes_sweep / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "raster_pass_ms": ms_raster,
+        "raster_pass_GBps": bytes_sweep / (ms_raster * 1e-3) / 1e9,
+        "raster_pass_frac": bytes_sweep / (ms_raster * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "per_kernel": per_kernel,
-        "basis": "row_pass / col_pass: one isolated launch each over the clipped marker, HIP events",
+        "basis": "row_pass / col_pass / raster_pass: one isolated launch each over the clipped "
+                 "marker (the column and raster passes out of place), HIP events",
     }
 
     # --- where the queue kernel's waves spend their time: one more (untimed)
@@ -301,7 +325,8 @@ def main():
         learned.install()
     work.buf.copy_(marker0.buf)
     st_tr = iwpp.reconstruct(work, mask, case.connectivity, case.tile, args.queue, main_sched,
-                             iwpp.default_options(sweep_rounds=args.sweeps, trace_phases=1), ws)
+                             iwpp.default_options(sweep_rounds=args.sweeps, trace_phases=1,
+                                                  split_sweeps=split), ws)
     iwpp.set_model(None)
     keys = ("us_wait", "us_fetch", "us_converge", "us_store", "us_notify")
     tot_us = sum(st_tr[k] for k in keys)
@@ -351,7 +376,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": case.name, "pixels": px, "connectivity": case.connectivity,
                    "tile": case.tile, "queue": args.queue, "queue_used": used_queue,
-                   "schedule": main_sched,
+                   "schedule": main_sched, "sweeps": "split" if split else "raster",
                    **case.meta},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -497,15 +522,16 @@ def bench_batch(args):
     rounds = max(1, batch_st["sweep_rounds"])
     tsize = cases[0].mask.itemsize
     candidates = {
-        "k_row_pass_wave (forward + backward, one launch per round)":
-            (batch_st["ms_rows"], rounds, 3 * tsize * px, ("k_row_pass",)),
-        "k_col_pass (in place, down and up: two launches per round)":
-            (batch_st["ms_cols"], 2 * rounds, 3 * tsize * px, ("k_col_pass",)),
+        "k_col_pass raster, downward (in place, clipping in round 0; one launch per round)":
+            (batch_st["ms_rows"], rounds, 3 * tsize * px, ("k_col_pass<*, 1, [56]>",)),
+        "k_col_pass raster, upward (in place, seeding flags in the last round; one launch "
+        "per round)":
+            (batch_st["ms_cols"], rounds, 3 * tsize * px, ("k_col_pass<*, -1, [57]>",)),
         "k_queue_" + ("mutex" if batch_st["queue"] == 0 else "bsp"):
             (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * tsize,
-             ("k_queue_",)),
+             ("k_queue_*",)),
         "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
-            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_", "k_unstable_scan")),
+            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_*", "k_unstable_scan*")),
     }
     roofline, per_kernel = single_kernel_roofline(candidates, "c4_batch_256_u8", largest=True)
     roofline["end_to_end"] = {

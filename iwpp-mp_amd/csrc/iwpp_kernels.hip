@@ -352,15 +352,27 @@ struct LanePx {
 #pragma unroll
         for (int k = 0; k < NP; ++k) c[k] = 0;
     }
+    // The lane's mask minimum (per half with pairs): the Q of its summaries in
+    // both directions, for reduce_masked.
+    __device__ __forceinline__ uint32_t mask_min() const {
+        uint32_t q = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) q = vmin(q, c[k]);
+        return q;
+    }
     // Summary (P, Q) of the lane's pixels in scan direction D.
     template <int D>
     __device__ __forceinline__ void reduce(uint32_t& P, uint32_t& Q) {
-        uint32_t p = 0, q = 0xFFFFFFFFu;
+        reduce_masked<D>(P, Q, mask_min());
+    }
+    // The same with the lane's mask minimum q (mask_min) given.
+    template <int D>
+    __device__ __forceinline__ void reduce_masked(uint32_t& P, uint32_t& Q, uint32_t q) {
+        uint32_t p = 0;
 #pragma unroll
         for (int kk = 0; kk < NP; ++kk) {
             const int k = D > 0 ? kk : NP - 1 - kk;
             p = vmax(v[k], vmin(p, c[k]));
-            q = vmin(q, c[k]);
         }
         if constexpr (!PAIRS) {
             P = p;
@@ -376,8 +388,8 @@ struct LanePx {
     }
     // Scan the lane's pixels in direction D from x, the value next to its
     // first pixel in that direction (after reduce<D>).  Returns whether any
-    // pixel changed.
-    template <int D>
+    // pixel changed (TRACK; always false otherwise).
+    template <int D, bool TRACK = true>
     __device__ __forceinline__ bool apply(uint32_t x) {
         if constexpr (PAIRS) {
             const uint32_t x2 = max(hp, min(x, hq));  // entering the second half
@@ -388,7 +400,7 @@ struct LanePx {
         for (int kk = 0; kk < NP; ++kk) {
             const int k = D > 0 ? kk : NP - 1 - kk;
             x = vmax(v[k], vmin(x, c[k]));
-            d |= x != v[k];
+            if constexpr (TRACK) d |= x != v[k];
             v[k] = x;
         }
         return d;
@@ -1235,14 +1247,16 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
         uint32_t v[NP], c[NP], m[NP];
         L::split(jw, v);
         L::split(iw, c);
+        // A raster row tests for a change once, at the end, against the
+        // loaded values (every step only raises pixels, the clip only lowers
+        // them); a column row tests its one step.
+        uint32_t v0[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) v0[k] = v[k];
         bool d = false;
         if constexpr (CLIP) {
 #pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                const uint32_t t = L::vmin(v[k], c[k]);
-                d |= t != v[k];
-                v[k] = t;
-            }
+            for (int k = 0; k < NP; ++k) v[k] = L::vmin(v[k], c[k]);
         }
         if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
 #pragma unroll
@@ -1273,7 +1287,7 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
             const uint32_t out = L::vmax(v[k], L::vmin(m[k], c[k]));
-            d |= out != v[k];
+            if constexpr (!RASTER) d |= out != v[k];
             prev[k] = out;
         }
         if constexpr (RASTER) {
@@ -1285,15 +1299,19 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
                 px.v[k] = prev[k];
                 px.c[k] = c[k];
             }
+            const uint32_t qm = px.mask_min();
             uint32_t Pi, Qi;
-            px.template reduce<1>(Pi, Qi);
+            px.template reduce_masked<1>(Pi, Qi, qm);
             scan_lanes_fwd(Pi, Qi);
-            d |= px.template apply<1>(max(from_left(Pi, 0u), min(lc, from_left(Qi, 0xFFFFFFFFu))));
-            px.template reduce<-1>(Pi, Qi);
+            px.template apply<1, false>(max(from_left(Pi, 0u), min(lc, from_left(Qi, 0xFFFFFFFFu))));
+            px.template reduce_masked<-1>(Pi, Qi, qm);
             scan_lanes_bwd(Pi, Qi, lane);
-            d |= px.template apply<-1>(max(from_right(Pi, 0u), min(rc, from_right(Qi, 0xFFFFFFFFu))));
+            px.template apply<-1, false>(max(from_right(Pi, 0u), min(rc, from_right(Qi, 0xFFFFFFFFu))));
 #pragma unroll
-            for (int k = 0; k < NP; ++k) prev[k] = px.v[k];
+            for (int k = 0; k < NP; ++k) {
+                prev[k] = px.v[k];
+                d |= prev[k] != v0[k];
+            }
         }
         any_change |= d;
         if (!INPLACE || d) {  // in place: only vectors that rose are written
@@ -1314,17 +1332,22 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
                 eR = here ? (L::PAIRS ? (l >> 16) | (cl & 0xFFFF0000u) : l | (cl << 16)) : eR;
             }
             // row y against its own neighbours and row r1 = y - DIR, and
-            // r1 against row y (the row before y in the block: i > 0)
+            // r1 against row y (the row before y in the block: i > 0).  A
+            // raster row passes the first test by construction: its column
+            // step took r1's final pixels inside the strip, and its scans
+            // leave it stable along the strip.
             uint32_t h[NP];
-            nbmax(prev, h, false);
+            if constexpr (CONN == 8 || !RASTER) nbmax(prev, h, false);
             bool uy = false, ur = false;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 const uint32_t dn = CONN == 8 ? L::vmax(h[k], prev[k]) : prev[k];
-                const uint32_t q = i > 0 ? L::vmax(h[k], v1[k]) : h[k];
-                uy |= L::vmax(L::vmin(q, c[k]), prev[k]) != prev[k];
+                if constexpr (!RASTER) {
+                    const uint32_t q = i > 0 ? L::vmax(h[k], v1[k]) : h[k];
+                    uy |= L::vmax(L::vmin(q, c[k]), prev[k]) != prev[k];
+                    v1[k] = dn;
+                }
                 if (i > 0) ur |= L::vmax(L::vmin(dn, c1[k]), r1[k]) != r1[k];
-                v1[k] = dn;
                 r1[k] = prev[k];
                 c1[k] = c[k];
             }
