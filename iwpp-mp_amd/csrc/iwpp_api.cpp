@@ -1043,7 +1043,7 @@ int plan_and_setup(Run& r, const iwpp_image* imgs, int n, int tile, int queue, i
         maxh = std::max(maxh, imgs[i].height);
         px += (long long)imgs[i].width * imgs[i].height;
     }
-    if (plan_schedule(n == 1 ? imgs[0].width : maxw, n == 1 ? imgs[0].height : maxh, r.dtype,
+    if (plan_schedule(n == 1 ? imgs[0].width : maxw, n == 1 ? imgs[0].height : maxh, n, r.dtype,
                       r.conn, tile, queue, schedule, r.opt.sweep_rounds, plan))
         return fail(IWPP_ERR_ARG, "no schedule for this configuration");
     bool ready = false;
@@ -1437,8 +1437,8 @@ extern "C" int iwpp_autotune(const iwpp_image* imgs, int n, int dtype, int conn,
     if (rc) return rc;
     if (he != hipSuccess) return fail(IWPP_ERR_HIP, "autotune: %s", hipGetErrorString(he));
     SchedulePlan plan;
-    plan_schedule(maxside, maxside, dtype, conn, 0, top.queue, IWPP_SCHED_PLAIN, top.sweep_rounds,
-                  &plan);
+    plan_schedule(maxside, maxside, n, dtype, conn, 0, top.queue, IWPP_SCHED_PLAIN,
+                  top.sweep_rounds, &plan);
     top.tile = plan.tile;
     tune_store(tune_key(dtype, conn, n, px, maxside, sclass),
                TuneValue{top.queue, top.sweep_rounds, top.col_block_rows});

@@ -458,6 +458,37 @@ DEV Image image_of(const ImgDesc& d) { return Image{d.J, d.I, d.pj, d.pi, d.W, d
 DEV uint64_t first_lane64(uint64_t v) {
     return (uint64_t)first_lane((uint32_t)v) | (uint64_t)first_lane((uint32_t)(v >> 32)) << 32;
 }
+// The descriptor with every field in scalar registers.  It is the same in
+// every lane, but read from the device table (several images) it arrives in
+// vector registers (the compiler cannot use scalar loads on memory the kernel
+// may write), and everything computed from it -- block geometry, row
+// addresses, 64-bit offsets -- would then be computed per lane.
+DEV ImgDesc scalar_desc(ImgDesc d) {
+    d.J = (uint8_t*)first_lane64((uint64_t)d.J);
+    d.I = (const uint8_t*)first_lane64((uint64_t)d.I);
+    d.P = (uint8_t*)first_lane64((uint64_t)d.P);
+    d.pj = (size_t)first_lane64((uint64_t)d.pj);
+    d.pi = (size_t)first_lane64((uint64_t)d.pi);
+    d.pp = (size_t)first_lane64((uint64_t)d.pp);
+    d.W = (int)first_lane((uint32_t)d.W);
+    d.H = (int)first_lane((uint32_t)d.H);
+    d.nmx = (int)first_lane((uint32_t)d.nmx);
+    d.nmy = (int)first_lane((uint32_t)d.nmy);
+    d.R = (int)first_lane((uint32_t)d.R);
+    d.strips = (int)first_lane((uint32_t)d.strips);
+    d.mp_base = (int)first_lane((uint32_t)d.mp_base);
+    d.row_base = (int)first_lane((uint32_t)d.row_base);
+    d.cw_base = (int)first_lane((uint32_t)d.cw_base);
+    d.sw_base = (int)first_lane((uint32_t)d.sw_base);
+    d.tm.active = (const uint8_t*)first_lane64((uint64_t)d.tm.active);
+    d.tm.tw = (int)first_lane((uint32_t)d.tm.tw);
+    d.tm.th = (int)first_lane((uint32_t)d.tm.th);
+    d.tm.ntx = (int)first_lane((uint32_t)d.tm.ntx);
+    d.rounds = (int)first_lane((uint32_t)d.rounds);
+    d.edge_base = (long long)first_lane64((uint64_t)d.edge_base);
+    d.se_base = (int)first_lane((uint32_t)d.se_base);
+    return d;
+}
 // The descriptor of the image owning global megapixel m, for the queue
 // kernels: the kernel argument for a single image (ONE: scalar registers, no
 // load), else the device table, the fields a visit uses taken through
@@ -996,7 +1027,7 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
     using L = LanePx<T, VB>;
     constexpr int NW = V::NW, CB = 64 * VB, WPB = kWG / 64;
     const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     bool any_change = false;
     for (int r = blockIdx.x * WPB + wv; r < ds.total; r += gridDim.x * WPB) {
         const ImgDesc d = desc(ds, find_row(ds, r));
@@ -1115,13 +1146,25 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
     const int xs0 = xb0 / (int)sizeof(T);             // first pixel of the strip
     const int xs1 = (xb0 + 64 * VB) / (int)sizeof(T); // first pixel right of the strip
     const int y0 = seg * R, y1 = min(H, y0 + R);
-    const bool on = o < rowbytes ? tile_on(tm, o / (int)sizeof(T), y0) : false;
-    // With a tile schedule, a block whose tiles are all off (at its first row)
-    // is skipped in both directions of a round (the same test in both), so its
-    // marker keeps its value, which neighbouring blocks read as a lower bound.
-    // (Out of place, a skipped block leaves the destination plane as it was:
-    // only the in-place mode is used with tile schedules.)
-    if (tm.active && !wave_any(on)) return;
+    // With a tile schedule, the lane's tile is read once per tile row the
+    // block spans (a block of at most tm.th rows spans two; taller ones read
+    // it again at each row, the slow way): a per-row read of the table in the
+    // row loop would hold every row on a dependent load.  A block whose tiles
+    // are all off is skipped in both directions of a round (the same test in
+    // both), so its marker keeps its value, which neighbouring blocks read as
+    // a lower bound.  (Out of place, a skipped block leaves the destination
+    // plane as it was: only the in-place mode is used with tile schedules.)
+    const bool lane_px = o < rowbytes;
+    const int ytb = tm.active ? (y0 / tm.th + 1) * tm.th : 0;  // first row of the next tile row
+    const bool two_rows = tm.active == nullptr || y1 - 1 < ytb + tm.th;
+    const bool on0 = lane_px && tile_on(tm, o / (int)sizeof(T), y0);
+    const bool on1 = lane_px && tile_on(tm, o / (int)sizeof(T), min(y1 - 1, ytb));
+    if (tm.active && two_rows && !wave_any(on0 || on1)) return;
+    auto row_on = [&](int y) -> bool {
+        if (tm.active == nullptr) return lane_px;
+        if (two_rows) return y < ytb ? on0 : on1;
+        return lane_px && tile_on(tm, o / (int)sizeof(T), y);
+    };
     // The previous output row, as the lane's pixel registers (16-bit pairs
     // with 16-byte lanes: pixel k and k + NP in one register).
     uint32_t prev[NP];
@@ -1258,7 +1301,7 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
 #pragma unroll
             for (int k = 0; k < NP; ++k) v[k] = L::vmin(v[k], c[k]);
         }
-        if (!on || (tm.active && !tile_on(tm, o / (int)sizeof(T), y))) {
+        if (!row_on(y)) {
 #pragma unroll
             for (int k = 0; k < NP; ++k) c[k] = 0;
         }
@@ -1395,9 +1438,11 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
 template <class T, int VB, int CONN, int DIR, int MODE>
 __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, uint8_t* flags,
                                                   uint32_t* edges) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's index, in a scalar register: every address derived from it
+    // (block geometry, row bases) is then computed once per wave, not per lane
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (w >= ds.total) return;
-    const ImgDesc d = desc(ds, find_cw(ds, w));
+    const ImgDesc d = scalar_desc(desc(ds, find_cw(ds, w)));
     if (ds.round >= d.rounds) return;  // this image's sweep rounds are done
     const int local = w - d.cw_base;
     const int strip = local % d.strips, seg = local / d.strips;
@@ -2018,9 +2063,11 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
     constexpr int LPM = kMP / NPX;          // lanes per megapixel
     constexpr int MPS = 64 / LPM;           // megapixels per strip
     const int lane = lane_id();
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's index, in a scalar register: every address derived from it
+    // (block geometry, row bases) is then computed once per wave, not per lane
+    const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (w >= ds.total) return;
-    const ImgDesc dsc = desc(ds, find_sw(ds, w));
+    const ImgDesc dsc = scalar_desc(desc(ds, find_sw(ds, w)));
     const Image im = image_of(dsc);
     const int nmx = dsc.nmx;
     uint8_t* flags = flags_all + dsc.mp_base;
@@ -2144,9 +2191,10 @@ __global__ __launch_bounds__(kWG) void k_unstable_scan(Descs ds, uint8_t* flags_
 template <class T, int CONN>
 __global__ __launch_bounds__(kWG) void k_seed_edges(Descs ds, const uint32_t* edges_all, int spx,
                                                      uint8_t* flags) {
-    const int gw = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)), lane = lane_id();
+    const int gw = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
+    const int lane = lane_id();
     if (gw >= ds.total) return;  // the whole wave
-    const ImgDesc d = desc(ds, search_se(ds, gw));
+    const ImgDesc d = scalar_desc(desc(ds, search_se(ds, gw)));
     const Image im = image_of(d);
     const int w = gw - d.se_base, R = d.R, mp_base = d.mp_base, nmx = d.nmx;
     const int nedges = d.strips - 1, nbounds = (im.H + R - 1) / R - 1;

@@ -1,11 +1,18 @@
-// Plain heuristic schedule, from measurements on MI355X: the mutex variant was
-// fastest or tied on 4096^2 u8 seeds and 4096^2 u16 h-dome images.  Whole
-// reconstructions with k = 0, 1, 2, 3 uniform sweep rounds (ms): 1024^2 h-dome
-// 0.178 0.176 0.197 0.216; 4096^2 seeds 0.353 0.351 0.353 0.387; 6144^2 seeds
-// 0.496 0.518 0.494 0.544; 8192^2 seeds 0.698 0.690 0.596 0.668; 8192^2 h-dome
-// 0.707 0.684 0.642 0.657; 16384^2 u16 h-dome 1.97 1.51 1.97 2.68; 32768^2
-// seeds 6.24 3.09 4.07 5.48.  Here: 2 rounds for short sides of 6144..16383
-// pixels, 1 otherwise.  The tile is 1024 (2048 when the short side is >= 8192).
+// Plain heuristic schedule, from measurements on MI355X with raster sweep
+// rounds (two passes per round): whole reconstructions with k = 0, 1, 2
+// uniform rounds (ms), mutex queue: 1024^2 h-dome 0.187 0.185 0.214;
+// 2048^2 h-dome 0.199 0.212 0.241; 4096^2 0.1% seeds 0.311 0.339 0.359;
+// 6144^2 seeds 0.472 0.528 0.494; 8192^2 (13 images of both generators,
+// seed densities 30-1000 ppm, blur 8-32, u8 and u16) in total 8.61 8.28 8.38
+// with the best k per image 0, 1 or 2; 12288^2 seeds 1.153 1.081 1.13;
+// 16384^2 u16 h-dome 1.785 1.013 1.329; 32768^2 seeds 6.31 2.50 3.37.  So:
+// one image whose sides are both below 8192 pixels takes no sweep round (its
+// queue phase is short: a round costs more than it saves), any other one
+// round; the longer side decides, since a long thin image would otherwise
+// leave the queue one long chain of megapixels.  A batch takes one round (its
+// single queue phase serves every image: 256 images of 256..4096 pixels a
+// side, 1 round 2.03 ms against more without).  The tile is 1024 (2048 when
+// the short side is >= 8192).
 #include "iwpp_sched.h"
 
 #include <algorithm>
@@ -62,16 +69,17 @@ void tune_clear() {
     g_tune.clear();
 }
 
-int plan_schedule(int w, int h, int dtype, int conn, int tile, int queue, int schedule,
-                  int sweep_rounds, SchedulePlan* out) {
+int plan_schedule(int w, int h, int nimages, int dtype, int conn, int tile, int queue,
+                  int schedule, int sweep_rounds, SchedulePlan* out) {
     (void)dtype;
     (void)conn;
     (void)schedule;
-    if (!out || w <= 0 || h <= 0) return -1;
-    const int side = std::min(w, h);
+    if (!out || w <= 0 || h <= 0 || nimages <= 0) return -1;
+    const int side = std::min(w, h), longer = std::max(w, h);
     out->tile = tile > 0 ? tile : (side >= 8192 ? 2048 : 1024);
     out->queue = queue == IWPP_QUEUE_AUTO ? IWPP_QUEUE_MUTEX : queue;
-    out->sweep_rounds = sweep_rounds >= 0 ? sweep_rounds : (side >= 6144 && side < 16384 ? 2 : 1);
+    out->sweep_rounds = sweep_rounds >= 0 ? sweep_rounds
+                                          : (nimages == 1 && longer < 8192 ? 0 : 1);
     return 0;
 }
 

@@ -11,8 +11,9 @@ Traces.  For one training image, with the schedule's tile size:
   * the features of every tile on the input marker (what the library sees),
   * d_t(k): megapixels of tile t whose marker still differs from the
     reconstruction after k full sweep rounds, k = 0..K, using the library's
-    own row and column passes (the same kernels and column blocking as a
-    reconstruction); the queue phase visits each of them at least once,
+    own raster passes (the same kernels and column blocks as a
+    reconstruction, out of place); the queue phase visits each of them at
+    least once,
   * the tile's mean reconstructed value.
 Labels.  For every training image the whole reconstruction is timed with
 k = 0..4 uniform sweep rounds; every tile of the image gets the fastest k,
@@ -35,8 +36,8 @@ import time
 import numpy as np
 
 from . import (MLP_FEATURES, MLP_PARAMS, MP_SIZE, DeviceImage, Workspace, clip, default_options,
-               mlp_adam, mlp_backward, mlp_forward, reconstruct, set_model, sweep_cols,
-               sweep_rows, tile_features)
+               mlp_adam, mlp_backward, mlp_forward, reconstruct, set_model, sweep_raster,
+               tile_features)
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MODEL_DIR = os.path.join(_PKG, "model")
@@ -160,18 +161,20 @@ def trace(mask, marker, connectivity, rounds=MAX_ROUNDS, tile=None):
     I = DeviceImage.from_numpy(mask)
     J = DeviceImage.from_numpy(marker)
     F = DeviceImage.from_numpy(marker)
-    reconstruct(F, I, connectivity=connectivity, queue="mutex")
+    st = reconstruct(F, I, connectivity=connectivity, queue="mutex")
     _, feat = tile_features(J, I, tile, connectivity, FEATURE_SUB)
     feats = feat.cpu().numpy().astype(np.float32)
     P = DeviceImage(h, w, I.dtype)
     clip(J, I)
     dirty = []
+    # the reconstruction's sweep rounds: raster passes down and up over its
+    # column blocks (out of place here, so that the trace is deterministic)
+    rows = st["col_block_rows"]
     for k in range(rounds + 1):
         dirty.append(_tile_sums(_dirty_megapixels(J, F), tile))
         if k < rounds:
-            sweep_rows(J, I)
-            sweep_cols(J, P, I, connectivity, +1, 0)
-            sweep_cols(P, J, I, connectivity, -1, 0)
+            sweep_raster(J, P, I, connectivity, +1, rows)
+            sweep_raster(P, J, I, connectivity, -1, rows)
     torch.cuda.synchronize()
     final = F.numpy().astype(np.float64)
     nty, ntx = -(-h // tile), -(-w // tile)

@@ -145,9 +145,15 @@ static void sched_checks() {
     iwpp::SchedulePlan p;
     for (int w : {1, 100, 4096, 32768})
         for (int q : {IWPP_QUEUE_MUTEX, IWPP_QUEUE_DQ, IWPP_QUEUE_UQ, IWPP_QUEUE_AUTO}) {
-            CHECK(iwpp::plan_schedule(w, w, IWPP_U8, 8, 0, q, IWPP_SCHED_PLAIN, -1, &p) == 0);
-            CHECK(p.tile % 64 == 0 && p.sweep_rounds >= 0);
+            for (int n : {1, 7}) {
+                CHECK(iwpp::plan_schedule(w, w, n, IWPP_U8, 8, 0, q, IWPP_SCHED_PLAIN, -1, &p) == 0);
+                CHECK(p.tile % 64 == 0 && p.sweep_rounds >= 0);
+                CHECK(p.sweep_rounds == (n == 1 && w < 8192 ? 0 : 1));
+            }
         }
+    CHECK(iwpp::plan_schedule(1, 100000, 1, IWPP_U8, 8, 0, 0, IWPP_SCHED_PLAIN, -1, &p) == 0 &&
+          p.sweep_rounds == 1);  // a long thin image sweeps
+    CHECK(iwpp::plan_schedule(64, 64, 0, IWPP_U8, 8, 0, 0, IWPP_SCHED_PLAIN, -1, &p) < 0);
     iwpp::tune_store(iwpp::tune_key(1, 8, 1, 1 << 20, 1024, 0), iwpp::TuneValue{2, 3, 256});
     iwpp::TuneValue tv{};
     CHECK(iwpp::tune_lookup(iwpp::tune_key(1, 8, 1, 1 << 20, 1024, 0), &tv) && tv.queue == 2 &&
