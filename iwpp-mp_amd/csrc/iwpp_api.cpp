@@ -957,11 +957,17 @@ int learned_plan_end_batch(Run& r, SchedulePlan& plan) {
     return IWPP_OK;
 }
 
-// One image: tile t runs k_t = its predicted rounds (at least the first,
-// which has already been launched); the image runs R = the largest k_t that
-// tiles of at least 1/64 of the image ask for (a tile asking for more gets R),
-// and a tile with k_t < R skips the rounds from k_t on (tile masks); the
-// seeding pass visits tiles by decreasing predicted priority.
+// One image, tile t predicted to take k_t rounds (at least the first, which
+// has already been launched).  Raster sweeps: the image runs R = the
+// area-weighted mean of the k_t, rounded, in every tile.  A raster pass lasts
+// as long as one wave takes down its column block, whatever the number of
+// blocks that take part, so a round masked to part of the image costs about
+// as much as a whole one (measured at 16384^2: a second round on a quarter of
+// the tiles 0.45 ms, on all of them 0.45 ms): per-tile rounds buy nothing.
+// Split sweeps (whose row passes skip the chunks of tiles that are off): R =
+// the largest k_t that tiles of at least 1/64 of the image ask for (a tile
+// asking for more gets R), and a tile with k_t < R skips the rounds from k_t
+// on (tile masks).
 int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     iwpp_workspace* ws = r.ws;
     const ImgDesc& d = r.h[0];
@@ -976,7 +982,7 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
     uint8_t* h_masks = (uint8_t*)(h_order + 2 * nt + 1);
     std::vector<int> k(nt);
     std::vector<double> area_at(6, 0.0);  // image area asking for exactly r rounds
-    double total_area = 0;
+    double total_area = 0, weighted = 0;
     for (int t = 0; t < nt; ++t) {
         const int tx = t % ntx, ty = t / ntx;
         const double area = (double)(std::min(d.W, (tx + 1) * tile) - tx * tile) *
@@ -984,13 +990,18 @@ int learned_plan_end_single(Run& r, SchedulePlan& plan) {
         k[t] = std::max(1, clamp_rounds(pred[(size_t)t * 2]));
         area_at[k[t]] += area;
         total_area += area;
+        weighted += area * k[t];
     }
     int rounds = 4;
-    while (rounds > 1 && area_at[rounds] < total_area / 64.0) {
-        area_at[rounds - 1] += area_at[rounds];
-        --rounds;
+    if (!r.opt.split_sweeps) {
+        rounds = std::max(1, (int)std::lround(weighted / std::max(1.0, total_area)));
+    } else {
+        while (rounds > 1 && area_at[rounds] < total_area / 64.0) {
+            area_at[rounds - 1] += area_at[rounds];
+            --rounds;
+        }
     }
-    for (int t = 0; t < nt; ++t) k[t] = std::min(k[t], rounds);
+    for (int t = 0; t < nt; ++t) k[t] = r.opt.split_sweeps ? std::min(k[t], rounds) : rounds;
     plan.sweep_rounds = rounds;
     r.st.schedule = IWPP_SCHED_LEARNED;
     int skipped = 0;

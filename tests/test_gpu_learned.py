@@ -196,27 +196,34 @@ def _spread_model(marker, mask, tile, conn):
 @pytest.mark.parametrize("conn", [4, 8])
 @pytest.mark.parametrize("queue", ["mutex", "dq", "uq"])
 def test_learned_reconstruction_exact(torch_cuda, model_guard, dtype, conn, queue):
+    # Raster sweeps run the plan's rounds in every tile; split sweeps (the
+    # wide shape, whose tiles differ) run later rounds with tile masks.
     rng = np.random.default_rng(zlib.crc32(f"{dtype.__name__}/{conn}/{queue}".encode()))
     masked_wide = 0  # runs of the wide shape (rows held in registers) with tile masks
     for mi, (name, kb) in enumerate(MODELS):
         iwpp.set_model(_model_with_bias(mi + 3, kb))
         for shape, tile, kind in (((300, 517), 64, "seeds"), ((700, 390), 128, "hdome"),
                                   ((129, 1000), 64, "seeds"), ((90, 5000), 256, "seeds")):
-            say("learned", name, dtype.__name__, conn, queue, shape, tile, kind)
             marker, mask = rand_case(rng, *shape, dtype, kind)
             if shape[1] == 5000:  # tiles that differ: h-dome marker on the left third
                 hd = np.maximum(mask.astype(np.int64) - int(mask.max()) // 5, 0)
                 marker[:, :1700] = hd[:, :1700].astype(dtype)
                 iwpp.set_model(_spread_model(marker, mask, tile, conn))
             want = oracle.reconstruct(marker, mask, conn)
-            J, I = iwpp.DeviceImage.from_numpy(marker), iwpp.DeviceImage.from_numpy(mask)
-            st = iwpp.reconstruct(J, I, connectivity=conn, tile=tile, queue=queue,
-                                  schedule="learned",
-                                  options=iwpp.default_options(learned_min_pixels=0))
-            assert st["schedule"] == iwpp.SCHEDULES["learned"]
-            got = J.numpy()
-            assert np.array_equal(got, want), (name, shape, tile, kind, int((got != want).sum()))
-            masked_wide += shape[1] == 5000 and st["sweeps_skipped_tiles"] > 0
+            for split in ((0, 1) if shape[1] == 5000 else (0,)):
+                say("learned", name, dtype.__name__, conn, queue, shape, tile, kind, split)
+                J, I = iwpp.DeviceImage.from_numpy(marker), iwpp.DeviceImage.from_numpy(mask)
+                st = iwpp.reconstruct(J, I, connectivity=conn, tile=tile, queue=queue,
+                                      schedule="learned",
+                                      options=iwpp.default_options(learned_min_pixels=0,
+                                                                   split_sweeps=split))
+                assert st["schedule"] == iwpp.SCHEDULES["learned"]
+                got = J.numpy()
+                assert np.array_equal(got, want), (name, shape, tile, kind, split,
+                                                   int((got != want).sum()))
+                if not split:
+                    assert st["sweeps_skipped_tiles"] == 0  # raster: uniform rounds
+                masked_wide += split and st["sweeps_skipped_tiles"] > 0
     assert masked_wide > 0  # the mixed models skip tiles in later rounds
 
 
