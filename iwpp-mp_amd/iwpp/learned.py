@@ -17,7 +17,9 @@ Traces.  For one training image, with the schedule's tile size:
   * the tile's mean reconstructed value.
 Labels.  For every training image the whole reconstruction is timed with
 k = 0..4 uniform sweep rounds; every tile of the image gets the fastest k,
-(the fewest rounds among those within 2% of it), and priority_t = mean
+(the fewest rounds among those within 2% of it; labels(..., per_tile=True)
+gives each tile its own k from a cost model fitted to the timings and the
+dirty counts, for split sweeps), and priority_t = mean
 reconstructed value of t / max mask value, so tiles
 whose flood carries the highest values are seeded first (their propagation
 overwrites lower floods, which would otherwise be propagated and then
@@ -222,20 +224,23 @@ def cost_model(timings, dirty_total):
     return (a, b, c) if b > 0 and c > 0 else None
 
 
-def labels(tr, timings):
+def labels(tr, timings, per_tile=False):
     """(k, priority) per tile.
-    k: with a physical cost model of the image (cost_model), each tile's own
+    k: the image's k* = the fewest uniform rounds within TIE of the fastest
+    measured, for every tile: the library runs an image's rounds in every
+    tile under raster sweeps (a masked raster round costs about a whole one).
+    per_tile (for split sweeps, whose later rounds can be masked per tile):
+    with a physical cost model of the image (cost_model), each tile's own
     argmin over k of  k * b * (tile pixels / image pixels) + c * d_t(k)
     (its share of the sweep rounds against the queue work its dirty
-    megapixels leave), ties to fewer rounds; otherwise the image's k* = the
-    fewest uniform rounds within TIE of the fastest measured.
+    megapixels leave), ties to fewer rounds; k* when the fit is not physical.
     priority: the tile's mean reconstructed value over the image's largest
     mask value."""
     t = np.asarray(timings, dtype=np.float64)
     k_img = float(np.flatnonzero(t <= TIE * t.min())[0])  # fewest rounds within TIE of the best
     prio = (tr["final_mean"] / max(tr["maxval"], 1.0)).astype(np.float32)
     d = np.asarray(tr["dirty"], dtype=np.float64)[:, : t.size] if "dirty" in tr else None
-    fit = cost_model(t, d.sum(0)) if d is not None else None
+    fit = cost_model(t, d.sum(0)) if d is not None and per_tile else None
     if fit is None:
         k = np.full(prio.shape, k_img, np.float32)
     else:

@@ -228,7 +228,8 @@ def test_labels_per_tile_cost_model():
     fit = learned.cost_model(t, D)
     np.testing.assert_allclose(fit, (a, b, c), rtol=1e-6)
     tr = {"final_mean": np.zeros(3), "maxval": 1.0, "dirty": dirty, "pixels": px}
-    y = learned.labels(tr, t)
+    assert learned.labels(tr, t)[:, 0].tolist() == [2, 2, 2]  # the image rule: its k* for all
+    y = learned.labels(tr, t, per_tile=True)
     share = px / px.sum()
     want = [int(np.argmin(np.arange(5) * b * share[i] + c * dirty[i])) for i in range(3)]
     assert y[:, 0].tolist() == want
