@@ -2577,13 +2577,15 @@ void k_queue_mutex(Descs ds, QueueArgs q) {
     PhaseTimer pt(q.trace != 0);
     const uint32_t seeded = a_load(ctl_at(q.ctl, C_SEEDED));
     const uint32_t nwaves = gridDim.x * (kWG / 64);
-    // Batch: about three batches per wave over the seeded region, 1..16.
+    // Batch: the seeded region shared out evenly, one batch per wave, 1..16.
     // Consecutive tickets are neighbouring megapixels: one wave visiting them
     // in turn fetches halos its previous visit already raised, so larger
-    // batches also cut re-visits (c5: 58.9K visits one at a time, 42.8K in
-    // batches of 16), until too few waves share the work (measured
-    // optimum: 8-16 at c5, 4 at c3, 1-2 at c2).
-    const uint32_t B = max(1u, min(16u, seeded / (3u * nwaves)));
+    // batches also cut re-visits.  Measured at 32768^2 after one raster round
+    // (33K seeded megapixels, 4096 waves): a third of the share per batch
+    // 0.77 ms (72K visits), the share 0.64-0.67 ms (65K), twice the share
+    // 0.70-0.76 ms (too few waves get work); 16384^2 u16 0.275 / 0.281 /
+    // 0.349 ms.
+    const uint32_t B = max(1u, min(16u, seeded / nwaves));
     if (seeded == 0) return;  // `pending` starts at zero: nothing to do
     lq_init(L);
     uint32_t claimed = 0;  // lane k < nclaimed holds the k-th claimed megapixel
