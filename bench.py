@@ -34,19 +34,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profile_traffic(workload, patterns, largest=False):
+def profile_traffic(workload, patterns):
     """HBM bytes per launch of a kernel family from the committed rocprofv3
     summary of this workload and queue (profiles/<workload>_<queue>_kernels.md,
     or profiles/<workload>_kernels.md for the batch: FETCH_SIZE and
     WRITE_SIZE passes, calibrated to bytes there), averaged over the rows whose
     kernel name matches one of `patterns` (shell-style), weighted by their
-    dispatches.  largest: each row's largest dispatch (a batch bench, whose
-    per-image loop shares the rows).  (None, reason) when there is no
-    summary."""
+    dispatches, from each row's median dispatch.  (None, reason) when there is
+    no summary."""
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.md")
     if not os.path.exists(path):
         return None, f"no {os.path.relpath(path, ROOT)}"
-    col = 8 if largest else 7
+    col = 7
     num = den = 0.0
     table = False  # inside the traffic table (the first one whose header names FETCH_SIZE)
     for line in open(path):
@@ -66,9 +65,8 @@ def profile_traffic(workload, patterns, largest=False):
         den += calls
     if den == 0:
         return None, f"no {' | '.join(patterns)} rows in {os.path.relpath(path, ROOT)}"
-    what = "largest dispatch of each kernel" if largest else "median dispatch of each kernel"
     return num / den, (f"{os.path.relpath(path, ROOT)}: calibrated FETCH_SIZE + WRITE_SIZE of the "
-                       f"{what}, weighted by dispatches over the kernels {' | '.join(patterns)}")
+                       f"median dispatch of each kernel, weighted by dispatches over the kernels {' | '.join(patterns)}")
 
 
 def cpu_model():
@@ -83,7 +81,29 @@ def cpu_model():
     return None
 
 
-def single_kernel_roofline(candidates, workload, largest=False):
+def seeding_bytes(shapes, tsize, fused, block_rows, seeded):
+    """Algorithmic bytes of the seeding phase over images of `shapes` (h, w).
+    Without sweeps k_unstable_scan reads every pixel's marker and mask.  When
+    the last sweep pass has set the megapixel flags (fused), k_seed_edges reads
+    only what that pass could not test: the 4-byte border words of both sides of
+    each inner strip edge (strips of 1024 bytes: 64 lanes of 16 bytes) for every
+    row, and marker and mask of the two rows beside each block boundary.  Both
+    then write one flag byte per megapixel, and k_seed_queue reads the flags and
+    writes an 8-byte ring entry per seeded megapixel."""
+    total = 0
+    for h, w in shapes:
+        nmp = -(-h // 64) * -(-w // 64)
+        if fused:
+            strips = -(-w * tsize // 1024)
+            bounds = max(0, -(-h // block_rows) - 1)
+            total += 8 * h * max(0, strips - 1) + bounds * 2 * 2 * tsize * w
+        else:
+            total += 2 * tsize * h * w
+        total += 2 * nmp
+    return total + 8 * seeded
+
+
+def single_kernel_roofline(candidates, workload):
     """Roofline of the single kernel with the largest average launch time.
     candidates: name -> (total ms per reconstruction, launches, algorithmic
     bytes per launch, profile name patterns).  Returns (roofline dict, table)."""
@@ -96,7 +116,7 @@ def single_kernel_roofline(candidates, workload, largest=False):
                        "frac": ach / HBM_PEAK_GBS if ach else None, "patterns": patterns}
     dom = max(table, key=lambda k: table[k]["avg_launch_ms"])
     t = table[dom]
-    traffic, traffic_src = profile_traffic(workload, t["patterns"], largest)
+    traffic, traffic_src = profile_traffic(workload, t["patterns"])
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -259,10 +279,8 @@ def main():
     # the algorithmic bytes one launch must move: 3*sizeof(T) per pixel a
     # sweep pass owns (read marker, read mask, write marker; raster, row and
     # column passes alike), 3*sizeof(T) per pixel of each megapixel visit
-    # (64*64 pixels) in the queue phase, 2*sizeof(T) per pixel for the
-    # seeding scan (an upper bound for the seeding phase: on a first queue
-    # attempt the last pass sets the flags and k_seed_edges only reads the
-    # strip edges).  The rocprofv3 names of the in-place passes end in their
+    # (64*64 pixels) in the queue phase, and for the seeding phase what
+    # seeding_bytes() counts.  The rocprofv3 names of the in-place passes end in their
     # template mode: raster 5 (6: clipping, 7: seeding), column 2 (3: seeding).
     queue_name = "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>")
     ms_rows = statistics.mean(s_["ms_rows"] for s_ in stats)
@@ -286,7 +304,10 @@ def main():
         **sweeps,
         queue_name: (phases["ms_queue"], 1, visits * 64 * 64 * 3 * tsize, ("k_queue_*",)),
         "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
-            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_*", "k_unstable_scan*")),
+            (phases["ms_seed"], 1,
+             seeding_bytes([case.marker.shape], tsize, rounds > 0, stats[-1]["col_block_rows"],
+                           counters["seeded"]),
+             ("k_seed_*", "k_unstable_scan*")),
     }
     roofline, per_kernel = single_kernel_roofline(candidates, f"{case.name}_{used_queue}")
     passes = (3 if split else 2) * rounds
@@ -529,11 +550,17 @@ def bench_batch(args):
             (batch_st["ms_cols"], rounds, 3 * tsize * px, ("k_col_pass<*, -1, [57]>",)),
         "k_queue_" + ("mutex" if batch_st["queue"] == 0 else "bsp"):
             (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * tsize,
-             ("k_queue_*",)),
+             ("k_queue_mutex<*, false>" if batch_st["queue"] == 0 else "k_queue_bsp<*, false>",)),
         "seeding (k_seed_edges | k_unstable_scan, k_seed_queue)":
-            (phases["ms_seed"], 1, 2 * tsize * px, ("k_seed_*", "k_unstable_scan*")),
+            (phases["ms_seed"], 1,
+             seeding_bytes([c.mask.shape for c in cases], tsize, batch_st["sweep_rounds"] > 0,
+                           batch_st["col_block_rows"], batch_st["seeded"]),
+             ("k_seed_*", "k_unstable_scan*")),
     }
-    roofline, per_kernel = single_kernel_roofline(candidates, "c4_batch_256_u8", largest=True)
+    # Traffic from the median dispatch: the profile's dispatches of the batched
+    # queue kernel (template flag false: several images) are the batched
+    # calls' and the autotuner's trials; the median is a batched call's.
+    roofline, per_kernel = single_kernel_roofline(candidates, "c4_batch_256_u8")
     roofline["end_to_end"] = {
         "basis": "3*sizeof(T)*pixels / time of the fastest batched form",
         "achieved": 3 * tsize * px / (best_ms * 1e-3) / 1e9,
