@@ -103,6 +103,19 @@ def seeding_bytes(shapes, tsize, fused, block_rows, seeded):
     return total + 8 * seeded
 
 
+def seam_bytes(shapes, tsize, block_rows, seam_rows):
+    """Algorithmic bytes of one raster fix-up launch over images of `shapes`
+    (h, w): 3*sizeof(T) per pixel of the windows beside the block boundaries
+    (min(seam_rows, block_rows / 2) rows each; the downward windows, after a
+    boundary, end at the image's last row)."""
+    k = min(seam_rows, block_rows // 2)
+    total = 0
+    for h, w in shapes:
+        rows = sum(min(k, h - yb) for yb in range(block_rows, h, block_rows))
+        total += 3 * tsize * w * rows
+    return total
+
+
 def single_kernel_roofline(candidates, workload):
     """Roofline of the single kernel with the largest average launch time.
     candidates: name -> (total ms per reconstruction, launches, algorithmic
@@ -285,6 +298,7 @@ def main():
     queue_name = "k_queue_" + ("mutex" if used_queue == "mutex" else f"bsp<{used_queue.upper()}>")
     ms_rows = statistics.mean(s_["ms_rows"] for s_ in stats)
     ms_cols = statistics.mean(s_["ms_cols"] for s_ in stats)
+    ms_seams = statistics.mean(s_["ms_seams"] for s_ in stats)
     if split:
         sweeps = {
             "k_row_pass (forward + backward, one launch per round)":
@@ -293,9 +307,17 @@ def main():
                 (ms_cols, 2 * rounds, bytes_sweep, ("k_col_pass<*, [23]>",)),
         }
     else:
+        # the fix-ups run as k_col_pass modes 5 (down) and 7 or 5 (up): their
+        # profile rows are shared with the passes of those modes
+        seam_launch = seam_bytes([case.marker.shape], tsize, stats[-1]["col_block_rows"],
+                                 stats[-1]["seam_rows"])
         sweeps = {
             "k_col_pass raster, downward (in place, clipping in round 0; one launch per round)":
                 (ms_rows, rounds, bytes_sweep, ("k_col_pass<*, 1, [56]>",)),
+            "k_col_pass raster fix-ups beside the block boundaries (down and up: two launches "
+            "per round)":
+                (ms_seams, 2 * rounds if stats[-1]["seam_rows"] > 0 else 0, seam_launch,
+                 ("k_col_pass<*, 1, 5>",)),
             "k_col_pass raster, upward (in place, seeding flags in the last round; one launch "
             "per round)":
                 (ms_cols, rounds, bytes_sweep, ("k_col_pass<*, -1, [57]>",)),
@@ -311,10 +333,10 @@ def main():
     }
     roofline, per_kernel = single_kernel_roofline(candidates, f"{case.name}_{used_queue}")
     passes = (3 if split else 2) * rounds
-    sweep_avg = phases["ms_sweep"] / max(1, passes)
+    sweep_avg = (phases["ms_sweep"] - (0 if split else ms_seams)) / max(1, passes)
     roofline["sweep_family"] = {
-        "kernels": ("k_row_pass*, k_col_pass" if split else "k_col_pass (raster)") +
-                   " (every pass of the sweep phase)",
+        "kernels": ("k_row_pass*, k_col_pass (every pass of the sweep phase)" if split else
+                    "k_col_pass raster (every full pass of the sweep phase; fix-ups excluded)"),
         "avg_launch_ms": sweep_avg,
         "achieved": bytes_sweep / (sweep_avg * 1e-3) / 1e9 if sweep_avg > 0 else None,
         "frac": bytes_sweep / (sweep_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if sweep_avg > 0 else None,
@@ -548,6 +570,11 @@ def bench_batch(args):
         "k_col_pass raster, upward (in place, seeding flags in the last round; one launch "
         "per round)":
             (batch_st["ms_cols"], rounds, 3 * tsize * px, ("k_col_pass<*, -1, [57]>",)),
+        "k_col_pass raster fix-ups beside the block boundaries (down and up: two launches "
+        "per round)":
+            (batch_st["ms_seams"], 2 * rounds if batch_st["seam_rows"] > 0 else 0,
+             seam_bytes([c.mask.shape for c in cases], tsize, batch_st["col_block_rows"],
+                        batch_st["seam_rows"]), ("k_col_pass<*, 1, 5>",)),
         "k_queue_" + ("mutex" if batch_st["queue"] == 0 else "bsp"):
             (phases["ms_queue"], 1, batch_st["tile_visits"] * 64 * 64 * 3 * tsize,
              ("k_queue_mutex<*, false>" if batch_st["queue"] == 0 else "k_queue_bsp<*, false>",)),

@@ -105,6 +105,11 @@ typedef struct {
     int col_block_rows;      /* rows per block of the column passes used */
     int seed_class;          /* seed class the autotuner cache was looked up with
                                 (-1: not measured, no cached tuning applies) */
+    float ms_seams;          /* raster sweeps: device time of the fix-up passes
+                                beside the block boundaries (in ms_sweep, not in
+                                ms_rows / ms_cols) */
+    int seam_rows;           /* rows of those fix-up windows asked for (each is
+                                at most half a block; 0 = none) */
 } iwpp_stats;
 
 /* Tuning knobs of iwpp_reconstruct_ex; iwpp_default_options() fills them. */
@@ -132,6 +137,11 @@ typedef struct {
                              scanned across its strip after its column step;
                              1: a row pass (forward and backward) and two
                              column passes (down, up) */
+    int seam_rows;        /* raster sweeps: after each raster pass, a short pass
+                             of this many rows beside every block boundary
+                             (at most half a block), whose first row reads
+                             the other block's final row; 0 = none, negative
+                             (default -1) = the library's choice */
 } iwpp_options;
 
 void iwpp_default_options(iwpp_options* opt);
@@ -301,22 +311,26 @@ int iwpp_unstable_megapixels(const void* marker, const void* mask, size_t pitch_
                              size_t pitch_mask, int width, int height, int dtype,
                              int connectivity, uint8_t* flags, int* count, void* stream);
 
-/* One sweep round (two raster passes, down and up; with split_sweeps a row
- * pass and two column passes; all in place, clipping first) over n images of
+/* One sweep round (two raster passes, down and up, each followed by its
+ * fix-up pass beside the block boundaries; with split_sweeps a row pass and
+ * two column passes; all in place, clipping first) over n images of
  * one dtype, whose upward pass sets the megapixel seeding flags as a
  * reconstruction's last sweep round does
  * before its queue phase (the test of pairs inside each column block and
  * strip during the pass, then the test across strip edges and block
  * boundaries).  flags (device uint8): one per megapixel, the images' grids
  * (ceil(h/64) x ceil(w/64), row-major) one after another in batch order;
- * 1 = a pixel of that megapixel can still be raised by a neighbour.
+ * 1 = a pixel of that megapixel can still be raised by a neighbour (with
+ * fix-up passes, also some megapixels that the upward pass flagged and its
+ * fix-up made stable: flags are only ever added).
  * block_rows: column block height (0 = as a reconstruction chooses it,
- * otherwise a multiple of 32).  No reference counterpart: the reference
+ * otherwise a multiple of 32).  opt (NULL = defaults): its split_sweeps and
+ * seam_rows apply.  No reference counterpart: the reference
  * seeds every megapixel (IWPP MP/src/globalQueue.cu:83-97); this exposes the
  * pass that replaces it, so that its flags can be checked on their own. */
 int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int connectivity,
-                          int block_rows, int split_sweeps, uint8_t* flags, iwpp_workspace* ws,
-                          void* stream);
+                          int block_rows, const iwpp_options* opt, uint8_t* flags,
+                          iwpp_workspace* ws, void* stream);
 
 /* Allocation-free form of iwpp_reconstruct_ex.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

@@ -101,8 +101,10 @@ struct iwpp_workspace {
     size_t tiles_n = 0;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // phase timing,
                                                                         // [4]: learned plan ready
-    std::vector<hipEvent_t> ev_round;  // per sweep round: row pass start, row pass end,
-                                       // column passes start
+    std::vector<hipEvent_t> ev_round;  // per sweep round (kRoundEvents each): raster --
+                                       // down pass start, down pass end, up pass start,
+                                       // up pass end; split -- row pass start, row pass
+                                       // end, column passes start
     ImgDesc* descs = nullptr;   // batch descriptors (device)
     size_t descs_n = 0;
     uint32_t* img_of_mp = nullptr;  // global megapixel -> image (device)
@@ -127,6 +129,14 @@ struct iwpp_workspace {
 
 namespace {
 constexpr long long kLearnedMinPixels = 1ll << 27;  // one image: plan from 11585^2 up
+// Raster fix-up window (rows beside each block boundary) when the options
+// leave it to the library.  Measured on MI355X (one raster round, mutex
+// queue): 16384^2 u16 h-dome 0.997 ms without, 0.900 with 16 rows (seeded
+// megapixels 12.6K -> 1.4K), 0.948 with 32; 32768^2 u8 seeds 2.154 / 2.144 /
+// 2.131 (32K -> 26.5K / 22.5K seeded): each fix-up launch costs about 35 us.
+constexpr int kSeamRows = 16;
+// Events per timed sweep round (Workspace::ev_round).
+constexpr size_t kRoundEvents = 4;
 }
 
 extern "C" void iwpp_default_options(iwpp_options* o) {
@@ -141,6 +151,7 @@ extern "C" void iwpp_default_options(iwpp_options* o) {
     o->check_queue = 0;
     o->learned_min_pixels = kLearnedMinPixels;
     o->split_sweeps = 0;
+    o->seam_rows = -1;
 }
 
 extern "C" iwpp_workspace* iwpp_workspace_create(size_t max_pixels, int max_dtype) {
@@ -283,6 +294,7 @@ struct Run {
     // the flags in the workspace are those of the marker as it now is
     int seed_round = -1;
     bool flags_ready = false;
+    int seam = 0;  // rows of the raster fix-up windows (0: none)
     // learned plan still to be completed (the network's outputs read, the
     // rounds fixed): before the first upward column pass, which then sets the
     // seeding flags if the plan keeps to one round
@@ -372,6 +384,7 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
     // 32768^2 u8 (2048 waves; queue phase 0.93 -> 0.53 ms against 128 rows),
     // 256 at 16384^2 u16 (2048 waves), 64 at 4096^2 u8 (1024 waves), 32 at
     // 1024^2: about 2048 waves, and at most 64 blocks down a column.
+    r.seam = r.opt.split_sweeps ? 0 : r.opt.seam_rows < 0 ? kSeamRows : r.opt.seam_rows;
     int R = r.opt.col_block_rows;
     if (R <= 0) {
         int maxh = 0;
@@ -491,21 +504,29 @@ int seeding_pass(Run& r, int k, bool* seed) {
 int raster_round(Run& r, int k, hipEvent_t* ev) {
     const int rp = learned_plan_predict(r);
     if (rp) return rp;
+    const Descs ds = with_total(r.ds, r.total_cw, k);
+    Descs fix = ds;
+    fix.seam = r.seam;
     if (ev) HIP_TRY(hipEventRecord(ev[0], r.s));
-    HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, +1,
-                                  k == 0 ? COL_RASTER_CLIP : COL_RASTER, r.wide, nullptr, r.s));
-    if (ev) {
-        HIP_TRY(hipEventRecord(ev[1], r.s));
-        HIP_TRY(hipEventRecord(ev[2], r.s));
-    }
+    HIP_TRY(launch_col_pass_batch(ds, r.dtype, r.conn, +1, k == 0 ? COL_RASTER_CLIP : COL_RASTER,
+                                  r.wide, nullptr, r.s));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], r.s));
+    if (r.seam > 0)
+        HIP_TRY(launch_col_pass_batch(fix, r.dtype, r.conn, +1, COL_RASTER, r.wide, nullptr, r.s));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], r.s));
     int rc = plan_end_now(r);
     if (rc) return rc;
     bool seed = false;
     rc = seeding_pass(r, k, &seed);
     if (rc) return rc;
-    HIP_TRY(launch_col_pass_batch(with_total(r.ds, r.total_cw, k), r.dtype, r.conn, -1,
-                                  seed ? COL_RASTER_SEED : COL_RASTER, r.wide, nullptr, r.s,
-                                  seed ? r.ws->flags : nullptr, seed ? r.ws->edges : nullptr));
+    const int mode = seed ? COL_RASTER_SEED : COL_RASTER;
+    uint8_t* flags = seed ? r.ws->flags : nullptr;
+    uint32_t* edges = seed ? r.ws->edges : nullptr;
+    HIP_TRY(launch_col_pass_batch(ds, r.dtype, r.conn, -1, mode, r.wide, nullptr, r.s, flags, edges));
+    if (ev) HIP_TRY(hipEventRecord(ev[3], r.s));
+    if (r.seam > 0)
+        HIP_TRY(launch_col_pass_batch(fix, r.dtype, r.conn, -1, mode, r.wide, nullptr, r.s, flags,
+                                      edges));
     r.flags_ready = seed;
     return IWPP_OK;
 }
@@ -534,12 +555,12 @@ int sweep_phase(Run& r, int k0, int k1) {
         hipEvent_t* ev = nullptr;  // this round's pass events (timed runs)
         if (r.timed) {
             std::vector<hipEvent_t>& er = r.ws->ev_round;
-            while (er.size() < 3 * (size_t)(k + 1)) {
+            while (er.size() < kRoundEvents * (size_t)(k + 1)) {
                 hipEvent_t e = nullptr;
                 HIP_TRY(hipEventCreate(&e));
                 er.push_back(e);
             }
-            ev = er.data() + 3 * k;
+            ev = er.data() + kRoundEvents * k;
             r.timed_rounds = k + 1;
         }
         if (!r.opt.split_sweeps) {
@@ -1096,6 +1117,7 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
     r.st.tile = plan.tile;
     r.st.schedule = IWPP_SCHED_PLAIN;
     r.st.col_block_rows = r.h.empty() ? 0 : r.h[0].R;
+    r.st.seam_rows = r.seam;
     r.st.seed_class = r.seed_class;
     if (r.graph && r.queue != Q_MUTEX)
         return fail(IWPP_ERR_UNSUPPORTED, "graph capture needs the mutex queue variant");
@@ -1142,12 +1164,20 @@ int reconstruct_run(Run& r, const iwpp_image* imgs, int n, int tile, int queue, 
             HIP_TRY(hipEventElapsedTime(&a, r.ws->ev[0], r.ws->ev[1]));
             const hipEvent_t* er = r.ws->ev_round.data();
             for (int k = 0; k < r.timed_rounds; ++k) {
-                float rows = 0, cols = 0;
-                HIP_TRY(hipEventElapsedTime(&rows, er[3 * k], er[3 * k + 1]));
-                HIP_TRY(hipEventElapsedTime(&cols, er[3 * k + 2],
-                                            k + 1 < r.timed_rounds ? er[3 * k + 3] : r.ws->ev[1]));
+                const hipEvent_t* e = er + kRoundEvents * k;
+                const hipEvent_t end = k + 1 < r.timed_rounds ? e[kRoundEvents] : r.ws->ev[1];
+                float rows = 0, cols = 0, fix0 = 0, fix1 = 0;
+                HIP_TRY(hipEventElapsedTime(&rows, e[0], e[1]));
+                if (r.opt.split_sweeps) {
+                    HIP_TRY(hipEventElapsedTime(&cols, e[2], end));
+                } else {  // raster: the fix-up passes between the events
+                    HIP_TRY(hipEventElapsedTime(&fix0, e[1], e[2]));
+                    HIP_TRY(hipEventElapsedTime(&cols, e[2], e[3]));
+                    HIP_TRY(hipEventElapsedTime(&fix1, e[3], end));
+                }
                 r.st.ms_rows += rows;
                 r.st.ms_cols += cols;
+                r.st.ms_seams += fix0 + fix1;
             }
         }
         r.st.ms_sweep = a;
@@ -1631,7 +1661,7 @@ extern "C" int iwpp_unstable_megapixels(const void* marker, const void* mask, si
 }
 
 extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, int conn,
-                                     int block_rows, int split_sweeps, uint8_t* flags,
+                                     int block_rows, const iwpp_options* opt, uint8_t* flags,
                                      iwpp_workspace* ws, void* stream) {
     g_err.clear();
     if (!imgs || n <= 0) return fail(IWPP_ERR_ARG, "empty batch");
@@ -1653,8 +1683,11 @@ extern "C" int iwpp_sweep_round_seed(const iwpp_image* imgs, int n, int dtype, i
     r.queue = Q_MUTEX;
     r.no_tune = true;
     iwpp_default_options(&r.opt);
+    if (opt) {
+        r.opt.split_sweeps = opt->split_sweeps ? 1 : 0;
+        r.opt.seam_rows = opt->seam_rows;
+    }
     r.opt.col_block_rows = block_rows;
-    r.opt.split_sweeps = split_sweeps ? 1 : 0;
     HIP_TRY(call_stream(ws, stream, &r.s));
     rc = setup_images(r, imgs, n);
     if (rc) return rc;

@@ -98,6 +98,10 @@ struct Descs {
     int n;
     int total;           // work items of the launch (kernel specific)
     int round = 0;       // sweep round of the launch (0-based)
+    // Raster fix-up launches (COL_RASTER, COL_RASTER_SEED): > 0 = each wave
+    // passes only over the `seam` rows after (downward) or before (upward)
+    // the boundary below its block (at most R / 2 rows); 0 = whole blocks.
+    int seam = 0;
     ImgDesc one;
 };
 
@@ -158,7 +162,11 @@ hipError_t launch_row_pass_batch(const Descs& ds, int dtype, bool clip, bool wid
 // the block or strip read from J), COL_RASTER in place, COL_RASTER_CLIP in
 // place clipping the marker to the mask first (pixels read from outside the
 // block or strip are clipped as they are read), COL_RASTER_SEED in place with
-// the seeding flags and border columns of COL_IN_PLACE_SEED.
+// the seeding flags and border columns of COL_IN_PLACE_SEED.  With ds.seam >
+// 0, COL_RASTER and COL_RASTER_SEED make a fix-up pass over the rows beside
+// each block boundary, whose first row then reads the other block's final
+// row; the seeding form also tests the row beyond its window's far end
+// against the window's last row (it may add flags, never clears one).
 enum ColMode : int {
     COL_J_TO_P = 0,
     COL_P_TO_J = 1,

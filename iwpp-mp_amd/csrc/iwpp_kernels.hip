@@ -1141,13 +1141,17 @@ __global__ __launch_bounds__(kWG) void k_row_pass_wave(Descs ds, uint32_t* chang
 template <class T, int VB, int CONN, int DIR, bool INPLACE, bool SEED = false,
           bool RASTER = false, bool CLIP = false>
 DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, size_t ps,
-                   size_t pd, size_t pi, int W, int H, int R, int strip, int seg,
+                   size_t pd, size_t pi, int W, int H, int y0, int y1, int strip,
                    const TileMask& tm, uint32_t* changed, uint8_t* flags = nullptr,
-                   int mp_base = 0, int nmx = 0, uint32_t* edges = nullptr) {
+                   int mp_base = 0, int nmx = 0, uint32_t* edges = nullptr,
+                   bool far_end = false) {
     using V = VecIO<T, VB>;
     using L = LanePx<T, VB>;
     constexpr int NW = V::NW, NP = L::NP;
     static_assert(!CLIP || RASTER, "clipping passes are raster passes");
+    // in-place raster passes that do not clip follow another pass of the
+    // round (or an earlier round), whose scans left their rows closed
+    constexpr bool SKIP_CLOSED = RASTER && INPLACE && !CLIP;
     const int lane = lane_id();
     const int rowbytes = W * (int)sizeof(T);
     const int xb0 = strip * 64 * VB;  // first byte of the strip
@@ -1155,7 +1159,6 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
     const int o = xb0 + lane * VB;
     const int xs0 = xb0 / (int)sizeof(T);             // first pixel of the strip
     const int xs1 = (xb0 + 64 * VB) / (int)sizeof(T); // first pixel right of the strip
-    const int y0 = seg * R, y1 = min(H, y0 + R);
     // With a tile schedule, the lane's tile is read once per tile row the
     // block spans (a block of at most tm.th rows spans two; taller ones read
     // it again at each row, the slow way): a per-row read of the table in the
@@ -1352,27 +1355,56 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             prev[k] = out;
         }
         if constexpr (RASTER) {
-            // the row across the strip, forward then backward, entering
-            // from the pixels beside the strip
-            L px;
+            // SKIP_CLOSED: the scans change nothing in a row where no single
+            // step can raise a pixel (every pixel at least min(its mask, each
+            // neighbour along the row), the pixels beside the strip counted
+            // as neighbours): the scans are such steps, in sequence.  Rows
+            // that a previous pass left closed mostly stay closed after the
+            // column step (two in three of the upward pass's rows at 32768^2
+            // after the first downward one), so the wave tests that first
+            // (about a fifth of the scans' instructions) and scans only when
+            // some lane finds a pixel that can rise.
+            bool open = true;
+            if constexpr (SKIP_CLOSED) {
+                uint32_t left, right;
+                if constexpr (L::PAIRS) {
+                    left = from_left(prev[NP - 1] >> 16, lc) | (prev[NP - 1] << 16);
+                    right = (prev[0] >> 16) | (from_right(prev[0] & 0xFFFFu, rc) << 16);
+                } else {
+                    left = from_left(prev[NP - 1], lc);
+                    right = from_right(prev[0], rc);
+                }
+                bool rise = false;
 #pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                px.v[k] = prev[k];
-                px.c[k] = c[k];
+                for (int k = 0; k < NP; ++k) {
+                    const uint32_t a = k > 0 ? prev[k - 1] : left;
+                    const uint32_t b = k < NP - 1 ? prev[k + 1] : right;
+                    rise |= L::vmax(prev[k], L::vmin(L::vmax(a, b), c[k])) != prev[k];
+                }
+                open = wave_any(rise);
             }
-            const uint32_t qm = px.mask_min();
-            uint32_t Pi, Qi;
-            px.template reduce_masked<1>(Pi, Qi, qm);
-            scan_lanes_fwd(Pi, Qi);
-            px.template apply<1, false>(max(from_left(Pi, 0u), min(lc, from_left(Qi, 0xFFFFFFFFu))));
-            px.template reduce_masked<-1>(Pi, Qi, qm);
-            scan_lanes_bwd(Pi, Qi, lane);
-            px.template apply<-1, false>(max(from_right(Pi, 0u), min(rc, from_right(Qi, 0xFFFFFFFFu))));
+            if (open) {
+                // the row across the strip, forward then backward, entering
+                // from the pixels beside the strip
+                L px;
 #pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                prev[k] = px.v[k];
-                d |= prev[k] != v0[k];
+                for (int k = 0; k < NP; ++k) {
+                    px.v[k] = prev[k];
+                    px.c[k] = c[k];
+                }
+                const uint32_t qm = px.mask_min();
+                uint32_t Pi, Qi;
+                px.template reduce_masked<1>(Pi, Qi, qm);
+                scan_lanes_fwd(Pi, Qi);
+                px.template apply<1, false>(max(from_left(Pi, 0u), min(lc, from_left(Qi, 0xFFFFFFFFu))));
+                px.template reduce_masked<-1>(Pi, Qi, qm);
+                scan_lanes_bwd(Pi, Qi, lane);
+                px.template apply<-1, false>(max(from_right(Pi, 0u), min(rc, from_right(Qi, 0xFFFFFFFFu))));
+#pragma unroll
+                for (int k = 0; k < NP; ++k) prev[k] = px.v[k];
             }
+#pragma unroll
+            for (int k = 0; k < NP; ++k) d |= prev[k] != v0[k];
         }
         any_change |= d;
         if (!INPLACE || d) {  // in place: only vectors that rose are written
@@ -1396,7 +1428,8 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             // r1 against row y (the row before y in the block: i > 0).  A
             // raster row passes the first test by construction: its column
             // step took r1's final pixels inside the strip, and its scans
-            // leave it stable along the strip.
+            // (or the closure test that found them idle) leave it stable
+            // along the strip.
             uint32_t h[NP];
             if constexpr (CONN == 8 || !RASTER) nbmax(prev, h, false);
             bool uy = false, ur = false;
@@ -1441,6 +1474,29 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
     if (tm.active) run_block(std::true_type{});
     else run_block(std::false_type{});
     if constexpr (SEED) {
+        const int yt = DIR > 0 ? y1 : y0 - 1;  // the row beyond the last one computed
+        if (far_end && n > 0 && yt >= 0 && yt < H) {
+            // a fix-up window's far end: row yt (not changed by this pass,
+            // final) and the window's last row r1, both ways
+            uint32_t w[NW], wm[NW], t[NP], ct[NP], hr[NP], ht[NP];
+            V::load(S + (size_t)yt * ps, o, rowbytes, w);
+            V::load(I + (size_t)yt * pi, o, rowbytes, wm);
+            L::split(w, t);
+            L::split(wm, ct);
+            if constexpr (CONN == 8) {
+                nbmax(r1, hr, true);
+                nbmax(t, ht, true);
+            }
+            bool ut = false, ur = false;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const uint32_t fr = CONN == 8 ? hr[k] : r1[k], ft = CONN == 8 ? ht[k] : t[k];
+                ut |= L::vmax(L::vmin(fr, ct[k]), t[k]) != t[k];
+                ur |= L::vmax(L::vmin(ft, c1[k]), r1[k]) != r1[k];
+            }
+            flag_band(DIR > 0 ? y1 - 1 : y0, ur);
+            flag_band(yt, ut);
+        }
         if (n > 0) {
             flag_band(band >= 0 ? -64 : 0, false);  // flush
             edge_flush();
@@ -1468,27 +1524,42 @@ __global__ __launch_bounds__(kWG) void k_col_pass(Descs ds, uint32_t* changed, u
     if (ds.round >= d.rounds) return;  // this image's sweep rounds are done
     const int local = w - d.cw_base;
     const int strip = local % d.strips, seg = local / d.strips;
+    int y0 = seg * d.R, y1 = min(d.H, y0 + d.R);
+    if constexpr (MODE == 5 || MODE == 7) {
+        if (ds.seam > 0) {  // fix-up window after the block's top boundary (down) or before its bottom one
+            const int k = min(ds.seam, d.R / 2);
+            if (DIR > 0) {
+                if (seg == 0) return;
+                y1 = min(d.H, y0 + k);
+            } else {
+                if (y1 >= d.H) return;
+                y0 = y1 - k;
+            }
+        }
+    }
+    uint8_t* const J = d.J;
     if constexpr (MODE == 0)
-        col_block<T, VB, CONN, DIR, false>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+        col_block<T, VB, CONN, DIR, false>(J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, y0, y1, strip, d.tm, changed);
     else if constexpr (MODE == 1)
-        col_block<T, VB, CONN, DIR, false>(d.P, d.J, d.I, d.pp, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+        col_block<T, VB, CONN, DIR, false>(d.P, J, d.I, d.pp, d.pj, d.pi, d.W, d.H, y0, y1, strip, d.tm, changed);
     else if constexpr (MODE == 2)
-        col_block<T, VB, CONN, DIR, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm, changed);
+        col_block<T, VB, CONN, DIR, true>(J, J, d.I, d.pj, d.pj, d.pi, d.W, d.H, y0, y1, strip, d.tm, changed);
     else if constexpr (MODE == 3)  // in place, with the seeding flags
-        col_block<T, VB, CONN, DIR, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg, d.tm,
+        col_block<T, VB, CONN, DIR, true, true>(J, J, d.I, d.pj, d.pj, d.pi, d.W, d.H, y0, y1, strip, d.tm,
                                                changed, flags, d.mp_base, d.nmx, edges + d.edge_base);
     else if constexpr (MODE == 4)  // raster, read J write P
-        col_block<T, VB, CONN, DIR, false, false, true>(d.J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, d.R, strip,
-                                                       seg, d.tm, changed);
+        col_block<T, VB, CONN, DIR, false, false, true>(J, d.P, d.I, d.pj, d.pp, d.pi, d.W, d.H, y0, y1,
+                                                       strip, d.tm, changed);
     else if constexpr (MODE == 5)  // raster, in place
-        col_block<T, VB, CONN, DIR, true, false, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip,
-                                                      seg, d.tm, changed);
+        col_block<T, VB, CONN, DIR, true, false, true>(J, J, d.I, d.pj, d.pj, d.pi, d.W, d.H, y0, y1, strip,
+                                                      d.tm, changed);
     else if constexpr (MODE == 6)  // raster, in place, clipping first
-        col_block<T, VB, CONN, DIR, true, false, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R,
-                                                            strip, seg, d.tm, changed);
+        col_block<T, VB, CONN, DIR, true, false, true, true>(J, J, d.I, d.pj, d.pj, d.pi, d.W, d.H, y0, y1,
+                                                            strip, d.tm, changed);
     else  // raster, in place, with the seeding flags
-        col_block<T, VB, CONN, DIR, true, true, true>(d.J, d.J, d.I, d.pj, d.pj, d.pi, d.W, d.H, d.R, strip, seg,
-                                                     d.tm, changed, flags, d.mp_base, d.nmx, edges + d.edge_base);
+        col_block<T, VB, CONN, DIR, true, true, true>(J, J, d.I, d.pj, d.pj, d.pi, d.W, d.H, y0, y1, strip,
+                                                     d.tm, changed, flags, d.mp_base, d.nmx,
+                                                     edges + d.edge_base, ds.seam > 0);
 }
 
 // ------------------------------------------------- megapixel (64 x 64) tile --

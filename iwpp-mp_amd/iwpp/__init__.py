@@ -44,7 +44,8 @@ class Options(ctypes.Structure):
                 ("max_local_iters", ctypes.c_int), ("queue_capacity", ctypes.c_int),
                 ("workers_per_cu", ctypes.c_int), ("timeout_s", ctypes.c_double),
                 ("trace_phases", ctypes.c_int), ("check_queue", ctypes.c_int),
-                ("learned_min_pixels", ctypes.c_longlong), ("split_sweeps", ctypes.c_int)]
+                ("learned_min_pixels", ctypes.c_longlong), ("split_sweeps", ctypes.c_int),
+                ("seam_rows", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -60,7 +61,8 @@ class Stats(ctypes.Structure):
                 ("us_notify", ctypes.c_double), ("ms_rows", ctypes.c_float),
                 ("ms_cols", ctypes.c_float), ("repeat_visits", ctypes.c_longlong),
                 ("state_left", ctypes.c_longlong), ("col_block_rows", ctypes.c_int),
-                ("seed_class", ctypes.c_int)]
+                ("seed_class", ctypes.c_int), ("ms_seams", ctypes.c_float),
+                ("seam_rows", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -96,7 +98,7 @@ SIGNATURES = {
     "iwpp_mlp_backward": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, ctypes.c_float, ctypes.c_float,
                                _VP, _VP, _VP]),
     "iwpp_mlp_adam": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.c_float, _I, _VP]),
-    "iwpp_sweep_round_seed": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP]),
+    "iwpp_sweep_round_seed": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _VP]),
     "iwpp_reconstruct_noalloc": (_I, [_VP, _VP, _SZ, _SZ, _I, _I, _I, _I,
                                       ctypes.POINTER(Options), _VP]),
     "iwpp_last_error": (ctypes.c_char_p, []),
@@ -534,20 +536,22 @@ def unstable_megapixels(marker, mask, connectivity, stream=None):
     return flags.cpu().numpy(), int(count.item())
 
 
-def sweep_round_seed(pairs, connectivity=8, block_rows=0, split_sweeps=False, workspace=None,
-                     stream=None):
+def sweep_round_seed(pairs, connectivity=8, block_rows=0, split_sweeps=False, seam_rows=-1,
+                     workspace=None, stream=None):
     """One in-place sweep round over a batch of (marker, mask) pairs whose
-    last pass (upward raster pass, or with split_sweeps the upward column
-    pass) sets the megapixel seeding flags; returns one uint8 numpy flag grid
-    (ceil(h/64), ceil(w/64)) per image."""
+    last pass (upward raster pass and its fix-up, or with split_sweeps the
+    upward column pass) sets the megapixel seeding flags; returns one uint8
+    numpy flag grid (ceil(h/64), ceil(w/64)) per image.  seam_rows: rows of
+    the raster fix-up passes (0 none, negative the library's choice)."""
     torch = _torch()
     arr, dtype = _batch_array(pairs)
     grids = [((mk.h + MP_SIZE - 1) // MP_SIZE, (mk.w + MP_SIZE - 1) // MP_SIZE) for mk, _ in pairs]
     total = sum(a * b for a, b in grids)
     flags = torch.full((total,), 0xEE, dtype=torch.uint8, device=pairs[0][0].buf.device)
+    opt = default_options(split_sweeps=1 if split_sweeps else 0, seam_rows=seam_rows)
     _raise(lib().iwpp_sweep_round_seed(arr, len(pairs), dtype, connectivity, block_rows,
-                                       1 if split_sweeps else 0, flags.data_ptr(), workspace.handle if workspace else None,
-                                       _stream(stream)))
+                                       ctypes.byref(opt), flags.data_ptr(),
+                                       workspace.handle if workspace else None, _stream(stream)))
     torch.cuda.synchronize()
     host = flags.cpu().numpy()
     out, off = [], 0

@@ -215,6 +215,28 @@ def test_reconstruct_split_sweeps(torch_cuda, queue, conn, dtype):
                 assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn)), (h, w, kind, st)
 
 
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_reconstruct_seam_rows(torch_cuda, conn, dtype):
+    # raster fix-up windows of every size up to (and past) half a block,
+    # one and two sweep rounds, and the sizes where a window meets the image
+    # bottom (a last block shorter than the window) or there is one block
+    rng = np.random.default_rng(zlib.crc32(f"seams/{conn}/{dtype.__name__}".encode()))
+    size = np.dtype(dtype).itemsize
+    for h, w in ((70, 300), (200, 333), (97, 4100 // size), (330, 2100 // size)):
+        for kind in ("random", "seeds", "hdome"):
+            J, I = rand_pair(rng, h, w, dtype, kind)
+            want = oracle.reconstruct(J, I, conn)
+            for seam in (0, 1, 7, 16, 40):
+                for sweeps in (1, 2):
+                    say("seam rows", conn, dtype.__name__, h, w, kind, seam, sweeps)
+                    dj, di = up(J), up(I)
+                    st = iwpp.reconstruct(dj, di, conn, 0, "mutex", "plain",
+                                          iwpp.default_options(sweep_rounds=sweeps, seam_rows=seam,
+                                                               col_block_rows=32))
+                    assert np.array_equal(dj.numpy(), want), (h, w, kind, seam, sweeps, st)
+
+
 @pytest.mark.parametrize("queue", VARIANTS)
 def test_golden_jpeg_crops(torch_cuda, queue):
     import os
@@ -342,31 +364,57 @@ def _seed_batches(dtype):
     ]
 
 
+def _fixup_bands(h, block_rows, seam_rows):
+    """Megapixel rows that an upward fix-up pass can leave flagged though
+    stable: those holding a row of a window [yb - k, yb) before a block
+    boundary yb, or the row just above it (k = min(seam_rows, R / 2))."""
+    k = min(seam_rows, block_rows // 2)
+    bands = set()
+    for yb in range(block_rows, h, block_rows):
+        for y in range(yb - k - 1, yb):
+            bands.add(y // 64)
+    return sorted(bands)
+
+
+# seam: rows of the raster fix-up passes (0 = none; the fused flags are then
+# exactly the stability test's), or None for the library's choice.
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("conn", [4, 8])
-@pytest.mark.parametrize("split", [False, True])
-def test_fused_seeding_flags(torch_cuda, dtype, conn, split):
+@pytest.mark.parametrize("split,seam", [(False, 0), (False, None), (False, 24), (True, 0)])
+def test_fused_seeding_flags(torch_cuda, dtype, conn, split, seam):
     # The last column pass of a sweep phase sets the seeding flags itself
     # (tests inside its blocks and strips, then across strip edges and block
-    # boundaries).  Its flags must be exactly those of the stability test on
-    # the marker it leaves: a missing flag loses work, an extra one wastes a
-    # visit.
-    rng = np.random.default_rng(zlib.crc32(f"seedflags/{dtype.__name__}/{conn}/{split}".encode()))
+    # boundaries).  Without fix-up passes its flags must be exactly those of
+    # the stability test on the marker it leaves: a missing flag loses work,
+    # an extra one wastes a visit.  The upward fix-up pass re-tests the rows
+    # it changes and adds flags, so a flag set before it for a megapixel
+    # that it then made stable stays: every unstable megapixel must still be
+    # flagged, and extras may lie only beside a fix-up window.
+    rng = np.random.default_rng(zlib.crc32(f"seedflags/{dtype.__name__}/{conn}/{split}/{seam}".encode()))
+    seam_rows = -1 if seam is None else seam
     for name, shapes, pitch_kind in _seed_batches(dtype):
         for block_rows in (0, 32, 96):
             for kind in ("seeds", "hdome", "random"):
-                say("fused seeding", dtype.__name__, conn, split, name, block_rows, kind)
+                say("fused seeding", dtype.__name__, conn, split, seam, name, block_rows, kind)
                 cases = [rand_pair(rng, h, w, dtype, kind) for h, w in shapes]
                 pairs = []
                 for (J, I), (h, w) in zip(cases, shapes):
                     pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
                     pairs.append((up(J, pitch), up(I, pitch)))
-                flags = iwpp.sweep_round_seed(pairs, conn, block_rows, split)
+                flags = iwpp.sweep_round_seed(pairs, conn, block_rows, split, seam_rows)
                 for (J, I), (dj, _), f in zip(cases, pairs, flags):
                     after = dj.numpy()
                     want = oracle.unstable_blocks(after, I, conn)
-                    assert np.array_equal(f, want), (name, J.shape, block_rows, kind,
-                                                     int(f.sum()), int(want.sum()))
+                    info = (name, J.shape, block_rows, kind, int(f.sum()), int(want.sum()))
+                    if split or seam == 0:
+                        assert np.array_equal(f, want), info
+                    else:
+                        assert set(np.unique(f)) <= {0, 1}, info
+                        assert (f >= want).all(), info  # no unstable megapixel left out
+                        if block_rows and seam is not None:
+                            extra = np.nonzero((f > want).any(axis=1))[0]
+                            allowed = _fixup_bands(J.shape[0], block_rows, seam)
+                            assert set(extra.tolist()) <= set(allowed), (info, extra, allowed)
                     # the round only raised pixels towards the reconstruction
                     assert (after >= oracle.clip(J, I)).all()
                     assert (after <= oracle.reconstruct(J, I, conn)).all()
