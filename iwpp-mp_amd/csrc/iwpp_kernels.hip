@@ -1665,9 +1665,10 @@ DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
 // is a multiple of 4 pixels, and pitches and bases of 4 bytes): 18 lanes per
 // row, 4 pixels each (one 32-bit word for uint8, two for uint16), 3 rows per
 // load instruction, so the halo columns come with the rows (one cache line or
-// two per row) instead of one line per halo pixel.  All loads of a batch are
-// issued before any is used, from clamped (always valid) addresses; uint16
-// loads in two batches (the registers of one would halve the waves per SIMD).
+// two per row) instead of one line per halo pixel.  All loads are issued
+// before any is used, from clamped (always valid) addresses, in one batch
+// for both pixel types (uint16 tiles hold the kernel at two waves per SIMD by
+// their LDS, so the 88 registers of a uint16 batch cost no occupancy).
 // A lane's 4 pixels go to LDS as 2 (uint8) or 4 (uint16) words built with
 // byte permutes, masked to 0 where they lie outside the image; rows outside
 // the image are written as 0 (a separate path: megapixels on the image's top
@@ -1675,7 +1676,7 @@ DEV uint32_t ldp(const uint8_t* base, size_t pitch, int x, int y) {
 template <class T>
 DEV void tile_fetch(const Image& im, int mx, int my, lds8* s) {
     constexpr int WPL = (int)sizeof(T);                 // 32-bit words per lane and plane
-    constexpr int RPI = 3, NIT = 22, NB = (int)sizeof(T), BIT = NIT / NB;
+    constexpr int RPI = 3, NIT = 22, NB = 1, BIT = NIT / NB;
     constexpr int RB = TileFmt<T>::RB;
     const int lane = lane_id();
     const int x0 = mx * kMP, y0 = my * kMP;
