@@ -283,7 +283,8 @@ def main():
     bytes_sweep = 3 * tsize * px  # algorithmic bytes of one directional pass
     phases = {k: statistics.mean(s[k] for s in stats) for k in ("ms_sweep", "ms_seed", "ms_queue")}
     counters = {k: stats[-1][k] for k in ("tile_visits", "seeded", "sweep_rounds", "queue_rounds",
-                                          "restarts", "queue", "tile")}
+                                          "restarts", "queue", "tile", "col_block_rows",
+                                          "seam_rows")}
     used_queue = {0: "mutex", 1: "dq", 2: "uq"}[stats[-1]["queue"]]
     rounds = counters["sweep_rounds"]
     visits = statistics.mean(s["tile_visits"] for s in stats)
@@ -311,9 +312,12 @@ def main():
         # profile rows are shared with the passes of those modes
         seam_launch = seam_bytes([case.marker.shape], tsize, stats[-1]["col_block_rows"],
                                  stats[-1]["seam_rows"])
+        # (with one round the downward pass is the clipping one, mode 6 alone;
+        # the upward fix-up of the last round shares mode 7 with the upward pass)
+        down_pat = "k_col_pass<*, 1, 6>" if rounds == 1 else "k_col_pass<*, 1, [56]>"
         sweeps = {
             "k_col_pass raster, downward (in place, clipping in round 0; one launch per round)":
-                (ms_rows, rounds, bytes_sweep, ("k_col_pass<*, 1, [56]>",)),
+                (ms_rows, rounds, bytes_sweep, (down_pat,)),
             "k_col_pass raster fix-ups beside the block boundaries (down and up: two launches "
             "per round)":
                 (ms_seams, 2 * rounds if stats[-1]["seam_rows"] > 0 else 0, seam_launch,
@@ -566,7 +570,8 @@ def bench_batch(args):
     tsize = cases[0].mask.itemsize
     candidates = {
         "k_col_pass raster, downward (in place, clipping in round 0; one launch per round)":
-            (batch_st["ms_rows"], rounds, 3 * tsize * px, ("k_col_pass<*, 1, [56]>",)),
+            (batch_st["ms_rows"], rounds, 3 * tsize * px,
+             ("k_col_pass<*, 1, 6>" if rounds == 1 else "k_col_pass<*, 1, [56]>",)),
         "k_col_pass raster, upward (in place, seeding flags in the last round; one launch "
         "per round)":
             (batch_st["ms_cols"], rounds, 3 * tsize * px, ("k_col_pass<*, -1, [57]>",)),
