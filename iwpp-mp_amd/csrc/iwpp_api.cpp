@@ -894,7 +894,7 @@ int learned_plan_begin(Run& r, const SchedulePlan& plan, bool* pending) {
     if (rc) return rc;
     float* pred = (float*)ws->host_plan;
     HIP_TRY(grow(ws->feat_part, ws->feat_part_n,
-                 (size_t)L.total * features_slabs(L.tile, kFeatureSub) * 16));
+                 (size_t)L.total * features_slabs(L.tile, kFeatureSub, r.dtype) * 16));
     HIP_TRY(grow(ws->feat_raw, ws->feat_raw_n, (size_t)L.total * 19));  // statistics, outputs, priorities
     if (n == 1) {
         const ImgDesc& d = r.h[0];
@@ -1520,7 +1520,7 @@ extern "C" int iwpp_tile_features(const void* marker, const void* mask, size_t p
     if (!ws) ws = default_ws();
     if (!ws) return fail(IWPP_ERR_HIP, "no HIP device");
     const size_t nt = (size_t)((w + tile - 1) / tile) * ((h + tile - 1) / tile);
-    HIP_TRY(grow(ws->feat_part, ws->feat_part_n, nt * features_slabs(tile, sub) * 16));
+    HIP_TRY(grow(ws->feat_part, ws->feat_part_n, nt * features_slabs(tile, sub, dtype) * 16));
     HIP_TRY(launch_tile_features(Image{(uint8_t*)marker, (const uint8_t*)mask, pm, pk, w, h},
                                  dtype, conn, tile, sub, ws->feat_part, raw, feat,
                                  (hipStream_t)stream));

@@ -26,7 +26,7 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 // raw: ntiles x 16 statistics, feat: ntiles x 16 normalised features
 // (tiles of `tile` pixels, raster order; every `sub`-th row sampled).
 // Slabs per tile of the feature kernel (partials: tiles * slabs * 16 floats).
-int features_slabs(int tile, int sub);
+int features_slabs(int tile, int sub, int dtype);
 hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, int sub,
                                 float* part, float* raw, float* feat, hipStream_t s);
 // The same over several images (ds: device descriptor table); image i owns

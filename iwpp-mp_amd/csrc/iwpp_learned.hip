@@ -21,11 +21,15 @@ namespace iwpp {
 //  - J(p)) over border pixels p, q the pixel across the border: every pixel
 //  of the top and bottom rows, the left and right columns on the sampled rows,
 //  14 pixels sampled, 15 max I.
-// Work split: one workgroup per (tile, slab of kSlab sampled rows); partial
+// Work split: one workgroup per (tile, slab of slab_rows(dtype) sampled rows); partial
 // statistics go to part[tile][slab][16]; a second kernel combines and
 // normalises them per image.
 // Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
-constexpr int kSlab = 4;
+// Sampled rows per workgroup: 8 for uint8 (the integer statistics leave a
+// workgroup bound by its load and reduction latency: 2048 workgroups of 4 rows
+// took 28 us at 32768^2, 1024 of 8 22 us), 4 for uint16 (float statistics:
+// 17 us against 22 with 8 at 16384^2).
+constexpr int slab_rows(int dtype) { return dtype == 1 ? 8 : 4; }
 constexpr int kRawLds = 256;  // tiles of one unit whose statistics the plan keeps in LDS
 constexpr int kPlanThreads = 512;  // workgroup of the plan's network: 8 waves, 256 VGPRs each
 constexpr int kPlanTiles = kPlanThreads / 64;  // tiles per workgroup (one image): one per wave
@@ -33,6 +37,15 @@ static_assert(kOrderMax <= kPlanThreads, "the seeding order takes one tile per t
 constexpr int kStats = 16;
 
 DEV bool is_max_stat(int k) { return k == 4 || k == 8 || k == 15; }
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+DEV u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+// Byte b of w, for b in {0, 2} (lo) or {1, 3} (hi), as the two 16-bit halves.
+DEV u16x2 bytes_lo(uint32_t w) { return as_u16x2(__builtin_amdgcn_perm(0u, w, 0x0c020c00u)); }
+DEV u16x2 bytes_hi(uint32_t w) { return as_u16x2(__builtin_amdgcn_perm(0u, w, 0x0c030c01u)); }
+// High bit of every zero byte of w (exact: the low seven bits plus 0x7F
+// carry into the high bit, never into the next byte).
+DEV uint32_t zero_bytes(uint32_t w) { return ~(((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u; }
 
 DEV float wave_reduce(float v, bool mx) {
 #pragma unroll
@@ -78,6 +91,14 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
     float acc[kStats];
 #pragma unroll
     for (int k = 0; k < kStats; ++k) acc[k] = 0.f;
+    // uint8: the statistics of 4 pixels at a time in integer registers (byte
+    // sums of absolute differences, 4-byte dot products, zero-byte counts,
+    // maxima on 16-bit halves).  A thread's sums stay below 2^24, so they are
+    // the integers that float accumulation pixel by pixel would hold.
+    uint32_t s_c = 0, s_j = 0, s_cc = 0, s_jj = 0, s_cj = 0, s_pair = 0;
+    uint32_t n_eq = 0, n_jz = 0, n_seed = 0, n_px = 0;
+    u16x2 mx_c = {0, 0}, mx_j = {0, 0}, mx_g = {0, 0};
+    constexpr int kSlab = slab_rows((int)sizeof(T));
     const int items = kSlab * nvec;
     // A thread's items (at most IPT: tile <= 2048 pixels of <= 2 bytes, 16-byte
     // vectors, 8 rows, 256 threads) are all loaded before any is used, from
@@ -122,6 +143,36 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
             iw[w] = iv[u][w] & keep;
         }
         const int xv = o / (int)sizeof(T);  // first pixel of the vector
+        if constexpr (sizeof(T) == 1) {
+            const int nv = min(NPX, x1 - xv);  // pixels of the vector inside the tile
+            n_px += (uint32_t)nv;
+#pragma unroll
+            for (int w = 0; w < VB / 4; ++w) {
+                const int vb = min(max(nv - 4 * w, 0), 4);
+                const uint32_t bm = vb >= 4 ? 0xFFFFFFFFu : (1u << (8 * vb)) - 1u;
+                const uint32_t c = iw[w] & bm, j = jw[w] & bm;
+                s_c = __builtin_amdgcn_sad_u8(c, 0u, s_c);
+                s_j = __builtin_amdgcn_sad_u8(j, 0u, s_j);
+                s_cc = __builtin_amdgcn_udot4(c, c, s_cc, false);
+                s_jj = __builtin_amdgcn_udot4(j, j, s_jj, false);
+                s_cj = __builtin_amdgcn_udot4(c, j, s_cj, false);
+                // pairs (0, 1) and (2, 3) whose second pixel is in the tile
+                const uint32_t pm = (vb >= 2 ? 0x000000FFu : 0u) | (vb >= 4 ? 0x00FF0000u : 0u);
+                s_pair = __builtin_amdgcn_sad_u8(c & pm, (c >> 8) & pm, s_pair);
+                const uint32_t hv = bm & 0x80808080u;
+                const uint32_t zx = zero_bytes(c ^ j) & hv, zj = zero_bytes(j) & hv;
+                n_eq += (uint32_t)__builtin_popcount(zx);
+                n_jz += (uint32_t)__builtin_popcount(zj);
+                n_seed += (uint32_t)__builtin_popcount(zx & ~zj);
+                const u16x2 cl = bytes_lo(c), ch = bytes_hi(c), jl = bytes_lo(j), jh = bytes_hi(j);
+                mx_c = __builtin_elementwise_max(mx_c, __builtin_elementwise_max(cl, ch));
+                mx_j = __builtin_elementwise_max(mx_j, __builtin_elementwise_max(jl, jh));
+                mx_g = __builtin_elementwise_max(
+                    mx_g, __builtin_elementwise_max(__builtin_elementwise_sub_sat(cl, jl),
+                                                    __builtin_elementwise_sub_sat(ch, jh)));
+            }
+            continue;
+        }
         float prev_i = 0.f;
 #pragma unroll
         for (int k = 0; k < NPX; ++k) {
@@ -144,6 +195,20 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
             acc[15] = fmaxf(acc[15], c);
         }
     }
+    }
+    if constexpr (sizeof(T) == 1) {
+        acc[0] = (float)s_c;
+        acc[1] = (float)s_j;
+        acc[2] = (float)((int)s_c - (int)s_j);
+        acc[3] = (float)(s_cc + s_jj - 2u * s_cj);  // sum (c - j)^2
+        acc[4] = (float)max(mx_g.x, mx_g.y);
+        acc[5] = (float)n_eq;
+        acc[6] = (float)n_jz;
+        acc[7] = (float)n_seed;
+        acc[8] = (float)max(mx_j.x, mx_j.y);
+        acc[9] = (float)s_pair;
+        acc[14] = (float)n_px;
+        acc[15] = (float)max(mx_c.x, mx_c.y);
     }
     // border inflow: rows y0 / y1-1 in the first slab, the left and right
     // columns on this slab's sampled rows (a column read touches one cache
@@ -626,13 +691,15 @@ __global__ void k_to_bf16(const float* w, uint16_t* wbf, int n) {
 }
 
 // ------------------------------------------------------------ launchers ----
-int features_slabs(int tile, int sub) { return ((tile + sub - 1) / sub + kSlab - 1) / kSlab; }
+int features_slabs(int tile, int sub, int dtype) {
+    return ((tile + sub - 1) / sub + slab_rows(dtype) - 1) / slab_rows(dtype);
+}
 
 hipError_t launch_tile_features_batch(const Descs& ds, const uint32_t* tile_base, int ntiles,
                                       bool v16, int dtype, int conn, int tile, int sub,
                                       float* part, float* raw, float* feat, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    const int nslab = features_slabs(tile, sub);
+    const int nslab = features_slabs(tile, sub, dtype);
     const int grid = ntiles * nslab;
     if (dtype == 1) {
         if (v16) hipLaunchKernelGGL((k_tile_features_part<uint8_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
@@ -667,7 +734,7 @@ hipError_t launch_tile_features(const Image& im, int dtype, int conn, int tile, 
 hipError_t launch_plan_features(const Descs& ds, const uint32_t* tile_base, int ntiles, bool v16,
                                 int dtype, int tile, int sub, float* part, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    const int nslab = features_slabs(tile, sub);
+    const int nslab = features_slabs(tile, sub, dtype);
     const int grid = ntiles * nslab;
     if (dtype == 1) {
         if (v16) hipLaunchKernelGGL((k_tile_features_part<uint8_t, 16>), grid, 256, 0, s, ds, tile_base, tile, sub, nslab, part);
@@ -690,7 +757,7 @@ hipError_t launch_plan_predict(const Descs& ds, const uint32_t* tile_base, int n
     // large for LDS, then the priorities (ntiles) shared by the workgroups
     float* prio = raw + (size_t)ntiles * 18;
     hipLaunchKernelGGL(k_plan_predict, grid, kPlanThreads, 0, s, ds, tile_base, ntiles,
-                       features_slabs(tile, sub), tile, sub, dtype, conn, part, raw, params, out,
+                       features_slabs(tile, sub, dtype), tile, sub, dtype, conn, part, raw, params, out,
                        count, flag, seq, prio, order);
     return hipGetLastError();
 }
