@@ -659,10 +659,8 @@ int queue_launches(Run& r) {
     iwpp_workspace* ws = r.ws;
     const int kind = r.queue;
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[1], r.s));
-    HIP_TRY(launch_fill_u32(ws->ctl, 0u, (size_t)C_NSLOTS * kCtlStride, r.s));
-    HIP_TRY(launch_fill_u32(ws->st, 0u, (size_t)r.total_mp, r.s));
-    if (r.q.chk) HIP_TRY(launch_fill_u32(r.q.chk, 0u, (size_t)r.total_mp, r.s));
-    if (kind == Q_MUTEX) HIP_TRY(launch_ring_init(ws->ring, r.cap, r.s));
+    HIP_TRY(launch_queue_reset(ws->ctl, (size_t)C_NSLOTS * kCtlStride, ws->st, (size_t)r.total_mp,
+                               r.q.chk, kind == Q_MUTEX ? ws->ring : nullptr, r.cap, r.s));
     if (!r.flags_ready)  // (after a restart the marker has changed: scan)
         HIP_TRY(launch_unstable_batch(with_total(r.ds, r.total_sw), r.dtype, r.conn, r.wide,
                                       ws->flags, nullptr, r.s));
@@ -671,7 +669,6 @@ int queue_launches(Run& r) {
                                   ws->edges, ws->flags, r.s));
     r.flags_ready = false;
     HIP_TRY(launch_seed_queue(ws->flags, r.total_mp, kind, r.q, r.s));
-    HIP_TRY(launch_prepare_queue(r.q, r.s));
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[2], r.s));
     HIP_TRY(launch_queue_phase(r.ds, r.dtype, r.conn, kind, r.q, r.opt.workers_per_cu, r.s));
     if (r.timed) HIP_TRY(hipEventRecord(ws->ev[3], r.s));

@@ -39,6 +39,7 @@ enum Ctl : int {
     C_STATE_LEFT,    // queue checks: megapixel state words not clear after the phase
     C_START,         // bsp: blocks past the start gate
     C_NOTRES,        // bsp: the grid was not co-resident (relaunch it smaller)
+    C_SEEDDONE,      // seeding: workgroups of k_seed_queue finished
     C_NSLOTS
 };
 
@@ -197,10 +198,14 @@ int col_strips(int W, int dtype, bool wide);
 hipError_t launch_unstable_batch(const Descs& ds, int dtype, int conn, bool wide,
                                  uint8_t* flags, uint32_t* count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
-hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s);
+// Compacts the flags into the queue and sets its starting counters (after
+// launch_queue_reset).
 hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const QueueArgs& q,
                              hipStream_t s);
-hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s);
+// Clears ctl[0..nctl), st[0..nmp) and chk (when not null), and sets ring slot
+// i to expect ticket i for i < cap (when ring is not null): one launch.
+hipError_t launch_queue_reset(uint32_t* ctl, size_t nctl, uint32_t* st, size_t nmp, uint32_t* chk,
+                              unsigned long long* ring, size_t cap, hipStream_t s);
 // Autotuner key: counts[0] += pixels with min(marker, mask) > 0, counts[1] +=
 // pixels sampled, over every 64th row of every image (counts zeroed before).
 hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,

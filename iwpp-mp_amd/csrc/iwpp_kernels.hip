@@ -2416,10 +2416,34 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
-__global__ void k_ring_init(unsigned long long* ring, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        ring[i] = (unsigned long long)i << 32;  // slot i expects push ticket i
+// Before a queue phase, in one launch: the control block and the megapixel
+// state words (and the check words, when on) cleared, and every slot of the
+// mutex ring set to expect its push ticket (ring == nullptr: DQ / UQ).  Short
+// launches in a row are spaced by the host's launch rate, not by their work.
+__global__ void k_queue_reset(uint32_t* ctl, size_t nctl, uint32_t* st, size_t nmp, uint32_t* chk,
+                              unsigned long long* ring, size_t cap) {
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = i0; i < nctl; i += step) ctl[i] = 0u;
+    for (size_t i = i0; i < nmp; i += step) st[i] = 0u;
+    if (chk)
+        for (size_t i = i0; i < nmp; i += step) chk[i] = 0u;
+    if (ring)
+        for (size_t i = i0; i < cap; i += step) ring[i] = (unsigned long long)i << 32;
+}
+
+// The queue's starting counters, once every seeded entry is reserved: the
+// read-queue size / pending count / mutex tickets from the tail, clamped to
+// the capacity (an overflow is flagged and the host re-seeds).
+DEV void prepare_queue(const QueueArgs& q) {
+    const uint32_t n = a_load(ctl_at(q.ctl, C_TAIL));
+    const uint32_t k = n < q.cap ? n : q.cap;
+    a_store(ctl_at(q.ctl, C_RSIZE), k);
+    a_store(ctl_at(q.ctl, C_PENDING), k);
+    if (n > q.cap) a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
+    a_store(ctl_at(q.ctl, C_TAIL), k);
+    a_store(ctl_at(q.ctl, C_SEEDED), k);  // mutex: seeded tickets are claimed apart,
+    a_store(ctl_at(q.ctl, C_HEAD), k);    // single pops start after them
 }
 
 // Compacts the unstable flags into queue A and marks each queued megapixel
@@ -2429,7 +2453,10 @@ __global__ void k_ring_init(unsigned long long* ring, size_t n) {
 // A wave takes K consecutive 64-flag chunks (ballot + popcount per chunk) and
 // reserves their queue entries with one atomicAdd on the tail: atomics on one
 // address serialise, one per chunk cost 40 us at 262144 megapixels.  Entries
-// keep the chunks' order inside a wave.
+// keep the chunks' order inside a wave.  The last workgroup to finish sets the
+// queue's starting counters (prepare_queue): each workgroup counts itself
+// done (C_SEEDDONE, cleared by k_queue_reset) after its tail reservations have
+// returned, so the last one's read of the tail follows all of them.
 template <int K>
 __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nmp, int kind,
                                                      QueueArgs q) {
@@ -2482,6 +2509,8 @@ __global__ __launch_bounds__(kWG) void k_seed_queue(const uint8_t* flags, int nm
             start += (uint32_t)__popcll(b[k]);
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && a_add(ctl_at(q.ctl, C_SEEDDONE), 1u) == gridDim.x - 1u) prepare_queue(q);
 }
 
 // ------------------------------------------------------------ phase timers ---
@@ -3135,18 +3164,6 @@ void k_queue_bsp(Descs ds, QueueArgs q) {
 
 // After seeding: read-queue size / pending count from the seeding count;
 // an over-full seeding raises the overflow flag.
-__global__ void k_prepare_queue(QueueArgs q) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint32_t n = a_load(ctl_at(q.ctl, C_TAIL));
-        const uint32_t k = n < q.cap ? n : q.cap;
-        a_store(ctl_at(q.ctl, C_RSIZE), k);
-        a_store(ctl_at(q.ctl, C_PENDING), k);
-        if (n > q.cap) a_store(ctl_at(q.ctl, C_OVERFLOW), 1u);
-        a_store(ctl_at(q.ctl, C_TAIL), k);
-        a_store(ctl_at(q.ctl, C_SEEDED), k);  // mutex: seeded tickets are claimed apart,
-        a_store(ctl_at(q.ctl, C_HEAD), k);    // single pops start after them
-    }
-}
 
 // Queue checks: how many megapixel state words are not clear.
 __global__ __launch_bounds__(kWG) void k_state_left(QueueArgs q) {
@@ -3449,13 +3466,6 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_ring_init(unsigned long long* ring, size_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const int grid = (int)std::min<size_t>((n + kWG - 1) / kWG, (size_t)cu_count() * 8);
-    hipLaunchKernelGGL(k_ring_init, grid, kWG, 0, s, ring, n);
-    return hipGetLastError();
-}
-
 hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,
                                hipStream_t s) {
     const dim3 grid((unsigned)ds.n, 16);
@@ -3470,8 +3480,12 @@ hipError_t launch_state_left(const QueueArgs& q, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_prepare_queue(const QueueArgs& q, hipStream_t s) {
-    hipLaunchKernelGGL(k_prepare_queue, 1, 64, 0, s, q);
+hipError_t launch_queue_reset(uint32_t* ctl, size_t nctl, uint32_t* st, size_t nmp, uint32_t* chk,
+                              unsigned long long* ring, size_t cap, hipStream_t s) {
+    const size_t n = std::max(std::max(nctl, nmp), ring ? cap : (size_t)0);
+    const int grid = (int)std::max<size_t>(1, std::min<size_t>((n + kWG - 1) / kWG,
+                                                                (size_t)cu_count() * 8));
+    hipLaunchKernelGGL(k_queue_reset, grid, kWG, 0, s, ctl, nctl, st, nmp, chk, ring, cap);
     return hipGetLastError();
 }
 
