@@ -3173,25 +3173,51 @@ __global__ __launch_bounds__(kWG) void k_state_left(QueueArgs q) {
     if (n) a_add(ctl_at(q.ctl, C_STATE_LEFT), n);  // (none when the phase ended clean)
 }
 
+// High bit of every zero byte of w (exact: the low seven bits plus 0x7F carry
+// into the high bit, never into the next byte).
+DEV uint32_t zero_bytes(uint32_t w) { return ~(((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u; }
+
 // Seed class of the markers (autotuner key): over every 64th row of every
 // image, the pixels where min(marker, mask) > 0 and the pixels sampled
 // (counts[0], counts[1]).  Block (i, b) samples image i's rows b, b + nb, ...
+// in 32-bit words (rows start 4-byte aligned; a word past the row end lies
+// inside the pitch and its bytes beyond the row are not counted):
+// min(marker, mask) > 0 where both are non-zero, counted per byte (uint8) or
+// half-word (uint16) from their zero masks.
 template <class T>
 __global__ __launch_bounds__(kWG) void k_marker_class(Descs ds, unsigned long long* counts) {
     const ImgDesc d = desc(ds, (int)blockIdx.x);
-    unsigned long long nz = 0, n = 0;
+    const int rowbytes = d.W * (int)sizeof(T), nwords = (rowbytes + 3) / 4;
+    uint32_t nz = 0, n = 0;
     for (int r = (int)blockIdx.y; r * 64 < d.H; r += (int)gridDim.y) {
         const int y = r * 64;
-        const T* J = reinterpret_cast<const T*>(d.J + (size_t)y * d.pj);
-        const T* I = reinterpret_cast<const T*>(d.I + (size_t)y * d.pi);
-        for (int x = (int)threadIdx.x; x < d.W; x += kWG) {
-            nz += min(*gp(J + x), *gp(I + x)) > 0;
-            ++n;
+        const uint32_t* J = reinterpret_cast<const uint32_t*>(d.J + (size_t)y * d.pj);
+        const uint32_t* I = reinterpret_cast<const uint32_t*>(d.I + (size_t)y * d.pi);
+        for (int w = (int)threadIdx.x; w < nwords; w += kWG) {
+            const int vb = min(4, rowbytes - 4 * w);  // bytes of the word inside the row
+            const uint32_t keep = vb >= 4 ? 0xFFFFFFFFu : (1u << (8 * vb)) - 1u;
+            const uint32_t j = *gp(J + w) & keep, i = *gp(I + w) & keep;
+            uint32_t both;
+            if constexpr (sizeof(T) == 1) {
+                // high bit of each byte that is non-zero in both
+                both = ~(zero_bytes(j) | zero_bytes(i)) & keep & 0x80808080u;
+            } else {
+                both = ((j & 0xFFFFu) && (i & 0xFFFFu) ? 1u : 0u) |
+                       ((j >> 16) && (i >> 16) ? 0x10000u : 0u);
+            }
+            nz += (uint32_t)__popc(both);
+            n += (uint32_t)vb / (uint32_t)sizeof(T);
         }
     }
-    if (n) {
-        __hip_atomic_fetch_add(gp(counts), nz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(gp(counts + 1), n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int m = 32; m >= 1; m >>= 1) {  // wave sums, then one atomic per wave
+        nz += __shfl_xor(nz, m, 64);
+        n += __shfl_xor(n, m, 64);
+    }
+    if (lane_id() == 0 && n) {
+        __hip_atomic_fetch_add(gp(counts), (unsigned long long)nz, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gp(counts + 1), (unsigned long long)n, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -3468,7 +3494,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
 
 hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,
                                hipStream_t s) {
-    const dim3 grid((unsigned)ds.n, 16);
+    // about four workgroups per CU in all; one image: at most one per sampled row
+    const int per_image = std::max(1, (cu_count() * 4 + ds.n - 1) / ds.n);
+    const int rows = ds.d == nullptr ? (ds.one.H + 63) / 64 : 64;
+    const dim3 grid((unsigned)ds.n, (unsigned)std::max(1, std::min(per_image, rows)));
     if (dtype == 1) hipLaunchKernelGGL((k_marker_class<uint8_t>), grid, kWG, 0, s, ds, counts);
     else hipLaunchKernelGGL((k_marker_class<uint16_t>), grid, kWG, 0, s, ds, counts);
     return hipGetLastError();

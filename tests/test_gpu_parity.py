@@ -502,6 +502,53 @@ def test_autotune_restores_markers_and_tunes(torch_cuda):
     iwpp.lib().iwpp_autotune_clear()
 
 
+def _class_case(rng, h, w, dtype, frac):
+    # mask non-zero everywhere; on every 64th row (what the class samples) a
+    # marker non-zero on exactly round(frac * sampled) pixels, elsewhere
+    # non-zero on every pixel (rows the class must not count)
+    top = 255 if dtype == np.uint8 else 4095
+    mask = rng.integers(1, top + 1, (h, w)).astype(dtype)
+    marker = mask.copy()
+    rows = np.arange(0, h, 64)
+    sampled = np.zeros((len(rows), w), dtype)
+    k = int(round(frac * sampled.size))
+    sampled.flat[rng.choice(sampled.size, k, replace=False)] = 1
+    marker[rows] = np.where(sampled > 0, mask[rows], 0)
+    return marker, mask
+
+
+def _numpy_class(cases):
+    nz = sum(int((np.minimum(J[::64], I[::64]) > 0).sum()) for J, I in cases)
+    n = sum(J[::64].size for J, _ in cases)
+    return 0 if nz * 100 < n else (1 if nz * 2 < n else 2)
+
+
+def test_seed_class_counts(torch_cuda):
+    # the autotuner's seed class (sparse < 1% of sampled pixels with
+    # min(marker, mask) > 0, partial < 50%, dense) on fractions either side of
+    # both thresholds, rows whose width ends inside a 32-bit word, both pixel
+    # types, an odd pitch and a batch; an 'auto' call measures it once some
+    # tuning is cached
+    rng = np.random.default_rng(15)
+    iwpp.autotune([(up(J), up(I)) for J, I in [rand_pair(rng, 70, 90, np.uint8, "seeds")]], 8, 1)
+    try:
+        for dtype, (h, w), pitch_kind in ((np.uint8, (130, 257), None), (np.uint16, (129, 1003), None),
+                                          (np.uint8, (200, 333), "odd")):
+            for frac in (0.009, 0.011, 0.49, 0.51):
+                J, I = _class_case(rng, h, w, dtype, frac)
+                pitch = odd_pitch(w, dtype) if pitch_kind == "odd" else None
+                say("seed class", dtype.__name__, h, w, pitch_kind, frac)
+                st = iwpp.reconstruct(up(J, pitch), up(I, pitch), 8, 0, "auto")
+                assert st["seed_class"] == _numpy_class([(J, I)]), (dtype, h, w, frac, st)
+        for frac in (0.009, 0.011, 0.49, 0.51):
+            cases = [_class_case(rng, h, w, np.uint8, frac) for h, w in ((130, 257), (65, 70), (300, 41))]
+            say("seed class batch", frac)
+            st = iwpp.reconstruct_batch([(up(J), up(I)) for J, I in cases], 8, 0, "auto")
+            assert st["seed_class"] == _numpy_class(cases), (frac, st)
+    finally:
+        iwpp.lib().iwpp_autotune_clear()
+
+
 def test_autotune_keys_on_seed_class(torch_cuda):
     # two workloads of the same sizes, one of sparse seeds and one of h-dome
     # markers, get separate cache entries; each later 'auto' call finds its own
