@@ -427,28 +427,19 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         return IWPP_OK;
     }
     // device tables: descriptors, and work item -> image for megapixels,
-    // rows, column-block waves and seeding waves (one allocation)
+    // rows, column-block waves and seeding waves (one allocation), the
+    // tables filled on the device from the descriptors' bases and counts
+    // (from the host they were megabytes of pageable copies per call)
     const size_t ntab = (size_t)mp + rows + cw + sw;
-    std::vector<uint32_t> tab(ntab);
-    uint32_t* t_mp = tab.data();
-    uint32_t* t_row = t_mp + mp;
-    uint32_t* t_cw = t_row + rows;
-    uint32_t* t_sw = t_cw + cw;
-    for (int i = 0; i < n; ++i) {
-        const ImgDesc& d = r.h[i];
-        std::fill(t_mp + d.mp_base, t_mp + d.mp_base + (size_t)d.nmx * d.nmy, (uint32_t)i);
-        std::fill(t_row + d.row_base, t_row + d.row_base + d.H, (uint32_t)i);
-        std::fill(t_cw + d.cw_base, t_cw + d.cw_base + (size_t)d.strips * ((d.H + d.R - 1) / d.R),
-                  (uint32_t)i);
-        std::fill(t_sw + d.sw_base, t_sw + d.sw_base + (size_t)d.strips * d.nmy, (uint32_t)i);
-    }
     if (!r.capturing) {  // a capture reuses the tables its dry run uploaded
         HIP_TRY(grow(ws->descs, ws->descs_n, (size_t)n));
         HIP_TRY(grow(ws->img_of_mp, ws->img_of_mp_n, ntab));
         HIP_TRY(hipMemcpyAsync(ws->descs, r.h.data(), sizeof(ImgDesc) * n,
                                hipMemcpyHostToDevice, r.s));
-        HIP_TRY(hipMemcpyAsync(ws->img_of_mp, tab.data(), 4 * ntab, hipMemcpyHostToDevice, r.s));
-        // the host vectors die with this call: make the copies complete first
+        HIP_TRY(launch_image_tables(ws->descs, n, ws->img_of_mp, ws->img_of_mp + mp,
+                                    ws->img_of_mp + mp + rows, ws->img_of_mp + mp + rows + cw,
+                                    r.s));
+        // the host descriptors die with this call: make the copy complete first
         HIP_TRY(hipStreamSynchronize(r.s));
     }
     r.ds.d = ws->descs;

@@ -198,6 +198,11 @@ int col_strips(int W, int dtype, bool wide);
 hipError_t launch_unstable_batch(const Descs& ds, int dtype, int conn, bool wide,
                                  uint8_t* flags, uint32_t* count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+// Work item -> image tables of a batch from its n device descriptors: image
+// i's megapixels, rows, column-block waves (strips x blocks of R rows) and
+// seeding waves (strips x megapixel rows), from its *_base on, get i.
+hipError_t launch_image_tables(const ImgDesc* d, int n, uint32_t* img_of_mp, uint32_t* img_of_row,
+                               uint32_t* img_of_cw, uint32_t* img_of_sw, hipStream_t s);
 // Compacts the flags into the queue and sets its starting counters (after
 // launch_queue_reset).
 hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const QueueArgs& q,

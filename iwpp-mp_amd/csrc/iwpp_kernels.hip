@@ -3173,6 +3173,22 @@ __global__ __launch_bounds__(kWG) void k_state_left(QueueArgs q) {
     if (n) a_add(ctl_at(q.ctl, C_STATE_LEFT), n);  // (none when the phase ended clean)
 }
 
+// Block (i, b) of a batch's work-item tables: image i's ranges, in strides
+// over the gridDim.y blocks of the image.
+__global__ __launch_bounds__(kWG) void k_image_tables(const ImgDesc* ds, uint32_t* img_of_mp,
+                                                      uint32_t* img_of_row, uint32_t* img_of_cw,
+                                                      uint32_t* img_of_sw) {
+    const int i = (int)blockIdx.x;
+    const ImgDesc d = ds[i];
+    const size_t t0 = (size_t)blockIdx.y * kWG + threadIdx.x, step = (size_t)gridDim.y * kWG;
+    const size_t nmp = (size_t)d.nmx * d.nmy, ncw = (size_t)d.strips * ((d.H + d.R - 1) / d.R);
+    const size_t nsw = (size_t)d.strips * d.nmy;
+    for (size_t t = t0; t < nmp; t += step) img_of_mp[d.mp_base + t] = (uint32_t)i;
+    for (size_t t = t0; t < (size_t)d.H; t += step) img_of_row[d.row_base + t] = (uint32_t)i;
+    for (size_t t = t0; t < ncw; t += step) img_of_cw[d.cw_base + t] = (uint32_t)i;
+    for (size_t t = t0; t < nsw; t += step) img_of_sw[d.sw_base + t] = (uint32_t)i;
+}
+
 // High bit of every zero byte of w (exact: the low seven bits plus 0x7F carry
 // into the high bit, never into the next byte).
 DEV uint32_t zero_bytes(uint32_t w) { return ~(((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u; }
@@ -3506,6 +3522,15 @@ hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* c
 hipError_t launch_state_left(const QueueArgs& q, hipStream_t s) {
     const int grid = std::max(1, std::min((q.nmp + kWG - 1) / kWG, cu_count() * 4));
     hipLaunchKernelGGL(k_state_left, grid, kWG, 0, s, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_image_tables(const ImgDesc* d, int n, uint32_t* img_of_mp, uint32_t* img_of_row,
+                               uint32_t* img_of_cw, uint32_t* img_of_sw, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int per_image = std::max(1, std::min(16, (cu_count() * 8 + n - 1) / n));
+    hipLaunchKernelGGL(k_image_tables, dim3((unsigned)n, (unsigned)per_image), kWG, 0, s, d,
+                       img_of_mp, img_of_row, img_of_cw, img_of_sw);
     return hipGetLastError();
 }
 
