@@ -1,15 +1,19 @@
 // Plain heuristic schedule, from measurements on MI355X with raster sweep
-// rounds (two passes per round): whole reconstructions with k = 0, 1, 2
-// uniform rounds (ms), mutex queue: 1024^2 h-dome 0.187 0.185 0.214;
-// 2048^2 h-dome 0.199 0.212 0.241; 4096^2 0.1% seeds 0.311 0.339 0.359;
-// 6144^2 seeds 0.472 0.528 0.494; 8192^2 (13 images of both generators,
-// seed densities 30-1000 ppm, blur 8-32, u8 and u16) in total 8.61 8.28 8.38
-// with the best k per image 0, 1 or 2; 12288^2 seeds 1.153 1.081 1.13;
-// 16384^2 u16 h-dome 1.785 1.013 1.329; 32768^2 seeds 6.31 2.50 3.37.  So:
-// one image whose sides are both below 8192 pixels takes no sweep round (its
-// queue phase is short: a round costs more than it saves), any other one
-// round; the longer side decides, since a long thin image would otherwise
-// leave the queue one long chain of megapixels.  A batch takes one round (its
+// rounds (two passes per round, each followed by its fix-up pass): whole
+// reconstructions with k = 0 and 1 uniform rounds (ms), mutex queue, images
+// of the four single-image generators (uint8 h-dome, 0.1% seeds, seeds with
+// disks; uint16 h-dome).  uint8: 3072^2 0.249 0.293; 4096^2 0.298 0.340;
+// 6144^2 in total 1.361 1.566; 8192^2 (three generators and four more seed
+// densities and blurs) 6.130 6.324, k = 0 best on six of nine (one tie);
+// 10240^2 2.267 2.291; 11264^2 2.714 2.421 (k = 1 best on each); 32768^2
+// seeds 5.83 2.19.  uint16 h-dome: 3072^2 0.228 0.237; 4096^2 0.442 0.367;
+// 5120^2 0.427 0.445; 6144^2 0.387 0.364; 8192^2 0.540 0.397; 10240^2 0.910
+// 0.645; 12288^2 1.162 0.731; 16384^2 1.873 0.901.  So one image takes no
+// sweep round while its longer side is below 10240 pixels (uint8) or 6144
+// (uint16) -- its queue phase is short and a round costs more than it saves
+// -- and one round from there on; the longer side decides, since a long thin
+// image would otherwise leave the queue one long chain of megapixels.  A
+// batch takes one round (its
 // single queue phase serves every image: 256 images of 256..4096 pixels a
 // side, 1 round 2.03 ms against more without).  The tile is 1024 (2048 when
 // the short side is >= 8192).
@@ -71,15 +75,15 @@ void tune_clear() {
 
 int plan_schedule(int w, int h, int nimages, int dtype, int conn, int tile, int queue,
                   int schedule, int sweep_rounds, SchedulePlan* out) {
-    (void)dtype;
     (void)conn;
     (void)schedule;
     if (!out || w <= 0 || h <= 0 || nimages <= 0) return -1;
     const int side = std::min(w, h), longer = std::max(w, h);
     out->tile = tile > 0 ? tile : (side >= 8192 ? 2048 : 1024);
     out->queue = queue == IWPP_QUEUE_AUTO ? IWPP_QUEUE_MUTEX : queue;
+    const int no_round_below = dtype == IWPP_U16 ? 6144 : 10240;
     out->sweep_rounds = sweep_rounds >= 0 ? sweep_rounds
-                                          : (nimages == 1 && longer < 8192 ? 0 : 1);
+                                          : (nimages == 1 && longer < no_round_below ? 0 : 1);
     return 0;
 }
 

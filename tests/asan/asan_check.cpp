@@ -143,12 +143,14 @@ static void synth_checks() {
 
 static void sched_checks() {
     iwpp::SchedulePlan p;
-    for (int w : {1, 100, 4096, 32768})
+    for (int w : {1, 100, 4096, 6144, 8192, 10240, 32768})
         for (int q : {IWPP_QUEUE_MUTEX, IWPP_QUEUE_DQ, IWPP_QUEUE_UQ, IWPP_QUEUE_AUTO}) {
             for (int n : {1, 7}) {
                 CHECK(iwpp::plan_schedule(w, w, n, IWPP_U8, 8, 0, q, IWPP_SCHED_PLAIN, -1, &p) == 0);
                 CHECK(p.tile % 64 == 0 && p.sweep_rounds >= 0);
-                CHECK(p.sweep_rounds == (n == 1 && w < 8192 ? 0 : 1));
+                CHECK(p.sweep_rounds == (n == 1 && w < 10240 ? 0 : 1));
+                CHECK(iwpp::plan_schedule(w, w, n, IWPP_U16, 4, 0, q, IWPP_SCHED_PLAIN, -1, &p) == 0);
+                CHECK(p.sweep_rounds == (n == 1 && w < 6144 ? 0 : 1));
             }
         }
     CHECK(iwpp::plan_schedule(1, 100000, 1, IWPP_U8, 8, 0, 0, IWPP_SCHED_PLAIN, -1, &p) == 0 &&
