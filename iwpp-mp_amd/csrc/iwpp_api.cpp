@@ -1041,8 +1041,9 @@ int measure_seed_class(Run& r) {
     if (rc) return rc;
     HIP_TRY(grow(ws->cls, ws->cls_n, 2));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(ws->cls), 0u, 4, r.s));
-    Descs ds = r.ds;
-    HIP_TRY(launch_marker_class(ds, r.dtype, ws->cls, r.s));
+    int max_h = 0;
+    for (const ImgDesc& d : r.h) max_h = std::max(max_h, d.H);
+    HIP_TRY(launch_marker_class(r.ds, r.dtype, ws->cls, max_h, r.s));
     HIP_TRY(hipMemcpyAsync(ws->host_ctl, ws->cls, 2 * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, r.s));
     HIP_TRY(hipStreamSynchronize(r.s));

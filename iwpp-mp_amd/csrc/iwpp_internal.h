@@ -212,8 +212,9 @@ hipError_t launch_seed_queue(const uint8_t* flags, int nmp, int kind, const Queu
 hipError_t launch_queue_reset(uint32_t* ctl, size_t nctl, uint32_t* st, size_t nmp, uint32_t* chk,
                               unsigned long long* ring, size_t cap, hipStream_t s);
 // Autotuner key: counts[0] += pixels with min(marker, mask) > 0, counts[1] +=
-// pixels sampled, over every 64th row of every image (counts zeroed before).
-hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,
+// pixels sampled, over every 64th row of every image (counts zeroed before;
+// max_h: the tallest image's height).
+hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts, int max_h,
                                hipStream_t s);
 // Counts the nonzero words of q.st[0..nmp) into C_STATE_LEFT (queue checks).
 hipError_t launch_state_left(const QueueArgs& q, hipStream_t s);

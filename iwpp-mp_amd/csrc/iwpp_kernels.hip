@@ -3195,24 +3195,36 @@ DEV uint32_t zero_bytes(uint32_t w) { return ~(((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu)
 
 // Seed class of the markers (autotuner key): over every 64th row of every
 // image, the pixels where min(marker, mask) > 0 and the pixels sampled
-// (counts[0], counts[1]).  Block (i, b) samples image i's rows b, b + nb, ...
-// in 32-bit words (rows start 4-byte aligned; a word past the row end lies
+// (counts[0], counts[1]).  Block (i, b) samples image i's rows 64 (4 b + k),
+// k < 4 (none past its last row), in 32-bit words, the four rows' loads
+// issued together (rows start 4-byte aligned; a word past the row end lies
 // inside the pitch and its bytes beyond the row are not counted):
 // min(marker, mask) > 0 where both are non-zero, counted per byte (uint8) or
-// half-word (uint16) from their zero masks.
+// half-word (uint16) from their zero masks.  One pair of atomics per block:
+// atomics on one address serialise (one pair per wave and sampled row cost
+// 0.36 ms over 256 images).
+constexpr int kClassRows = 4;
 template <class T>
 __global__ __launch_bounds__(kWG) void k_marker_class(Descs ds, unsigned long long* counts) {
     const ImgDesc d = desc(ds, (int)blockIdx.x);
     const int rowbytes = d.W * (int)sizeof(T), nwords = (rowbytes + 3) / 4;
+    const int r0 = (int)blockIdx.y * kClassRows;
+    if (r0 * 64 >= d.H) return;  // the whole workgroup
     uint32_t nz = 0, n = 0;
-    for (int r = (int)blockIdx.y; r * 64 < d.H; r += (int)gridDim.y) {
-        const int y = r * 64;
-        const uint32_t* J = reinterpret_cast<const uint32_t*>(d.J + (size_t)y * d.pj);
-        const uint32_t* I = reinterpret_cast<const uint32_t*>(d.I + (size_t)y * d.pi);
-        for (int w = (int)threadIdx.x; w < nwords; w += kWG) {
-            const int vb = min(4, rowbytes - 4 * w);  // bytes of the word inside the row
-            const uint32_t keep = vb >= 4 ? 0xFFFFFFFFu : (1u << (8 * vb)) - 1u;
-            const uint32_t j = *gp(J + w) & keep, i = *gp(I + w) & keep;
+    for (int w = (int)threadIdx.x; w < nwords; w += kWG) {
+        const int vb = min(4, rowbytes - 4 * w);  // bytes of the word inside the row
+        const uint32_t keep = vb >= 4 ? 0xFFFFFFFFu : (1u << (8 * vb)) - 1u;
+        uint32_t jv[kClassRows], iv[kClassRows];
+#pragma unroll
+        for (int k = 0; k < kClassRows; ++k) {
+            const int y = min((r0 + k) * 64, d.H - 1);  // (rows past the image not counted)
+            jv[k] = *gp(reinterpret_cast<const uint32_t*>(d.J + (size_t)y * d.pj) + w);
+            iv[k] = *gp(reinterpret_cast<const uint32_t*>(d.I + (size_t)y * d.pi) + w);
+        }
+#pragma unroll
+        for (int k = 0; k < kClassRows; ++k) {
+            if ((r0 + k) * 64 >= d.H) break;  // the same in every lane
+            const uint32_t j = jv[k] & keep, i = iv[k] & keep;
             uint32_t both;
             if constexpr (sizeof(T) == 1) {
                 // high bit of each byte that is non-zero in both
@@ -3225,15 +3237,27 @@ __global__ __launch_bounds__(kWG) void k_marker_class(Descs ds, unsigned long lo
             n += (uint32_t)vb / (uint32_t)sizeof(T);
         }
     }
-    for (int m = 32; m >= 1; m >>= 1) {  // wave sums, then one atomic per wave
+    for (int m = 32; m >= 1; m >>= 1) {  // wave sums, then the workgroup's in LDS
         nz += __shfl_xor(nz, m, 64);
         n += __shfl_xor(n, m, 64);
     }
-    if (lane_id() == 0 && n) {
-        __hip_atomic_fetch_add(gp(counts), (unsigned long long)nz, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(gp(counts + 1), (unsigned long long)n, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ uint32_t red[2][kWG / 64];
+    const int wv = (int)threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        red[0][wv] = nz;
+        red[1][wv] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int k = 0; k < kWG / 64; ++k) {
+            a += red[0][k];
+            b += red[1][k];
+        }
+        if (b) {
+            __hip_atomic_fetch_add(gp(counts), a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gp(counts + 1), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -3508,12 +3532,11 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts,
+hipError_t launch_marker_class(const Descs& ds, int dtype, unsigned long long* counts, int max_h,
                                hipStream_t s) {
-    // about four workgroups per CU in all; one image: at most one per sampled row
-    const int per_image = std::max(1, (cu_count() * 4 + ds.n - 1) / ds.n);
-    const int rows = ds.d == nullptr ? (ds.one.H + 63) / 64 : 64;
-    const dim3 grid((unsigned)ds.n, (unsigned)std::max(1, std::min(per_image, rows)));
+    // one workgroup per image and kClassRows sampled rows of the tallest image
+    const int rows = (max_h + 63) / 64;
+    const dim3 grid((unsigned)ds.n, (unsigned)std::max(1, (rows + kClassRows - 1) / kClassRows));
     if (dtype == 1) hipLaunchKernelGGL((k_marker_class<uint8_t>), grid, kWG, 0, s, ds, counts);
     else hipLaunchKernelGGL((k_marker_class<uint16_t>), grid, kWG, 0, s, ds, counts);
     return hipGetLastError();
