@@ -1360,10 +1360,11 @@ DEV void col_block(const uint8_t* S, uint8_t* D, const uint8_t* __restrict__ I, 
             // neighbour along the row), the pixels beside the strip counted
             // as neighbours): the scans are such steps, in sequence.  Rows
             // that a previous pass left closed mostly stay closed after the
-            // column step (two in three of the upward pass's rows at 32768^2
-            // after the first downward one), so the wave tests that first
-            // (about a fifth of the scans' instructions) and scans only when
-            // some lane finds a pixel that can rise.
+            // column step (about half of the upward pass's rows at 32768^2
+            // after the first downward one: its VALU instructions per wave
+            // 146K -> 115K), so the wave tests that first (about a fifth of
+            // the scans' instructions) and scans only when some lane finds a
+            // pixel that can rise.
             bool open = true;
             if constexpr (SKIP_CLOSED) {
                 uint32_t left, right;
