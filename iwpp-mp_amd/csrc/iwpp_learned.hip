@@ -27,8 +27,8 @@ namespace iwpp {
 // Rows are read as VB-byte vectors (16 when rows are 16-byte aligned).
 // Sampled rows per workgroup: 8 for uint8 (the integer statistics leave a
 // workgroup bound by its load and reduction latency: 2048 workgroups of 4 rows
-// took 28 us at 32768^2, 1024 of 8 22 us), 4 for uint16 (float statistics:
-// 17 us against 22 with 8 at 16384^2).
+// took 28 us at 32768^2, 1024 of 8 22 us), 4 for uint16 (16 us against 19.5
+// with 8 at 16384^2).
 constexpr int slab_rows(int dtype) { return dtype == 1 ? 8 : 4; }
 constexpr int kRawLds = 256;  // tiles of one unit whose statistics the plan keeps in LDS
 constexpr int kPlanThreads = 512;  // workgroup of the plan's network: 8 waves, 256 VGPRs each
@@ -73,9 +73,7 @@ template <class T, int VB>
 __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint32_t* tile_base,
                                                              int tile, int sub, int nslab,
                                                              float* part) {
-    constexpr int NPX = VB / (int)sizeof(T), PPW = 4 / (int)sizeof(T);
-    constexpr int BITS = 8 * (int)sizeof(T);
-    constexpr uint32_t PM = (uint32_t)((1u << BITS) - 1u);
+    constexpr int NPX = VB / (int)sizeof(T);
     const int gt = blockIdx.x / nslab, sl = blockIdx.x % nslab;
     const int img = image_of_tile(tile_base, ds.n, gt);
     const ImgDesc dd = ds.d ? ds.d[img] : ds.one;
@@ -91,12 +89,15 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
     float acc[kStats];
 #pragma unroll
     for (int k = 0; k < kStats; ++k) acc[k] = 0.f;
-    // uint8: the statistics of 4 pixels at a time in integer registers (byte
-    // sums of absolute differences, 4-byte dot products, zero-byte counts,
-    // maxima on 16-bit halves).  A thread's sums stay below 2^24, so they are
-    // the integers that float accumulation pixel by pixel would hold.
+    // The statistics of a 32-bit word's pixels at a time in integer registers
+    // (sums of absolute differences, dot products or 16-bit squares, zero
+    // counts, maxima on 16-bit halves): exact sums, converted to float once.
+    // uint8: a thread's sums stay below 2^24, so they are the integers float
+    // accumulation pixel by pixel would hold; uint16: the squared gaps are
+    // summed in 64 bits (float accumulation rounded them past 2^24).
     uint32_t s_c = 0, s_j = 0, s_cc = 0, s_jj = 0, s_cj = 0, s_pair = 0;
     uint32_t n_eq = 0, n_jz = 0, n_seed = 0, n_px = 0;
+    unsigned long long s_g2 = 0;
     u16x2 mx_c = {0, 0}, mx_j = {0, 0}, mx_g = {0, 0};
     constexpr int kSlab = slab_rows((int)sizeof(T));
     const int items = kSlab * nvec;
@@ -171,36 +172,40 @@ __global__ __launch_bounds__(256) void k_tile_features_part(Descs ds, const uint
                     mx_g, __builtin_elementwise_max(__builtin_elementwise_sub_sat(cl, jl),
                                                     __builtin_elementwise_sub_sat(ch, jh)));
             }
-            continue;
-        }
-        float prev_i = 0.f;
+        } else {
+            const int nv = min(NPX, x1 - xv);  // pixels of the vector inside the tile
+            n_px += (uint32_t)nv;
 #pragma unroll
-        for (int k = 0; k < NPX; ++k) {
-            if (xv + k >= x1) continue;  // past the tile (uniform per vector)
-            const float j = (float)((jw[k / PPW] >> (BITS * (k % PPW))) & PM);
-            const float c = (float)((iw[k / PPW] >> (BITS * (k % PPW))) & PM);
-            const float g = c - j;
-            acc[0] += c;
-            acc[1] += j;
-            acc[2] += g;
-            acc[3] += g * g;
-            acc[4] = fmaxf(acc[4], g);
-            acc[5] += (j == c);
-            acc[6] += (j == 0.f);
-            acc[7] += (j == c && j > 0.f);
-            acc[8] = fmaxf(acc[8], j);
-            if (k & 1) acc[9] += fabsf(c - prev_i);  // pair (x-1, x), x-1 even
-            prev_i = c;
-            acc[14] += 1.f;
-            acc[15] = fmaxf(acc[15], c);
+            for (int w = 0; w < VB / 4; ++w) {
+                const int vp = min(max(nv - 2 * w, 0), 2);  // pixels of the word in the tile
+                const uint32_t bm = vp >= 2 ? 0xFFFFFFFFu : vp == 1 ? 0xFFFFu : 0u;
+                const uint32_t c = iw[w] & bm, j = jw[w] & bm;
+                s_c = __builtin_amdgcn_sad_u16(c, 0u, s_c);
+                s_j = __builtin_amdgcn_sad_u16(j, 0u, s_j);
+                const u16x2 cp = as_u16x2(c), jp = as_u16x2(j);
+                const u16x2 d = __builtin_elementwise_max(cp, jp) - __builtin_elementwise_min(cp, jp);
+                s_g2 += (unsigned long long)((uint32_t)d.x * d.x) + (uint32_t)d.y * d.y;
+                // the pair (first, second pixel of the word) when both are in the tile
+                if (vp >= 2) s_pair += (uint32_t)max(cp.x, cp.y) - min(cp.x, cp.y);
+                const uint32_t x = c ^ j;
+                const bool e0 = vp >= 1 && (x & 0xFFFFu) == 0u, e1 = vp >= 2 && (x >> 16) == 0u;
+                const bool z0 = vp >= 1 && (j & 0xFFFFu) == 0u, z1 = vp >= 2 && (j >> 16) == 0u;
+                n_eq += (uint32_t)e0 + (uint32_t)e1;
+                n_jz += (uint32_t)z0 + (uint32_t)z1;
+                n_seed += (uint32_t)(e0 && !z0) + (uint32_t)(e1 && !z1);
+                mx_c = __builtin_elementwise_max(mx_c, cp);
+                mx_j = __builtin_elementwise_max(mx_j, jp);
+                mx_g = __builtin_elementwise_max(mx_g, __builtin_elementwise_sub_sat(cp, jp));
+            }
         }
     }
     }
-    if constexpr (sizeof(T) == 1) {
+    {
         acc[0] = (float)s_c;
         acc[1] = (float)s_j;
-        acc[2] = (float)((int)s_c - (int)s_j);
-        acc[3] = (float)(s_cc + s_jj - 2u * s_cj);  // sum (c - j)^2
+        acc[2] = (float)((long long)s_c - (long long)s_j);
+        if constexpr (sizeof(T) == 1) acc[3] = (float)(s_cc + s_jj - 2u * s_cj);  // sum (c - j)^2
+        else acc[3] = (float)s_g2;
         acc[4] = (float)max(mx_g.x, mx_g.y);
         acc[5] = (float)n_eq;
         acc[6] = (float)n_jz;
