@@ -383,7 +383,12 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
     // shorter ones give more waves.  Measured on MI355X, best: 512 rows at
     // 32768^2 u8 (2048 waves; queue phase 0.93 -> 0.53 ms against 128 rows),
     // 256 at 16384^2 u16 (2048 waves), 64 at 4096^2 u8 (1024 waves), 32 at
-    // 1024^2: about 2048 waves, and at most 64 blocks down a column.
+    // 1024^2: about 2048 waves, and at most 64 blocks down a column.  A batch
+    // takes at most 256: its images' heights are not multiples of the block
+    // height, so its waves' walks differ in length, and the pass lasts as
+    // long as its longest walks; more, shorter blocks balance them.  Measured
+    // on the 256-image batch (sides 256..4096, 404 Mpixel; 4-byte lanes):
+    // 512 rows (3.3K waves) 1.95 ms, 256 (6.7K) 1.88, 128 (13K) 1.99.
     r.seam = r.opt.split_sweeps ? 0 : r.opt.seam_rows < 0 ? kSeamRows : r.opt.seam_rows;
     int R = r.opt.col_block_rows;
     if (R <= 0) {
@@ -391,7 +396,7 @@ int setup_images(Run& r, const iwpp_image* imgs, int n) {
         for (int i = 0; i < n; ++i) maxh = std::max(maxh, r.h[i].H);
         long rr = std::max<long>(strip_rows / 2048, maxh / 64);
         rr = (rr + 16) / 32 * 32;
-        R = (int)std::min<long>(512, std::max<long>(32, rr));
+        R = (int)std::min<long>(n > 1 ? 256 : 512, std::max<long>(32, rr));
     }
     long rows = 0, cw = 0, sw = 0, mp = 0, se = 0;
     long long edges = 0;
