@@ -485,7 +485,7 @@ def bench_batch(args):
     pool0, mk0 = _pool_images([c.marker for c in cases], torch, iwpp)
     poolw, mkw = _pool_images([c.marker for c in cases], torch, iwpp)
     _, msk = _pool_images([c.mask for c in cases], torch, iwpp)
-    pairs = list(zip(mkw, msk))
+    pairs = iwpp.Batch(zip(mkw, msk))  # descriptor array built once for every call below
     ws = iwpp.Workspace()
 
     def reset():

@@ -325,13 +325,40 @@ class ImageDesc(ctypes.Structure):
                 ("width", ctypes.c_int), ("height", ctypes.c_int)]
 
 
-def _batch_array(pairs):
+def _build_array(pairs):
     arr = (ImageDesc * len(pairs))()
     dtype = pairs[0][0].dtype
     for k, (mk, ms) in enumerate(pairs):
         assert mk.h == ms.h and mk.w == ms.w and mk.dtype == ms.dtype == dtype
         arr[k] = ImageDesc(mk.ptr, ms.ptr, mk.pitch, ms.pitch, mk.w, mk.h)
     return arr, dtype
+
+
+class Batch:
+    """(marker, mask) DeviceImage pairs with their iwpp_image array built once.
+    Every batch entry point takes one in place of the list of pairs; a caller
+    that reconstructs the same buffers again and again saves rebuilding the
+    array (about 1 us of host time per pair: 0.2-0.25 ms for 256 images).  The
+    images must keep the buffers they had when the Batch was built."""
+
+    def __init__(self, pairs):
+        self.pairs = list(pairs)
+        self.arr, self.dtype = _build_array(self.pairs)
+
+    def __len__(self):
+        return len(self.pairs)
+
+    def __iter__(self):
+        return iter(self.pairs)
+
+    def __getitem__(self, k):
+        return self.pairs[k]
+
+
+def _batch_array(pairs):
+    if isinstance(pairs, Batch):
+        return pairs.arr, pairs.dtype
+    return _build_array(pairs)
 
 
 def reconstruct_batch(pairs, connectivity=8, tile=0, queue="auto", schedule="plain",

@@ -50,3 +50,20 @@ def test_default_options():
     o = iwpp.default_options()
     assert o.sweep_rounds == -1 and o.col_block_rows == 0 and o.max_local_iters == 8
     assert o.timeout_s == 60.0
+
+
+class _FakeImage:  # what the batch array reads from a DeviceImage (no device memory)
+    def __init__(self, h, w, dtype, pitch, ptr):
+        self.h, self.w, self.dtype, self.pitch, self.ptr = h, w, dtype, pitch, ptr
+
+
+def test_batch_array_prebuilt_matches_list():
+    pairs = [(_FakeImage(10 + k, 30 + 3 * k, 2, 256 * (k + 1), 0x10000 * (k + 1)),
+              _FakeImage(10 + k, 30 + 3 * k, 2, 512, 0x900000 + 0x1000 * k)) for k in range(7)]
+    arr, dtype = iwpp._batch_array(pairs)
+    b = iwpp.Batch(pairs)
+    arr_b, dtype_b = iwpp._batch_array(b)
+    assert dtype == dtype_b == 2 and len(b) == 7 and b[3] is pairs[3] and list(b) == pairs
+    assert bytes(arr) == bytes(arr_b)
+    assert [(d.marker, d.mask, d.pitch_marker, d.pitch_mask, d.width, d.height) for d in arr_b] == \
+        [(m.ptr, s.ptr, m.pitch, s.pitch, m.w, m.h) for m, s in pairs]

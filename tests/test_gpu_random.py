@@ -75,6 +75,7 @@ def test_random_batch(torch_cuda, block):
             cases.append(rand_pair(rng, h, w, dtype, ["random", "seeds", "hdome"][rng.integers(3)]))
         say("random batch", block, t, dtype.__name__, [J.shape for J, _ in cases], conn, queue, opt)
         pairs = [(iwpp.DeviceImage.from_numpy(J), iwpp.DeviceImage.from_numpy(I)) for J, I in cases]
-        st = iwpp.reconstruct_batch(pairs, conn, 0, queue, "plain", iwpp.default_options(**opt))
+        batch = iwpp.Batch(pairs) if t % 2 else pairs  # prebuilt descriptor array, or the list
+        st = iwpp.reconstruct_batch(batch, conn, 0, queue, "plain", iwpp.default_options(**opt))
         for (J, I), (dj, _) in zip(cases, pairs):
             assert np.array_equal(dj.numpy(), oracle.reconstruct(J, I, conn)), (t, J.shape, st)
